@@ -1,0 +1,332 @@
+"""aarbf -- Python host side of the MI355X attribute-aware RBF renderer.
+
+A thin layer over the C ABI in include/aarbf.h (libaarbf.so).  It mirrors
+the reference's host program (hostCode.cpp): pre-process particles
+(space-time AABB, Hilbert sort, attribute normalisation), generate leaf
+bounds, build the BVH, render with ParticleRBFRayGen, resolve TAA.  Device
+memory is held in torch tensors on the HIP device; all compute runs in the
+library's HIP kernels -- there is no host fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (BoundsConstants, Float3, Int2, PushConstants, RayGenData, RenderStats,
+                   TAAConstants)
+
+MAX_DEPTH = 10000  # sharedCode.h:27
+
+__all__ = [
+    "AarbfError", "Context", "Scene", "Camera", "camera_basis", "PushConstants", "RayGenData",
+    "BoundsConstants", "TAAConstants", "RenderStats", "MAX_DEPTH", "load_library",
+]
+
+
+class AarbfError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+def load_library():
+    return _lib.load()
+
+
+def _f3(v) -> Float3:
+    return Float3(float(v[0]), float(v[1]), float(v[2]))
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _torch():
+    import torch
+    return torch
+
+
+@dataclass
+class Camera:
+    """PushConstants.camera (sharedCode.h:50-55)."""
+    pos: np.ndarray
+    dir_00: np.ndarray
+    dir_du: np.ndarray
+    dir_dv: np.ndarray
+
+
+def camera_basis(look_from, look_at, look_up, cos_fovy: float, width: int, height: int) -> Camera:
+    """Camera basis of hostCode.cpp:669-682 in float32 arithmetic.
+
+    cos_fovy is the reference's image-plane scale (its 10th --camera value);
+    for a true vertical field of view f it is 2*tan(f/2).
+    """
+    f = np.float32
+    lf = np.asarray(look_from, dtype=f)
+    la = np.asarray(look_at, dtype=f)
+    lu = np.asarray(look_up, dtype=f)
+
+    def normalize(v):
+        return (v / np.sqrt(np.sum(v * v, dtype=f), dtype=f)).astype(f)
+
+    d00 = normalize(la - lf)
+    aspect = f(width) / f(height)
+    ddu = (f(cos_fovy) * aspect * normalize(np.cross(d00, lu).astype(f))).astype(f)
+    ddv = (f(cos_fovy) * normalize(np.cross(ddu, d00).astype(f))).astype(f)
+    d00 = (d00 - f(0.5) * ddu).astype(f)
+    d00 = (d00 - f(0.5) * ddv).astype(f)
+    return Camera(lf.copy(), d00, ddu, ddv)
+
+
+class Context:
+    """One aarbf_context: a device, a HIP stream and the handles made on it."""
+
+    def __init__(self, device: int = 0, stream=None):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        st = self.lib.aarbf_context_create(device, C.byref(h))
+        if st != 0:
+            raise AarbfError(st, "aarbf_context_create: " + self.lib.aarbf_status_string(st).decode())
+        self.h = h
+        self.device = device
+        if stream is not None:
+            self.set_stream(stream)
+
+    # -- plumbing ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.aarbf_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, st: int, what: str = ""):
+        if st != 0:
+            msg = self.lib.aarbf_last_error(self.h).decode() if self.h else ""
+            raise AarbfError(st, f"{what}: {self.lib.aarbf_status_string(st).decode()}: {msg}")
+
+    def set_stream(self, stream):
+        """Use a torch.cuda.Stream (or a raw hipStream_t int) for all work."""
+        ptr = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        self.check(self.lib.aarbf_set_stream(self.h, C.c_void_p(ptr)), "aarbf_set_stream")
+
+    def sync(self):
+        self.check(self.lib.aarbf_sync(self.h), "aarbf_sync")
+
+    def set_stats(self, enable: bool):
+        self.check(self.lib.aarbf_set_stats(self.h, 1 if enable else 0), "aarbf_set_stats")
+
+    def read_stats(self, reset: bool = True) -> dict:
+        s = RenderStats()
+        self.check(self.lib.aarbf_render_stats_read(self.h, C.byref(s), 1 if reset else 0), "stats")
+        return {n: int(getattr(s, n)) for n, _ in RenderStats._fields_}
+
+    # -- pre-process (hostCode.cpp:183-250) ---------------------------------
+    def spacetime_aabb(self, particles, aabb=None) -> np.ndarray:
+        a = np.array(aabb if aabb is not None else [1e38, 1e38, 1e38, -1e38, -1e38, -1e38], dtype=np.float32)
+        self.check(self.lib.aarbf_spacetime_aabb(self.h, particles.data_ptr(), particles.shape[0], _fptr(a)),
+                   "aarbf_spacetime_aabb")
+        return a
+
+    def hilbert_codes(self, particles, aabb):
+        torch = _torch()
+        a = np.asarray(aabb, dtype=np.float32)
+        codes = torch.empty(particles.shape[0], dtype=torch.int64, device=particles.device)
+        self.check(self.lib.aarbf_hilbert_codes(self.h, particles.data_ptr(), particles.shape[0], _fptr(a),
+                                                codes.data_ptr()), "aarbf_hilbert_codes")
+        return codes
+
+    def sort_particles(self, particles, codes, want_perm: bool = True):
+        torch = _torch()
+        n = particles.shape[0]
+        perm = torch.empty(n, dtype=torch.int32, device=particles.device) if want_perm else None
+        self.check(self.lib.aarbf_sort_particles(self.h, particles.data_ptr(), codes.data_ptr(), n,
+                                                 perm.data_ptr() if perm is not None else 0),
+                   "aarbf_sort_particles")
+        return perm
+
+    def attribute_range(self, particles, rng=None) -> np.ndarray:
+        r = np.array(rng if rng is not None else [1e20, -1e20], dtype=np.float32)
+        self.check(self.lib.aarbf_attribute_range(self.h, particles.data_ptr(), particles.shape[0], _fptr(r)),
+                   "aarbf_attribute_range")
+        return r
+
+    def normalize_attributes(self, particles, rng):
+        r = np.asarray(rng, dtype=np.float32)
+        self.check(self.lib.aarbf_normalize_attributes(self.h, particles.data_ptr(), particles.shape[0],
+                                                       _fptr(r)), "aarbf_normalize_attributes")
+
+    # -- tables ----------------------------------------------------------------
+    def tf(self, rgba) -> int:
+        t = np.ascontiguousarray(np.asarray(rgba, dtype=np.float32).reshape(-1, 4))
+        h = C.c_uint64()
+        self.check(self.lib.aarbf_tf_create(self.h, _fptr(t), t.shape[0], C.byref(h)), "aarbf_tf_create")
+        return h.value
+
+    def tf_rgba8(self, texels, srgb: bool = True) -> int:
+        t = np.ascontiguousarray(np.asarray(texels, dtype=np.uint8).reshape(-1, 4))
+        h = C.c_uint64()
+        self.check(self.lib.aarbf_tf_create_rgba8(self.h, t.ctypes.data_as(C.POINTER(C.c_uint8)), t.shape[0],
+                                                  1 if srgb else 0, C.byref(h)), "aarbf_tf_create_rgba8")
+        return h.value
+
+    def stbn(self, atlas) -> int:
+        a = np.ascontiguousarray(np.asarray(atlas, dtype=np.uint8))
+        assert a.ndim == 3 and a.shape[2] == 4
+        h = C.c_uint64()
+        self.check(self.lib.aarbf_stbn_create(self.h, a.ctypes.data_as(C.POINTER(C.c_uint8)), a.shape[1],
+                                              a.shape[0], C.byref(h)), "aarbf_stbn_create")
+        return h.value
+
+    # -- bounds + BVH --------------------------------------------------------------
+    def gen_bounds(self, particles, ppl: int, rbf_radius: float, radiusmap: int, radii=None, aabbs=None,
+                   num_particles: int | None = None):
+        torch = _torch()
+        n = particles.shape[0] if num_particles is None else num_particles
+        num_aabbs = (particles.shape[0] + ppl - 1) // ppl
+        if aabbs is None:
+            aabbs = torch.empty(num_aabbs * 6, dtype=torch.float32, device=particles.device)
+        bc = BoundsConstants(aabbs.data_ptr(), num_aabbs, particles.data_ptr(), n, ppl, float(rbf_radius),
+                             radiusmap, radii.data_ptr() if radii is not None else 0)
+        self.check(self.lib.aarbf_gen_bounds(self.h, C.byref(bc)), "aarbf_gen_bounds")
+        return aabbs
+
+    def bvh_build(self, aabbs, branching: int = 8) -> int:
+        h = C.c_uint64()
+        self.check(self.lib.aarbf_bvh_build(self.h, aabbs.data_ptr(), aabbs.numel() // 6, branching,
+                                            C.byref(h)), "aarbf_bvh_build")
+        return h.value
+
+    def bvh_refit(self, bvh: int, aabbs):
+        self.check(self.lib.aarbf_bvh_refit(self.h, bvh, aabbs.data_ptr()), "aarbf_bvh_refit")
+
+    def bvh_destroy(self, bvh: int):
+        self.check(self.lib.aarbf_bvh_destroy(self.h, bvh), "aarbf_bvh_destroy")
+
+    def bvh_levels(self, bvh: int) -> list[int]:
+        n = C.c_uint32()
+        counts = (C.c_uint32 * 32)()
+        self.check(self.lib.aarbf_bvh_levels(self.h, bvh, C.byref(n), counts, 32), "aarbf_bvh_levels")
+        return list(counts[: n.value])
+
+    def bvh_read_level(self, bvh: int, level: int) -> np.ndarray:
+        counts = self.bvh_levels(bvh)
+        out = np.empty((counts[level], 8), dtype=np.float32)
+        self.check(self.lib.aarbf_bvh_read_level(self.h, bvh, level, _fptr(out)), "aarbf_bvh_read_level")
+        return out
+
+    # -- render --------------------------------------------------------------------
+    def render(self, rg: RayGenData, pc: PushConstants, width: int, height: int, spp: int = 1):
+        self.check(self.lib.aarbf_render_rbf_spp(self.h, C.byref(rg), C.byref(pc), width, height, spp),
+                   "aarbf_render_rbf_spp")
+
+    def render_splat(self, rg: RayGenData, pc: PushConstants, width: int, height: int):
+        self.check(self.lib.aarbf_render_splat(self.h, C.byref(rg), C.byref(pc), width, height),
+                   "aarbf_render_splat")
+
+    def taa_resolve(self, tc: TAAConstants):
+        self.check(self.lib.aarbf_taa_resolve(self.h, C.byref(tc)), "aarbf_taa_resolve")
+
+
+@dataclass
+class Scene:
+    """Particles prepared the way hostCode.cpp prepares one timestep, held on
+    the device: Hilbert-sorted, attributes normalised, leaf bounds and BVH
+    built.  `particles` is float32 [N, 4]; `radii` optional float32 [N]
+    (permuted with the particles)."""
+    ctx: Context
+    particles: object
+    radii: object
+    aabb: np.ndarray
+    colormap: int
+    radiusmap: int
+    densitymap: int
+    rbf_radius: float
+    ppl: int = 1
+    branching: int = 8
+    aabbs: object = None
+    bvh: int = 0
+    perm: object = None
+    attr_range: np.ndarray = field(default_factory=lambda: np.array([0, 1], np.float32))
+
+    @staticmethod
+    def prepare(ctx: Context, xyzw: np.ndarray, colormap, radiusmap, densitymap, rbf_radius: float,
+                radii: np.ndarray | None = None, ppl: int = 1, branching: int = 8, normalize: bool = True,
+                device: str = "cuda"):
+        torch = _torch()
+        dev = torch.device(device, ctx.device) if device == "cuda" else torch.device(device)
+        p = torch.from_numpy(np.ascontiguousarray(xyzw, dtype=np.float32).reshape(-1, 4)).to(dev)
+        cm = colormap if isinstance(colormap, int) else ctx.tf(colormap)
+        rm = radiusmap if isinstance(radiusmap, int) else ctx.tf(radiusmap)
+        dm = densitymap if isinstance(densitymap, int) else ctx.tf(densitymap)
+        aabb = ctx.spacetime_aabb(p)
+        codes = ctx.hilbert_codes(p, aabb)
+        perm = ctx.sort_particles(p, codes)
+        r = None
+        if radii is not None:
+            r = torch.from_numpy(np.ascontiguousarray(radii, dtype=np.float32)).to(dev)
+            r = r[perm.long()].contiguous()
+        rng = ctx.attribute_range(p)
+        if normalize:
+            ctx.normalize_attributes(p, rng)
+        s = Scene(ctx, p, r, aabb, cm, rm, dm, float(rbf_radius), ppl, branching, perm=perm, attr_range=rng)
+        s.rebuild()
+        return s
+
+    def rebuild(self):
+        """Bounds + BVH build (hostCode.cpp:770-789)."""
+        self.aabbs = self.ctx.gen_bounds(self.particles, self.ppl, self.rbf_radius, self.radiusmap, self.radii)
+        if self.bvh:
+            self.ctx.bvh_destroy(self.bvh)
+        self.bvh = self.ctx.bvh_build(self.aabbs, self.branching)
+
+    def refit(self):
+        """Bounds + BVH refit (hostCode.cpp:770-779,790-794)."""
+        self.aabbs = self.ctx.gen_bounds(self.particles, self.ppl, self.rbf_radius, self.radiusmap, self.radii,
+                                         aabbs=self.aabbs)
+        self.ctx.bvh_refit(self.bvh, self.aabbs)
+
+    @property
+    def n(self) -> int:
+        return int(self.particles.shape[0])
+
+    def push_constants(self, camera: Camera, unit: float, frame_id: int = 0, accum_id: int = 1,
+                       blue_noise: bool = False, show_noise: bool = False, visualize_attributes: bool = True,
+                       light=(0.0, 0.0, 0.5)) -> PushConstants:
+        pc = PushConstants()
+        pc.accumID = accum_id & 0xFFFF
+        pc.frameID = frame_id & 0xFFFF
+        pc.enableBlueNoise = 1 if blue_noise else 0
+        pc.showNoise = 1 if show_noise else 0
+        pc.visualizeAttributes = 1 if visualize_attributes else 0
+        pc.particlesPerLeaf = self.ppl
+        pc.light.azimuth, pc.light.elevation, pc.light.ambient = (float(v) for v in light)
+        pc.camera.pos = _f3(camera.pos)
+        pc.camera.dir_00 = _f3(camera.dir_00)
+        pc.camera.dir_du = _f3(camera.dir_du)
+        pc.camera.dir_dv = _f3(camera.dir_dv)
+        pc.rbfRadius = self.rbf_radius
+        pc.unit = float(unit)
+        pc.world = self.bvh
+        pc.particles = self.particles.data_ptr()
+        pc.numParticles = self.n
+        pc.colormap, pc.radiusmap, pc.densitymap = self.colormap, self.radiusmap, self.densitymap
+        pc.radii = self.radii.data_ptr() if self.radii is not None else 0
+        return pc
+
+    def raygen_data(self, image, accum, width: int, height: int, stbn: int = 0) -> RayGenData:
+        rg = RayGenData()
+        rg.stbnTexture = stbn
+        rg.imageBuffer = image.data_ptr()
+        rg.accumBuffer = accum.data_ptr()
+        rg.fbSize = Int2(width, height)
+        rg.globalAABBMin = _f3(self.aabb[:3])
+        rg.globalAABBMax = _f3(self.aabb[3:])
+        return rg

@@ -1,0 +1,96 @@
+// internal.h -- shared host/device definitions of the aarbf library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <set>
+#include <string>
+
+#include "aarbf.h"
+
+namespace aarbf {
+
+// ---- transfer function / blue noise / BVH objects behind the handles ----
+
+struct TF {
+  float4 *table = nullptr; // device RGBA texels
+  uint32_t n = 0;
+};
+
+struct STBN {
+  uint8_t *rgba = nullptr; // device RGBA8 texels
+  int width = 0, height = 0;
+};
+
+constexpr int kMaxLevels = 32;
+
+// Hilbert-packed BVH: level 0 holds one (widened) box per leaf, node j of
+// level l+1 bounds nodes [j*K, j*K+K) of level l.  Every node is two float4
+// (lo.xyz, hi.xyz) in one allocation; level_ofs is in float4 units.
+struct BVH {
+  uint32_t K = 8, log2K = 3;
+  uint32_t num_levels = 0; // including level 0
+  uint32_t counts[kMaxLevels] = {};
+  uint64_t level_ofs[kMaxLevels] = {};
+  float4 *nodes = nullptr;
+  uint64_t total_nodes = 0;
+};
+
+// Device view of a BVH passed by value to kernels.
+struct BVHView {
+  const float4 *nodes;
+  uint32_t log2K;
+  uint32_t top; // root level
+  uint32_t counts[kMaxLevels];
+  uint32_t level_ofs[kMaxLevels]; // in float4 units (< 2^32 for <= 2^30 nodes)
+};
+
+} // namespace aarbf
+
+struct aarbf_context {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  bool stats = false;
+  uint64_t *d_stats = nullptr; // 8 counters
+  // grow-only device scratch
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // baked particle payload
+  float4 *baked = nullptr;
+  size_t baked_count = 0;
+  std::set<uint64_t> tfs, stbns, bvhs;
+};
+
+namespace aarbf {
+
+int set_error(aarbf_context *ctx, int status, const std::string &msg);
+int hip_error(aarbf_context *ctx, hipError_t e, const char *what);
+// Ensures ctx->scratch holds at least `bytes`; returns nullptr on failure.
+void *scratch(aarbf_context *ctx, size_t bytes);
+TF *get_tf(aarbf_context *ctx, uint64_t h);
+STBN *get_stbn(aarbf_context *ctx, uint64_t h);
+BVH *get_bvh(aarbf_context *ctx, uint64_t h);
+BVHView make_view(const BVH &b);
+
+} // namespace aarbf
+
+#define AARBF_HIP(ctx, call)                                                   \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) return aarbf::hip_error((ctx), e_, #call);           \
+  } while (0)
+
+#define AARBF_CHECK_LAUNCH(ctx, what)                                          \
+  do {                                                                         \
+    hipError_t e_ = hipGetLastError();                                         \
+    if (e_ != hipSuccess) return aarbf::hip_error((ctx), e_, what);            \
+  } while (0)
+
+#define AARBF_REQUIRE(ctx, cond, msg)                                          \
+  do {                                                                         \
+    if (!(cond)) return aarbf::set_error((ctx), AARBF_ERR_INVALID_ARGUMENT, (msg)); \
+  } while (0)

@@ -1,0 +1,220 @@
+"""Benchmark: Msamples/s of RBF delta tracking at 8,388,608 particles,
+1920x1080, 16 spp per frame, TAA over an orbit of frames, one MI355X.
+
+A step is one frame of the workload: one fused 16-spp launch of
+ParticleRBFRayGen (deviceCodeRBF.hlsl:206-336) with the camera at the
+frame's orbit position, followed by the TAA resolve
+(deviceCodeCommon.hlsl:67-115).  Particles, leaf bounds and the BVH are
+resident in HBM before the timed region; pre-processing is timed and
+reported separately.
+
+Prints ONE JSON line (rank 0).  Usage:
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "attribute-aware-rbfs_amd")]
+
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8, help="timed frames")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=16)
+    ap.add_argument("--orbit-frames", type=int, default=32)
+    ap.add_argument("--particles", type=int, default=8388608)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "render_traffic.json"),
+                    help="HBM bytes per render launch measured with rocprofv3 --pmc")
+    a = ap.parse_args()
+
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); the render path shards "
+              "by image tiles / requests with a full particle replica per GPU and no exchange, see DESIGN.md",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    import aarbf
+    from aarbf import scenes
+
+    if not torch.cuda.is_available():
+        log("bench.py: no HIP device")
+        return 1
+
+    t0 = time.time()
+    wl = scenes.mixture_8m(n=a.particles, width=a.width, height=a.height, spp=a.spp, frames=a.orbit_frames)
+    log(f"workload generated in {time.time() - t0:.1f} s")
+    W, H, S = wl.width, wl.height, wl.spp
+
+    ctx = aarbf.Context(0)
+    stream = torch.cuda.current_stream()
+    ctx.set_stream(stream)
+    torch.cuda.synchronize()
+    tp = time.time()
+    sc = aarbf.Scene.prepare(ctx, wl.xyzw, wl.colormap, wl.radiusmap, wl.densitymap, wl.rbf_radius,
+                             radii=wl.radii, ppl=1, branching=8)
+    torch.cuda.synchronize()
+    prep_s = time.time() - tp
+    log(f"pre-process + bounds + BVH: {prep_s * 1e3:.1f} ms (includes host->device copy)")
+
+    image = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    accum = torch.zeros_like(image)
+    taa = torch.zeros_like(image)
+    taa_prev = torch.zeros_like(image)
+    frame_buf = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+    rg = sc.raygen_data(image, accum, W, H)
+    tc = aarbf.TAAConstants(0, 0, aarbf._lib.Int2(W, H), image.data_ptr(), 0, frame_buf.data_ptr(),
+                            taa.data_ptr(), taa_prev.data_ptr())
+    cams = []
+    for f in range(wl.frames):
+        lf, la, lu = scenes.orbit_camera(f, wl.frames, 1.6)
+        cams.append(aarbf.camera_basis(lf, la, lu, wl.cos_fovy, W, H))
+
+    ev = []
+
+    def frame(f, timed_kernel=False):
+        cam = cams[f % len(cams)]
+        pc = sc.push_constants(cam, wl.unit, frame_id=(f * S) & 0xFFFF, accum_id=1)
+        if timed_kernel:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+        ctx.render(rg, pc, W, H, S)
+        if timed_kernel:
+            e1.record(stream)
+            ev.append((e0, e1))
+        ctx.taa_resolve(tc)
+
+    # instrumented launch (statistics kernel) for the algorithmic byte count
+    ctx.set_stats(True)
+    ctx.read_stats(reset=True)
+    frame(0)
+    torch.cuda.synchronize()
+    st = ctx.read_stats(reset=True)
+    ctx.set_stats(False)
+    samples = max(1, st["samples"])
+    # bytes per launch: node/leaf boxes 32 B, particle position+radius 16 B,
+    # colour payload 16 B per covering particle, accum read+write and image write 48 B per pixel
+    bytes_launch = 32 * st["nodes"] + 32 * st["leaves"] + 16 * st["particles"] + 16 * st["covering"] + 48 * W * H
+    log(f"stats per sample: steps {st['steps'] / samples:.0f} queries {st['queries'] / samples:.0f} "
+        f"nodes {st['nodes'] / samples:.0f} particles {st['particles'] / samples:.0f} "
+        f"covering {st['covering'] / samples:.1f} exhausted {st['exhausted'] / samples:.4f}")
+
+    for f in range(a.warmup):
+        frame(f)
+    torch.cuda.synchronize()
+
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    w0 = time.time()
+    t_start.record(stream)
+    for k in range(a.steps):
+        frame(a.warmup + k, timed_kernel=True)
+        if (k + 1) % 4 == 0:
+            torch.cuda.synchronize()
+            log(f"frame {k + 1}/{a.steps} done, {time.time() - w0:.1f} s")
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.time() - w0
+    gpu_ms = t_start.elapsed_time(t_end)
+    kern_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
+    total_samples = W * H * S * a.steps
+    msps = total_samples / (gpu_ms * 1e-3) / 1e6
+    achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
+
+    traffic = None
+    if a.traffic and os.path.exists(a.traffic):
+        try:
+            traffic = json.load(open(a.traffic)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(sc, wl, cams[0], a.cpu_threads)
+
+    out = {
+        "metric": "Msamples/s at 8M RBFs 1080p delta tracking; fwd+bwd ms/view; 1 MI355X",
+        "value": round(msps, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(gpu_ms / a.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 32-component Gaussian mixture, no dataset)",
+        "config": {
+            "workload": f"mixture-{wl.xyzw.shape[0]} delta tracking {W}x{H} {S} spp/frame, TAA, orbit of "
+                        f"{wl.frames} frames",
+            "particles": int(wl.xyzw.shape[0]), "width": W, "height": H, "spp": S, "particles_per_leaf": 1,
+            "bvh_branching": 8, "unit": wl.unit, "max_depth": aarbf.MAX_DEPTH,
+            "exhausted_fraction": st["exhausted"] / samples, "preprocess_ms": round(prep_s * 1e3, 1),
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "render_kernel", "kernel_ms": round(kern_ms, 3),
+            "algorithmic_bytes_per_launch": int(bytes_launch),
+            "algorithmic_bytes_per_sample": round(bytes_launch / samples, 1),
+        },
+        "cpu_baseline": cpu,
+        "wall_s": round(wall, 3),
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+def cpu_baseline(sc, wl, cam, threads):
+    """The oracle (scalar C restatement of the reference shaders, std::thread
+    -style pthreads over pixels) on frame 0 at 1 spp on a fixed 1/64 pixel
+    subsample (every 8th pixel in x and y)."""
+    import oracle
+    p = sc.particles.cpu().numpy()
+    r = sc.radii.cpu().numpy() if sc.radii is not None else None
+    W, H = wl.width, wl.height
+    threads = threads or min(16, os.cpu_count() or 1)
+    R = oracle.Render(p, r, sc.aabb, wl.colormap, wl.radiusmap, wl.densitymap, cam, W, H, wl.unit, wl.rbf_radius,
+                      frame_id=0, accum_id=1)
+    t0 = time.time()
+    prep = R.prepared()
+    setup = time.time() - t0
+    ys, xs = np.mgrid[0:H:8, 0:W:8]
+    pix = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.int32)
+    t0 = time.time()
+    prep.render(pix, spp=1, threads=threads)
+    dt = time.time() - t0
+    log(f"cpu baseline: {len(pix)} samples in {dt:.2f} s on {threads} threads (scene set-up {setup:.1f} s)")
+    return {"value": round(len(pix) / dt / 1e6, 6), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"frame 0, 1 spp, every 8th pixel in x and y ({len(pix)} samples of the {W}x{H} frame); "
+                      f"oracle/oracle.c -O2 -ffp-contract=off, {threads} threads, grid set-up {setup:.1f} s untimed"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,264 @@
+/*
+ * aarbf.h -- C ABI of the MI355X attribute-aware RBF renderer.
+ *
+ * The structs mirror the reference's shared host/device structs field by
+ * field (sharedCode.h:33-117), with GPRT buffer/texture/accel handles
+ * replaced by 64-bit values: device pointers for buffers, and handles
+ * returned by this library for transfer-function tables, the blue-noise
+ * atlas and the BVH.  Fields that have no counterpart in the reference are
+ * appended after the reference fields and marked "extension".
+ *
+ * Each entry point replaces one GPRT call sequence of hostCode.cpp; the
+ * comment above it cites the reference call it stands in for.
+ *
+ * Conventions:
+ *   - every function returns an aarbf_status (0 = OK); the message of the
+ *     last failure on a context is available from aarbf_last_error();
+ *   - no C++ exception crosses this boundary;
+ *   - device work is enqueued on the context's HIP stream (the library's
+ *     own stream, or one supplied with aarbf_set_stream) and is
+ *     asynchronous unless the function returns host data;
+ *   - a context is not thread-safe: one host thread drives it, as the
+ *     reference's single render loop does (hostCode.cpp:489-832).
+ */
+#ifndef AARBF_H
+#define AARBF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AARBF_MAX_DEPTH 10000 /* sharedCode.h:27 */
+#define AARBF_STBN_TILE 256   /* deviceCodeRBF.hlsl:245-248 */
+
+typedef enum aarbf_status {
+  AARBF_OK = 0,
+  AARBF_ERR_INVALID_ARGUMENT = 1,
+  AARBF_ERR_OUT_OF_MEMORY = 2,
+  AARBF_ERR_HIP = 3,
+  AARBF_ERR_IO = 4,
+  AARBF_ERR_FORMAT = 5,
+  AARBF_ERR_UNSUPPORTED = 6,
+  AARBF_ERR_NO_DEVICE = 7
+} aarbf_status;
+
+typedef struct aarbf_context aarbf_context;
+typedef struct aarbf_timesteps aarbf_timesteps;
+
+typedef struct aarbf_float3 {
+  float x, y, z;
+} aarbf_float3;
+
+typedef struct aarbf_int2 {
+  int32_t x, y;
+} aarbf_int2;
+
+/* sharedCode.h:33-73 (PushConstants). */
+typedef struct aarbf_push_constants {
+  uint16_t accumID;
+  uint16_t frameID;
+  uint16_t enableBlueNoise; /* 1: stochastic marching driven by the STBN atlas */
+  uint16_t showNoise;
+  uint16_t visualizeAttributes;
+  uint16_t particlesPerLeaf;
+  struct {
+    float azimuth;
+    float elevation;
+    float ambient;
+  } light;
+  struct {
+    aarbf_float3 pos;
+    aarbf_float3 dir_00;
+    aarbf_float3 dir_du;
+    aarbf_float3 dir_dv;
+  } camera;
+  float rbfRadius;
+  float unit;
+  uint64_t world;        /* BVH handle from aarbf_bvh_build */
+  uint64_t particles;    /* device float4[numParticles]: x, y, z, attribute */
+  uint32_t numParticles;
+  uint64_t colormap;     /* transfer-function handles from aarbf_tf_create* */
+  uint64_t radiusmap;
+  uint64_t densitymap;
+  /* extension: device float[numParticles] of per-particle radii; 0 = use
+   * rbfRadius for every particle (the reference's behaviour).  The radius of
+   * particle i is radii[i] * radiusmap(w_i).  rbfRadius still sets the
+   * 2*rbfRadius margin of the ray clip box (deviceCodeRBF.hlsl:218-219), so
+   * callers set it to at least max(radii[i] * radiusmap) / 2. */
+  uint64_t radii;
+} aarbf_push_constants;
+
+/* sharedCode.h:76-86 (BoundsConstants). */
+typedef struct aarbf_bounds_constants {
+  uint64_t aabbs; /* device float[6 * numAABBs]: min.xyz, max.xyz per leaf */
+  uint32_t numAABBs;
+  uint64_t particles;
+  uint32_t numParticles;
+  uint16_t particlesPerLeaf;
+  float rbfRadius;
+  uint64_t radiusmap;
+  uint64_t radii; /* extension, as in aarbf_push_constants */
+} aarbf_bounds_constants;
+
+/* sharedCode.h:88-105 (TAAConstants). */
+typedef struct aarbf_taa_constants {
+  uint16_t disableTAA;
+  uint16_t showNoise;
+  aarbf_int2 fbSize;
+  uint64_t imageTexture;  /* device float4[W*H]: this frame's render */
+  uint64_t guiTexture;    /* device uint32 RGBA8[W*H] overlay, or 0 for none */
+  uint64_t frameBuffer;   /* device uint32 BGRA8[W*H], or 0 to skip packing */
+  uint64_t taaBuffer;     /* device float4[W*H] */
+  uint64_t taaPrevBuffer; /* device float4[W*H] */
+} aarbf_taa_constants;
+
+/* sharedCode.h:108-117 (RayGenData). */
+typedef struct aarbf_raygen_data {
+  uint64_t stbnTexture; /* STBN handle from aarbf_stbn_create, or 0 */
+  uint64_t imageBuffer; /* device float4[W*H] */
+  uint64_t accumBuffer; /* device float4[W*H] */
+  aarbf_int2 fbSize;
+  aarbf_float3 globalAABBMin;
+  aarbf_float3 globalAABBMax;
+} aarbf_raygen_data;
+
+/* Counters gathered by the render kernels when statistics are enabled. */
+typedef struct aarbf_render_stats {
+  uint64_t samples;       /* ParticleRBFRayGen invocations */
+  uint64_t steps;         /* tracker iterations (primary + shadow) */
+  uint64_t queries;       /* point queries that reached the BVH root box */
+  uint64_t nodes;         /* internal BVH node boxes tested */
+  uint64_t leaves;        /* leaf boxes tested */
+  uint64_t particles;     /* particle spheres tested */
+  uint64_t covering;      /* particle spheres that covered the query point */
+  uint64_t exhausted;     /* samples whose tracker hit AARBF_MAX_DEPTH */
+} aarbf_render_stats;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Replaces gprtContextCreate(&GPU) (hostCode.cpp:280-281). */
+int aarbf_context_create(int device, aarbf_context **out);
+int aarbf_context_destroy(aarbf_context *ctx);
+/* Use an existing HIP stream (hipStream_t) for all later work; NULL
+ * restores the context's own stream. */
+int aarbf_set_stream(aarbf_context *ctx, void *hip_stream);
+void *aarbf_get_stream(aarbf_context *ctx);
+int aarbf_sync(aarbf_context *ctx);
+const char *aarbf_last_error(const aarbf_context *ctx);
+const char *aarbf_status_string(int status);
+/* Enable (1) or disable (0) render statistics; read them back (and reset
+ * them) with aarbf_render_stats_read. */
+int aarbf_set_stats(aarbf_context *ctx, int enable);
+int aarbf_render_stats_read(aarbf_context *ctx, aarbf_render_stats *out, int reset);
+/* Writes sizes and offsets of the structs above into out[0..n) and returns
+ * how many there are; bindings in other languages use it to check their
+ * struct mirrors.  No counterpart in the reference. */
+uint32_t aarbf_abi_layout(uint32_t *out, uint32_t n);
+
+/* ---- particle files (importParticles.h:8-32) ---------------------------- */
+
+/* Reads the header file at `path` (one timestep name per line) and, for each
+ * name, the binary file path + "_" + name: a little-endian uint64 count
+ * followed by count float4 (x, y, z, attribute).  Unlike the reference, a
+ * missing or truncated timestep file is an error. */
+int aarbf_import_particles(const char *path, aarbf_timesteps **out);
+uint32_t aarbf_timesteps_count(const aarbf_timesteps *ts);
+int aarbf_timesteps_get(const aarbf_timesteps *ts, uint32_t step, const float **xyzw,
+                        uint64_t *count, const char **name);
+void aarbf_timesteps_destroy(aarbf_timesteps *ts);
+/* Writes the same format (header plus one binary file per step). */
+int aarbf_export_particles(const char *path, uint32_t num_steps, const char *const *names,
+                           const float *const *xyzw, const uint64_t *counts);
+
+/* ---- pre-process (hostCode.cpp:183-250), on the device ------------------ */
+
+/* Folds the xyz bounds of n device float4 particles into aabb[6] (min.xyz,
+ * max.xyz, host memory, updated in place; start from {1e38,...,-1e38,...}
+ * as hostCode.cpp:183 does).  Synchronises. */
+int aarbf_spacetime_aabb(aarbf_context *ctx, uint64_t particles, uint64_t n, float aabb[6]);
+/* 48-bit Hilbert codes of the particle positions quantised to 16 bits per
+ * axis inside aabb (hostCode.cpp:196-209 + hilbert.h:181-225), written to
+ * device uint64[n]. */
+int aarbf_hilbert_codes(aarbf_context *ctx, uint64_t particles, uint64_t n, const float aabb[6],
+                        uint64_t codes);
+/* Sorts particles (and codes) in place by (code, x, y, z, w)
+ * (hostCode.cpp:212-221).  perm, if non-zero, receives device uint32[n]:
+ * the original index of each sorted particle. */
+int aarbf_sort_particles(aarbf_context *ctx, uint64_t particles, uint64_t codes, uint64_t n,
+                         uint64_t perm);
+/* Folds min/max of the attribute (w) into range[2] (host, in place; start
+ * from {1e20, -1e20} as hostCode.cpp:227-228).  Synchronises. */
+int aarbf_attribute_range(aarbf_context *ctx, uint64_t particles, uint64_t n, float range[2]);
+/* w <- (w - min) / (max - min) when max > min (hostCode.cpp:243-250). */
+int aarbf_normalize_attributes(aarbf_context *ctx, uint64_t particles, uint64_t n,
+                               const float range[2]);
+
+/* ---- transfer functions and blue noise ---------------------------------- */
+
+/* An RGBA float table of n texels, sampled like a linearly filtered,
+ * clamp-to-edge 1D texture (the reference's 64-texel TF textures,
+ * hostCode.cpp:355-365).  rgba is host memory, 4*n floats. */
+int aarbf_tf_create(aarbf_context *ctx, const float *rgba, uint32_t n, uint64_t *handle);
+/* The same from RGBA8 texels; srgb != 0 decodes RGB with the sRGB transfer
+ * function as an R8G8B8A8_SRGB texture does (alpha stays linear). */
+int aarbf_tf_create_rgba8(aarbf_context *ctx, const uint8_t *rgba8, uint32_t n, int srgb,
+                          uint64_t *handle);
+int aarbf_tf_destroy(aarbf_context *ctx, uint64_t handle);
+/* The blue-noise atlas: RGBA8 texels, width x height (the reference's
+ * stbn.png is an 8x16 grid of 256x256 tiles, hostCode.cpp:336-352). */
+int aarbf_stbn_create(aarbf_context *ctx, const uint8_t *rgba8, int width, int height,
+                      uint64_t *handle);
+int aarbf_stbn_destroy(aarbf_context *ctx, uint64_t handle);
+
+/* ---- bounds and BVH ----------------------------------------------------- */
+
+/* GenRBFBounds (deviceCodeBounds.hlsl:29-82), replacing
+ * gprtComputeLaunch1D(context, GenRBFBounds, ...) (hostCode.cpp:770-779). */
+int aarbf_gen_bounds(aarbf_context *ctx, const aarbf_bounds_constants *bc);
+
+/* Builds a BVH over numAABBs leaf boxes (device float[6*numAABBs], the
+ * layout written by aarbf_gen_bounds; NaN boxes are inactive).  Replaces
+ * gprtAccelBuild(BLAS) + gprtAccelBuild(TLAS) + gprtAccelGetHandle
+ * (hostCode.cpp:468-469,781-789).  `branching` is the node width (2..16; 0
+ * picks the default).  The handle goes into aarbf_push_constants.world. */
+int aarbf_bvh_build(aarbf_context *ctx, uint64_t aabbs, uint32_t numAABBs, uint32_t branching,
+                    uint64_t *handle);
+/* Recomputes node boxes for new leaf boxes with the same topology
+ * (gprtAccelUpdate, hostCode.cpp:790-794). */
+int aarbf_bvh_refit(aarbf_context *ctx, uint64_t handle, uint64_t aabbs);
+int aarbf_bvh_destroy(aarbf_context *ctx, uint64_t handle);
+/* Topology queries for tests: number of levels (leaves are level 0) and
+ * the node count of each; node boxes of one level copied to host float[8*n]
+ * (lo.xyz, pad, hi.xyz, pad). */
+int aarbf_bvh_levels(aarbf_context *ctx, uint64_t handle, uint32_t *num_levels,
+                     uint32_t *counts, uint32_t max_levels);
+int aarbf_bvh_read_level(aarbf_context *ctx, uint64_t handle, uint32_t level, float *boxes);
+
+/* ---- rendering ---------------------------------------------------------- */
+
+/* One launch of ParticleRBFRayGen (deviceCodeRBF.hlsl:206-336) over a W x H
+ * image: replaces gprtRayGenSetParameters + gprtRayGenLaunch2D
+ * (hostCode.cpp:370-378,813).  Delta tracking, or stochastic marching when
+ * pc->enableBlueNoise is set (then rg->stbnTexture is required). */
+int aarbf_render_rbf(aarbf_context *ctx, const aarbf_raygen_data *rg,
+                     const aarbf_push_constants *pc, int width, int height);
+/* spp consecutive launches fused into one: launch s uses frameID + s and
+ * accumID + s, exactly as spp calls of aarbf_render_rbf with the counters
+ * incremented between them (hostCode.cpp:826-827). */
+int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg,
+                         const aarbf_push_constants *pc, int width, int height, uint32_t spp);
+/* ParticleSplatRayGen (deviceCodeSplat.hlsl:58-154). */
+int aarbf_render_splat(aarbf_context *ctx, const aarbf_raygen_data *rg,
+                       const aarbf_push_constants *pc, int width, int height);
+/* CompositeGui's TAA resolve and BGRA8 packing (deviceCodeCommon.hlsl:67-115)
+ * plus the taa -> taaPrev history copy (hostCode.cpp:819-830). */
+int aarbf_taa_resolve(aarbf_context *ctx, const aarbf_taa_constants *tc);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AARBF_H */

@@ -1,0 +1,213 @@
+/*
+ * aarbf_detmath.h -- the renderer's portable, deterministic FP32 math.
+ *
+ * Stochastic tracking makes a discrete decision at every step (the
+ * free-flight distance t -= ln(1-xi)*unit, the accept test xi < density,
+ * the coverage test |P-x| < r).  One ULP of difference in a logarithm
+ * between two implementations is enough to flip such a decision and send
+ * the whole path elsewhere, so "same pixel for the same RNG stream" is
+ * only possible if host and device evaluate the transcendental functions
+ * with identical arithmetic.  The reference leaves these ULPs to the
+ * Vulkan driver (deviceCodeRBF.hlsl:75,153,111,132; sharedCode.h:139), so
+ * this header *defines* them for this renderer:
+ *
+ *   - only IEEE-754 binary32 operations that are correctly rounded on both
+ *     x86-64 (SSE) and gfx950 (VALU): +, -, *, /, sqrt, fma, floor, rint,
+ *     min/max and bit manipulation;
+ *   - every multiply-add that should be fused is written as an explicit
+ *     fmaf, and everything else must be compiled with -ffp-contract=off;
+ *   - polynomial coefficients are this project's own least-squares fits
+ *     (Chebyshev nodes, 100-bit reference values), rounded to binary32.
+ *
+ * The header is part of the C ABI contract: the HIP kernels and any host
+ * restatement of the renderer include it unchanged.  Accuracy against a
+ * correctly rounded libm is <= 2 ulp for aa_logf/aa_expf on the ranges the
+ * renderer uses (checked by the test suite).
+ */
+#ifndef AARBF_DETMATH_H
+#define AARBF_DETMATH_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#define AA_FN static inline __host__ __device__
+#define AA_FMAF(a, b, c) __builtin_fmaf((a), (b), (c))
+#define AA_SQRTF(a) __builtin_sqrtf(a)
+#define AA_FLOORF(a) __builtin_floorf(a)
+#define AA_RINTF(a) __builtin_rintf(a)
+#define AA_FMINF(a, b) __builtin_fminf((a), (b))
+#define AA_FMAXF(a, b) __builtin_fmaxf((a), (b))
+#else
+#include <math.h>
+#define AA_FN static inline
+#define AA_FMAF(a, b, c) fmaf((a), (b), (c))
+#define AA_SQRTF(a) sqrtf(a)
+#define AA_FLOORF(a) floorf(a)
+#define AA_RINTF(a) rintf(a)
+#define AA_FMINF(a, b) fminf((a), (b))
+#define AA_FMAXF(a, b) fmaxf((a), (b))
+#endif
+
+AA_FN uint32_t aa_f2u(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  return u;
+}
+AA_FN float aa_u2f(uint32_t u) {
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+#define AA_INF_F (aa_u2f(0x7f800000u))
+#define AA_NAN_F (aa_u2f(0x7fc00000u))
+
+/* ln(x).  x is split as m * 2^e with m in [sqrt(1/2), sqrt(2)); with
+ * f = m - 1 (exact by Sterbenz), ln(1+f) = f + f^2 * Q(f), Q a degree-8
+ * fit of (ln(1+f) - f) / f^2 on [-0.2929, 0.4143].  ln2 is split into a
+ * 16-bit head and a binary32 tail so e*ln2_hi is exact. */
+AA_FN float aa_logf(float x) {
+  uint32_t u = aa_f2u(x);
+  if (u == 0u || u == 0x80000000u) return -AA_INF_F;
+  if (u >> 31) return AA_NAN_F;             /* x < 0 */
+  if (u >= 0x7f800000u) return x;           /* +inf, NaN */
+  int e = 0;
+  if (u < 0x00800000u) {                    /* subnormal: scale by 2^25 */
+    u = aa_f2u(x * 33554432.0f);
+    e = -25;
+  }
+  e += (int)(u >> 23) - 127;
+  uint32_t mb = (u & 0x007fffffu) | 0x3f800000u;
+  if (mb > 0x3fb504f3u) { /* m > sqrt(2) */
+    mb -= 0x00800000u;
+    e += 1;
+  }
+  float f = aa_u2f(mb) - 1.0f;
+  float q = -0.07866667956113815f;
+  q = AA_FMAF(q, f, 0.12821750342845917f);
+  q = AA_FMAF(q, f, -0.131016343832016f);
+  q = AA_FMAF(q, f, 0.1418900042772293f);
+  q = AA_FMAF(q, f, -0.1662759929895401f);
+  q = AA_FMAF(q, f, 0.2000196874141693f);
+  q = AA_FMAF(q, f, -0.2500077188014984f);
+  q = AA_FMAF(q, f, 0.33333322405815125f);
+  q = AA_FMAF(q, f, -0.4999999701976776f);
+  float f2 = f * f;
+  float ef = (float)e;
+  float lo = AA_FMAF(ef, 1.4286068203094172e-06f, f2 * q);
+  float r = lo + f;
+  return AA_FMAF(ef, 0.693145751953125f, r);
+}
+
+/* e^x.  k = rint(x / ln2), r = x - k*ln2 (two-part, fused), e^r = 1 + r +
+ * r^2 * P(r) with P a degree-5 fit on |r| <= ln2/2, then scaled by 2^k
+ * through the exponent field (two factors when 2^k is outside the normal
+ * range). */
+AA_FN float aa_expf(float x) {
+  if (x != x) return x;
+  if (x > 88.72283935546875f) return AA_INF_F;
+  if (x < -103.97208404541016f) return 0.0f;
+  float k = AA_RINTF(x * 1.4426950216293335f);
+  float r = AA_FMAF(k, -0.693145751953125f, x);
+  r = AA_FMAF(k, -1.4286068203094172e-06f, r);
+  float p = 0.00019899285689461976f;
+  p = AA_FMAF(p, r, 0.0013933650916442275f);
+  p = AA_FMAF(p, r, 0.0083332983776927f);
+  p = AA_FMAF(p, r, 0.04166646674275398f);
+  p = AA_FMAF(p, r, 0.1666666716337204f);
+  p = AA_FMAF(p, r, 0.5f);
+  float y = AA_FMAF(r * r, p, r) + 1.0f;
+  int ki = (int)k;
+  if (ki > 127) { /* x just below the overflow threshold */
+    y *= 2.0f;
+    ki -= 1;
+  }
+  if (ki < -126) { /* gradual underflow: scale in two steps */
+    y *= aa_u2f((uint32_t)(ki + 126 + 127) << 23);
+    return y * aa_u2f(0x00800000u); /* 2^-126 */
+  }
+  return y * aa_u2f((uint32_t)(ki + 127) << 23);
+}
+
+/* x^y for x >= 0 (the renderer only raises colours and alphas, which are
+ * non-negative).  x == 0 gives 0 for y > 0, x == 1 gives 1; otherwise
+ * e^(y ln x).  Negative or NaN x gives NaN. */
+AA_FN float aa_powf(float x, float y) {
+  if (x != x || y != y) return AA_NAN_F;
+  if (x < 0.0f) return AA_NAN_F;
+  if (x == 1.0f || y == 0.0f) return 1.0f;
+  if (x == 0.0f) return (y > 0.0f) ? 0.0f : AA_INF_F;
+  return aa_expf(y * aa_logf(x));
+}
+
+/* Cube used for alpha^3 (deviceCodeRBF.hlsl:375-376). */
+AA_FN float aa_cube(float a) { return (a * a) * a; }
+
+/* |a - b| with the summation order x, y, z (HLSL distance()). */
+AA_FN float aa_dist3(float ax, float ay, float az, float bx, float by, float bz) {
+  float dx = ax - bx, dy = ay - by, dz = az - bz;
+  return AA_SQRTF((dx * dx + dy * dy) + dz * dz);
+}
+
+/* Gaussian RBF of sharedCode.h:137-140 with sigma = 3:
+ * exp(-0.5 * ((d * 3) / r)^2). */
+AA_FN float aa_rbf(float d, float r) {
+  float q = (d * 3.0f) / r;
+  return aa_expf(-0.5f * (q * q));
+}
+
+/* One LCG step (rng.h:52-57) and its [0,1] float (rng.h:59-61): the uint32
+ * is converted with round-to-nearest-even, so states >= 0xFFFFFF80 give
+ * exactly 1.0f. */
+AA_FN float aa_lcg_next(uint32_t *state) {
+  *state = *state * 1664525u + 1013904223u;
+  return (float)(*state) * 2.3283064365386963e-10f; /* 2^-32 */
+}
+
+AA_FN uint32_t aa_murmur_mix(uint32_t h, uint32_t k) {
+  k *= 0xcc9e2d51u;
+  k = (k << 15) | (k >> 17);
+  k *= 0x1b873593u;
+  h ^= k;
+  h = (h << 13) | (h >> 19);
+  return h * 5u + 0xe6546b64u;
+}
+
+AA_FN uint32_t aa_murmur_fmix(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+/* Per-pixel seed (rng.h:63-70): two murmur3 block mixes of the linear
+ * pixel index and the frame id, then the murmur3 finaliser. */
+AA_FN uint32_t aa_rng_seed(uint32_t frame_id, uint32_t px, uint32_t py, uint32_t width) {
+  uint32_t h = aa_murmur_mix(0u, px + py * width);
+  h = aa_murmur_mix(h, frame_id);
+  return aa_murmur_fmix(h);
+}
+
+/* 1D linearly filtered lookup into an RGBA float table of n texels with
+ * clamp-to-edge addressing, matching a normalised-coordinate texture fetch
+ * (texel i centred at (i + 0.5) / n).  Returns component c of the sample;
+ * the interpolation is a + f * (b - a), so constant tables are returned
+ * exactly.  NaN coordinates read texel 0. */
+AA_FN void aa_tf_coords(float u, uint32_t n, uint32_t *j0, uint32_t *j1, float *f) {
+  float x = u * (float)n - 0.5f;
+  x = AA_FMAXF(x, -1.0f); /* also maps NaN to -1 */
+  x = AA_FMINF(x, (float)n);
+  float fl = AA_FLOORF(x);
+  int i0 = (int)fl;
+  *f = x - fl;
+  int a = i0 < 0 ? 0 : (i0 > (int)n - 1 ? (int)n - 1 : i0);
+  int b = i0 + 1 < 0 ? 0 : (i0 + 1 > (int)n - 1 ? (int)n - 1 : i0 + 1);
+  *j0 = (uint32_t)a;
+  *j1 = (uint32_t)b;
+}
+
+AA_FN float aa_lerp_tex(float a, float b, float f) { return a + f * (b - a); }
+
+#endif /* AARBF_DETMATH_H */

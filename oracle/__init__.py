@@ -1,0 +1,304 @@
+"""Python loader of the CPU oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by the tests, the smoke check and the
+benchmark's CPU-baseline leg, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libhilbert_ref.so")
+
+f32p = C.POINTER(C.c_float)
+u64p = C.POINTER(C.c_uint64)
+u32p = C.POINTER(C.c_uint32)
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("accumID", C.c_uint32), ("frameID", C.c_uint32), ("enableBlueNoise", C.c_uint32),
+        ("showNoise", C.c_uint32), ("visualizeAttributes", C.c_uint32),
+        ("light_azimuth", C.c_float), ("light_elevation", C.c_float), ("light_ambient", C.c_float),
+        ("cam_pos", C.c_float * 3), ("cam_d00", C.c_float * 3), ("cam_du", C.c_float * 3),
+        ("cam_dv", C.c_float * 3),
+        ("rbfRadius", C.c_float), ("unit", C.c_float),
+        ("width", C.c_int32), ("height", C.c_int32),
+        ("aabb_min", C.c_float * 3), ("aabb_max", C.c_float * 3),
+        ("xyzw", f32p), ("radii", f32p), ("numParticles", C.c_uint32),
+        ("colormap", f32p), ("radiusmap", f32p), ("densitymap", f32p),
+        ("cm_n", C.c_uint32), ("rm_n", C.c_uint32), ("dm_n", C.c_uint32),
+        ("stbn", C.POINTER(C.c_uint8)), ("stbn_w", C.c_int32), ("stbn_h", C.c_int32),
+        ("max_depth", C.c_uint32),
+    ]
+
+
+def build(ref: bool = False) -> None:
+    """make -C oracle (and the reference hilbert.h harness when asked)."""
+    targets = ["all"] + (["ref"] if ref else [])
+    subprocess.run(["make", "-s", "-C", HERE, *targets], check=True)
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_hilbert_c2i.restype = C.c_uint64
+        L.orc_hilbert_c2i.argtypes = [u64p]
+        L.orc_hilbert_i2c.argtypes = [C.c_uint64, u64p]
+        L.orc_hilbert_codes.argtypes = [f32p, C.c_uint64, f32p, u64p]
+        L.orc_spacetime_aabb.argtypes = [f32p, C.c_uint64, f32p]
+        L.orc_sort_particles.argtypes = [f32p, u64p, C.c_uint64, u32p]
+        L.orc_attribute_range.argtypes = [f32p, C.c_uint64, f32p]
+        L.orc_normalize_attributes.argtypes = [f32p, C.c_uint64, f32p]
+        L.orc_tf_sample.argtypes = [f32p, C.c_uint32, C.c_float, f32p]
+        L.orc_gen_bounds.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_float, f32p, C.c_uint32, f32p, f32p,
+                                     C.c_uint32]
+        L.orc_bvh_num_levels.restype = C.c_uint32
+        L.orc_bvh_num_levels.argtypes = [C.c_uint32, C.c_uint32]
+        L.orc_bvh_build_level.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_uint32, f32p]
+        L.orc_render.restype = C.c_int
+        L.orc_render.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int32), C.c_uint64, C.c_uint32, f32p, f32p,
+                                 C.c_int, u64p]
+        L.orc_scene_new.restype = C.c_void_p
+        L.orc_scene_new.argtypes = [C.POINTER(OrcParams)]
+        L.orc_scene_render.restype = C.c_int
+        L.orc_scene_render.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_uint64, C.c_uint32, f32p, f32p,
+                                       C.c_int, u64p]
+        L.orc_scene_free.argtypes = [C.c_void_p]
+        L.orc_point_queries.restype = C.c_int
+        L.orc_point_queries.argtypes = [C.POINTER(OrcParams), f32p, C.c_uint64, f32p]
+        L.orc_light_direction.argtypes = [C.c_float, C.c_float, f32p]
+        L.orc_taa_resolve.argtypes = [f32p, f32p, f32p, C.c_int, C.c_int, C.c_int]
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's own hilbert.h compiled by `make -C oracle ref`, or None."""
+    if not os.path.exists(REF_LIB):
+        return None
+    L = C.CDLL(REF_LIB)
+    L.ref_hilbert_c2i.restype = C.c_uint64
+    L.ref_hilbert_c2i.argtypes = [u64p]
+    L.ref_hilbert_i2c.argtypes = [C.c_uint64, u64p]
+    L.ref_hilbert_codes.argtypes = [f32p, C.c_uint64, f32p, u64p]
+    return L
+
+
+def _f(a):
+    return a.ctypes.data_as(f32p)
+
+
+def c2i(x: int, y: int, z: int) -> int:
+    return int(lib().orc_hilbert_c2i((C.c_uint64 * 3)(x, y, z)))
+
+
+def i2c(index: int):
+    out = (C.c_uint64 * 3)()
+    lib().orc_hilbert_i2c(C.c_uint64(index), out)
+    return tuple(int(v) for v in out)
+
+
+def hilbert_codes(xyzw: np.ndarray, aabb) -> np.ndarray:
+    x = np.ascontiguousarray(xyzw, dtype=np.float32)
+    a = np.asarray(aabb, dtype=np.float32)
+    out = np.empty(x.shape[0], dtype=np.uint64)
+    lib().orc_hilbert_codes(_f(x), x.shape[0], _f(a), out.ctypes.data_as(u64p))
+    return out
+
+
+def spacetime_aabb(xyzw: np.ndarray, aabb=None) -> np.ndarray:
+    x = np.ascontiguousarray(xyzw, dtype=np.float32)
+    a = np.array(aabb if aabb is not None else [1e38, 1e38, 1e38, -1e38, -1e38, -1e38], dtype=np.float32)
+    lib().orc_spacetime_aabb(_f(x), x.shape[0], _f(a))
+    return a
+
+
+def sort_particles(xyzw: np.ndarray, codes: np.ndarray):
+    x = np.ascontiguousarray(xyzw, dtype=np.float32).copy()
+    c = np.ascontiguousarray(codes, dtype=np.uint64).copy()
+    perm = np.empty(x.shape[0], dtype=np.uint32)
+    lib().orc_sort_particles(_f(x), c.ctypes.data_as(u64p), x.shape[0], perm.ctypes.data_as(u32p))
+    return x, c, perm
+
+
+def attribute_range(xyzw: np.ndarray, rng=None) -> np.ndarray:
+    x = np.ascontiguousarray(xyzw, dtype=np.float32)
+    r = np.array(rng if rng is not None else [1e20, -1e20], dtype=np.float32)
+    lib().orc_attribute_range(_f(x), x.shape[0], _f(r))
+    return r
+
+
+def normalize_attributes(xyzw: np.ndarray, rng) -> np.ndarray:
+    x = np.ascontiguousarray(xyzw, dtype=np.float32).copy()
+    r = np.asarray(rng, dtype=np.float32)
+    lib().orc_normalize_attributes(_f(x), x.shape[0], _f(r))
+    return x
+
+
+def preprocess(xyzw: np.ndarray, radii=None):
+    """hostCode.cpp:183-250 for one timestep: returns sorted+normalised
+    particles, sorted codes, permutation, aabb, attribute range, radii."""
+    aabb = spacetime_aabb(xyzw)
+    codes = hilbert_codes(xyzw, aabb)
+    p, c, perm = sort_particles(xyzw, codes)
+    rng = attribute_range(p)
+    p = normalize_attributes(p, rng)
+    r = None if radii is None else np.ascontiguousarray(np.asarray(radii, np.float32)[perm])
+    return p, c, perm, aabb, rng, r
+
+
+def tf_sample(table: np.ndarray, u: float) -> np.ndarray:
+    t = np.ascontiguousarray(table, dtype=np.float32)
+    out = np.empty(4, dtype=np.float32)
+    lib().orc_tf_sample(_f(t), t.shape[0], C.c_float(u), _f(out))
+    return out
+
+
+def gen_bounds(xyzw, ppl: int, rbf_radius: float, radiusmap, radii=None, num_aabbs=None, num_particles=None):
+    x = np.ascontiguousarray(xyzw, dtype=np.float32)
+    n = x.shape[0] if num_particles is None else num_particles
+    na = (x.shape[0] + ppl - 1) // ppl if num_aabbs is None else num_aabbs
+    rm = np.ascontiguousarray(radiusmap, dtype=np.float32)
+    r = None if radii is None else np.ascontiguousarray(radii, dtype=np.float32)
+    out = np.empty(na * 6, dtype=np.float32)
+    lib().orc_gen_bounds(_f(x), n, ppl, C.c_float(rbf_radius), _f(rm), rm.shape[0],
+                         _f(r) if r is not None else None, _f(out), na)
+    return out
+
+
+def bvh_levels(aabbs: np.ndarray, K: int):
+    """Node boxes of every level of the Hilbert-packed BVH (level 0 = the
+    widened leaf boxes), as float32 [n, 8] arrays."""
+    L = lib()
+    a = np.ascontiguousarray(aabbs, dtype=np.float32)
+    n0 = a.size // 6
+    levels = []
+    # level 0: widened leaves (each leaf is its own 1-child node)
+    lv = np.empty((n0, 8), dtype=np.float32)
+    L.orc_bvh_build_level(_f(a), 1, n0, 1, _f(lv))
+    levels.append(lv)
+    n = n0
+    nl = L.orc_bvh_num_levels(n0, K)
+    while len(levels) < max(2, nl):
+        m = (n + K - 1) // K
+        nxt = np.empty((m, 8), dtype=np.float32)
+        L.orc_bvh_build_level(_f(levels[-1]), 0, n, K, _f(nxt))
+        levels.append(nxt)
+        n = m
+    return levels
+
+
+def light_direction(azimuth: float, elevation: float) -> np.ndarray:
+    out = np.empty(3, dtype=np.float32)
+    lib().orc_light_direction(C.c_float(azimuth), C.c_float(elevation), _f(out))
+    return out
+
+
+class Render:
+    """Keeps the numpy arrays referenced by an OrcParams alive."""
+
+    def __init__(self, xyzw, radii, aabb, colormap, radiusmap, densitymap, camera, width, height, unit,
+                 rbf_radius, frame_id=0, accum_id=1, blue_noise=False, stbn=None, show_noise=False,
+                 visualize_attributes=True, light=(0.0, 0.0, 0.5), max_depth=0):
+        self.keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self.keep.append(a)
+            return a
+
+        p = OrcParams()
+        p.accumID, p.frameID = accum_id, frame_id
+        p.enableBlueNoise, p.showNoise, p.visualizeAttributes = int(blue_noise), int(show_noise), int(
+            visualize_attributes)
+        p.light_azimuth, p.light_elevation, p.light_ambient = light
+        for name, v in (("cam_pos", camera.pos), ("cam_d00", camera.dir_00), ("cam_du", camera.dir_du),
+                        ("cam_dv", camera.dir_dv)):
+            getattr(p, name)[:] = [float(np.float32(x)) for x in v]
+        p.rbfRadius, p.unit = rbf_radius, unit
+        p.width, p.height = width, height
+        p.aabb_min[:] = [float(x) for x in aabb[:3]]
+        p.aabb_max[:] = [float(x) for x in aabb[3:6]]
+        x = arr(xyzw, np.float32)
+        p.xyzw = _f(x)
+        p.numParticles = x.shape[0]
+        if radii is not None:
+            p.radii = _f(arr(radii, np.float32))
+        for nm, t in (("colormap", colormap), ("radiusmap", radiusmap), ("densitymap", densitymap)):
+            a = arr(np.asarray(t, np.float32).reshape(-1, 4), np.float32)
+            setattr(p, nm, _f(a))
+            setattr(p, {"colormap": "cm_n", "radiusmap": "rm_n", "densitymap": "dm_n"}[nm], a.shape[0])
+        if stbn is not None:
+            s = arr(stbn, np.uint8)
+            p.stbn = s.ctypes.data_as(C.POINTER(C.c_uint8))
+            p.stbn_h, p.stbn_w = s.shape[0], s.shape[1]
+        p.max_depth = max_depth
+        self.p = p
+
+    def run(self, pixels=None, spp: int = 1, threads: int = 8, image=None, accum=None):
+        W, H = self.p.width, self.p.height
+        image = np.zeros((H, W, 4), np.float32) if image is None else image
+        accum = np.zeros((H, W, 4), np.float32) if accum is None else accum
+        counts = (C.c_uint64 * 2)()
+        if pixels is not None:
+            px = np.ascontiguousarray(np.asarray(pixels, dtype=np.int32).reshape(-1, 2))
+            st = lib().orc_render(C.byref(self.p), px.ctypes.data_as(C.POINTER(C.c_int32)), px.shape[0], spp,
+                                  _f(image), _f(accum), threads, counts)
+        else:
+            st = lib().orc_render(C.byref(self.p), None, 0, spp, _f(image), _f(accum), threads, counts)
+        if st != 0:
+            raise RuntimeError("orc_render failed")
+        return image, accum, (int(counts[0]), int(counts[1]))
+
+    def prepared(self):
+        """Bakes and grids the scene once; returns an object whose
+        render(pixels, spp, threads) only renders (for timing)."""
+        outer = self
+
+        class _Prepared:
+            def __init__(self):
+                self.h = lib().orc_scene_new(C.byref(outer.p))
+                if not self.h:
+                    raise RuntimeError("orc_scene_new failed")
+
+            def render(self, pixels, spp=1, threads=8):
+                W, H = outer.p.width, outer.p.height
+                image = np.zeros((H, W, 4), np.float32)
+                accum = np.zeros((H, W, 4), np.float32)
+                px = np.ascontiguousarray(np.asarray(pixels, dtype=np.int32).reshape(-1, 2))
+                counts = (C.c_uint64 * 2)()
+                lib().orc_scene_render(self.h, px.ctypes.data_as(C.POINTER(C.c_int32)), px.shape[0], spp,
+                                       _f(image), _f(accum), threads, counts)
+                return image, accum, (int(counts[0]), int(counts[1]))
+
+            def __del__(self):
+                if getattr(self, "h", None):
+                    lib().orc_scene_free(self.h)
+                    self.h = None
+
+        return _Prepared()
+
+    def point_queries(self, pts: np.ndarray) -> np.ndarray:
+        q = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+        out = np.empty((q.shape[0], 5), dtype=np.float32)
+        if lib().orc_point_queries(C.byref(self.p), _f(q), q.shape[0], _f(out)) != 0:
+            raise RuntimeError("orc_point_queries failed")
+        return out
+
+
+def taa_resolve(image: np.ndarray, taa: np.ndarray, taa_prev: np.ndarray, disable: bool = False):
+    H, W = image.shape[:2]
+    im = np.ascontiguousarray(image, np.float32)
+    lib().orc_taa_resolve(_f(im), _f(taa), _f(taa_prev), W, H, int(disable))

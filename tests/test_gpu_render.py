@@ -1,0 +1,155 @@
+"""GPU parity of ParticleRBFRayGen (delta tracking and stochastic marching)
+and the TAA resolve against the oracle at fixed RNG streams, through the C
+ABI.  The bar is bit-exact output; the stated tolerance (1e-4 relative) is
+checked as well so that a failure reports its size."""
+import numpy as np
+import pytest
+
+import aarbf
+from aarbf import scenes
+from helpers import default_camera, random_scene
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-4
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _compare(got, exp, what):
+    got = np.asarray(got, np.float32)
+    exp = np.asarray(exp, np.float32)
+    rel = np.abs(got - exp) / np.maximum(np.abs(exp), 1e-6)
+    bad = np.argwhere(rel > RTOL)
+    assert bad.size == 0, f"{what}: {len(bad)} values differ, first at {bad[:3].tolist()} " \
+                          f"got {got[tuple(bad[0])]} expected {exp[tuple(bad[0])]}"
+    assert np.array_equal(got, exp), f"{what}: within {RTOL} but not bit-exact"
+
+
+def _setup(ctx, xyzw, radii, tfs, rbf, ppl=1, K=8):
+    cm, rm, dm = tfs
+    sc = aarbf.Scene.prepare(ctx, xyzw, cm, rm, dm, rbf, radii=radii, ppl=ppl, branching=K)
+    op, _, operm, oaabb, _, orad = oracle.preprocess(xyzw, radii)
+    assert np.array_equal(sc.particles.cpu().numpy(), op)
+    assert np.array_equal(sc.aabb, oaabb)
+    return sc, op, orad
+
+
+def _gpu_render(ctx, sc, cam, W, H, unit, spp=1, frame=0, accum=1, stbn=0, **kw):
+    torch = _torch()
+    img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    acc = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    pc = sc.push_constants(cam, unit, frame_id=frame, accum_id=accum, **kw)
+    rg = sc.raygen_data(img, acc, W, H, stbn=stbn)
+    ctx.render(rg, pc, W, H, spp)
+    ctx.sync()
+    return img.cpu().numpy(), acc.cpu().numpy()
+
+
+CASES = [
+    # name, n, blue_noise, ppl, K, visualize_attributes, show_noise, spp
+    ("tracking", 3000, False, 1, 8, True, False, 2),
+    ("marching", 3000, True, 1, 8, True, False, 2),
+    ("tracking-ppl4-K4", 3000, False, 4, 4, True, False, 1),
+    ("marching-ppl3-K16", 2000, True, 3, 16, True, False, 1),
+    ("tracking-density-colour", 3000, False, 1, 2, False, False, 1),
+    ("marching-shownoise", 1500, True, 2, 8, True, True, 1),
+]
+
+
+@pytest.mark.parametrize("name,n,blue,ppl,K,vis,noise,spp", CASES)
+def test_render_matches_oracle(gpu_ctx, name, n, blue, ppl, K, vis, noise, spp):
+    W, H = 48, 40
+    xyzw, radii = random_scene(n, seed=n + ppl, extent=1.0, r=(0.03, 0.09))
+    tfs = (scenes.ramp_table(256), scenes.constant_table(n=4), scenes.ramp_table(64))
+    if not vis:
+        tfs = (scenes.reference_tf("colormap"), scenes.reference_tf("radiusmap"), scenes.reference_tf("densitymap"))
+    sc, op, orad = _setup(gpu_ctx, xyzw, radii, tfs, 0.05, ppl, K)
+    cam = default_camera(W, H, look_from=(0.4, 0.3, 2.2))
+    unit = 0.004
+    stbn_img = scenes.synthetic_stbn(seed=3) if blue else None
+    stbn = gpu_ctx.stbn(stbn_img) if blue else 0
+    light = (0.3, 0.2, 0.4)
+    img, acc = _gpu_render(gpu_ctx, sc, cam, W, H, unit, spp=spp, frame=5, accum=1, stbn=stbn, blue_noise=blue,
+                           show_noise=noise, visualize_attributes=vis, light=light)
+    r = oracle.Render(op, orad, sc.aabb, *tfs, cam, W, H, unit, 0.05, frame_id=5, accum_id=1, blue_noise=blue,
+                      stbn=stbn_img, show_noise=noise, visualize_attributes=vis, light=light)
+    oimg, oacc, _ = r.run(spp=spp, threads=8)
+    _compare(acc, oacc, f"{name} accum")
+    _compare(img, oimg, f"{name} image")
+    assert (oacc[..., :3] > 0).mean() > 0.05, "scene too empty to be a meaningful check"
+
+
+def test_fused_spp_equals_consecutive_launches(gpu_ctx):
+    W, H = 32, 24
+    xyzw, radii = random_scene(2000, seed=3)
+    tfs = (scenes.ramp_table(256), scenes.constant_table(n=4), scenes.ramp_table(64))
+    sc, _, _ = _setup(gpu_ctx, xyzw, radii, tfs, 0.05)
+    cam = default_camera(W, H, look_from=(0.0, 0.5, 2.0))
+    torch = _torch()
+    img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    acc = torch.zeros_like(img)
+    for s in range(3):
+        pc = sc.push_constants(cam, 0.004, frame_id=10 + s, accum_id=1 + s)
+        gpu_ctx.render(sc.raygen_data(img, acc, W, H), pc, W, H, 1)
+    gpu_ctx.sync()
+    fimg, facc = _gpu_render(gpu_ctx, sc, cam, W, H, 0.004, spp=3, frame=10, accum=1)
+    assert np.array_equal(acc.cpu().numpy(), facc) and np.array_equal(img.cpu().numpy(), fimg)
+
+
+def test_uniform_cube_full_frame(gpu_ctx):
+    """The CPU-reference workload (65,536 particles, 512x512, 1 spp
+    marching) rendered in full on both sides."""
+    wl = scenes.uniform_cube()
+    sc, op, orad = _setup(gpu_ctx, wl.xyzw, wl.radii, (wl.colormap, wl.radiusmap, wl.densitymap), wl.rbf_radius)
+    cam = aarbf.camera_basis(wl.look_from, wl.look_at, wl.look_up, wl.cos_fovy, wl.width, wl.height)
+    stbn_img = scenes.synthetic_stbn()
+    stbn = gpu_ctx.stbn(stbn_img)
+    img, acc = _gpu_render(gpu_ctx, sc, cam, wl.width, wl.height, wl.unit, spp=1, stbn=stbn, blue_noise=True)
+    r = oracle.Render(op, orad, sc.aabb, wl.colormap, wl.radiusmap, wl.densitymap, cam, wl.width, wl.height,
+                      wl.unit, wl.rbf_radius, blue_noise=True, stbn=stbn_img)
+    oimg, oacc, _ = r.run(spp=1, threads=16)
+    _compare(img, oimg, "uniform cube image")
+
+
+def test_stats_count_samples(gpu_ctx):
+    W, H = 16, 16
+    xyzw, radii = random_scene(1000, seed=1)
+    tfs = (scenes.ramp_table(16), scenes.constant_table(n=1), scenes.ramp_table(16))
+    sc, _, _ = _setup(gpu_ctx, xyzw, radii, tfs, 0.05)
+    cam = default_camera(W, H)
+    gpu_ctx.set_stats(True)
+    gpu_ctx.read_stats(reset=True)
+    _gpu_render(gpu_ctx, sc, cam, W, H, 0.004, spp=2)
+    st = gpu_ctx.read_stats(reset=True)
+    gpu_ctx.set_stats(False)
+    assert st["samples"] == W * H * 2
+    assert st["steps"] > 0 and st["nodes"] >= st["queries"] > 0 and st["covering"] <= st["particles"]
+
+
+def test_taa_matches_oracle(gpu_ctx):
+    torch = _torch()
+    rng = np.random.default_rng(0)
+    H, W = 37, 53
+    image = rng.uniform(0, 1, (H, W, 4)).astype(np.float32)
+    prev = rng.uniform(-0.5, 1.5, (H, W, 4)).astype(np.float32)
+    taa = np.zeros_like(image)
+    otaa, oprev = taa.copy(), prev.copy()
+    oracle.taa_resolve(image, otaa, oprev)
+    d_img = torch.from_numpy(image).cuda()
+    d_taa = torch.zeros_like(d_img)
+    d_prev = torch.from_numpy(prev).cuda()
+    d_frame = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+    tc = aarbf.TAAConstants(0, 0, aarbf._lib.Int2(W, H), d_img.data_ptr(), 0, d_frame.data_ptr(),
+                            d_taa.data_ptr(), d_prev.data_ptr())
+    gpu_ctx.taa_resolve(tc)
+    gpu_ctx.sync()
+    _compare(d_taa.cpu().numpy(), otaa, "taa")
+    assert np.array_equal(d_prev.cpu().numpy(), oprev)
+    f = d_frame.cpu().numpy().view(np.uint32)
+    a8 = (np.clip(otaa[..., 3], 0, 1) * np.float32(255) + np.float32(0.5)).astype(np.uint32)
+    assert np.array_equal(f >> 24, a8)  # no GUI layer: over() keeps the pixel
