@@ -115,10 +115,16 @@ class Context:
     def set_stream(self, stream):
         """Use a torch.cuda.Stream (or a raw hipStream_t int) for all work."""
         ptr = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        if not ptr:
+            raise ValueError("pass a non-default stream (torch.cuda.Stream()); 0 selects the context's own stream")
         self.check(self.lib.aarbf_set_stream(self.h, C.c_void_p(ptr)), "aarbf_set_stream")
 
     def sync(self):
         self.check(self.lib.aarbf_sync(self.h), "aarbf_sync")
+
+    def set_render_mode(self, mode: int):
+        """0: persistent segment-gather kernel (default); 1: per-pixel point queries."""
+        self.check(self.lib.aarbf_set_render_mode(self.h, int(mode)), "aarbf_set_render_mode")
 
     def set_stats(self, enable: bool):
         self.check(self.lib.aarbf_set_stats(self.h, 1 if enable else 0), "aarbf_set_stats")

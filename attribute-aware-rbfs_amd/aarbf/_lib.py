@@ -86,6 +86,7 @@ SIGNATURES = {
     "aarbf_last_error": (C.c_char_p, [P]),
     "aarbf_status_string": (C.c_char_p, [C.c_int]),
     "aarbf_set_stats": (C.c_int, [P, C.c_int]),
+    "aarbf_set_render_mode": (C.c_int, [P, C.c_int]),
     "aarbf_render_stats_read": (C.c_int, [P, C.POINTER(RenderStats), C.c_int]),
     "aarbf_import_particles": (C.c_int, [C.c_char_p, C.POINTER(P)]),
     "aarbf_timesteps_count": (u32, [P]),
@@ -143,6 +144,14 @@ def load(path: str | None = None) -> C.CDLL:
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise RuntimeError(f"libaarbf.so not built at {p}: run attribute-aware-rbfs_amd/build.py")
+    # torch ships its own libamdhip64 / libhsa-runtime64 (same sonames as
+    # ROCm's).  Loading torch first makes the library bind to that already
+    # loaded runtime, so one HIP runtime owns the device and torch tensors,
+    # streams and events are valid handles for the library.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(p)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -89,6 +89,8 @@ int aarbf_context_create(int device, aarbf_context **out) {
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&ctx->d_stats, sizeof(uint64_t) * 8) != hipSuccess ||
+      hipMalloc(&ctx->d_counter, sizeof(uint32_t) * 4) != hipSuccess ||
+      hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
       hipMemset(ctx->d_stats, 0, sizeof(uint64_t) * 8) != hipSuccess) {
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
     delete ctx;
@@ -121,6 +123,7 @@ int aarbf_context_destroy(aarbf_context *ctx) {
   if (ctx->scratch) hipFree(ctx->scratch);
   if (ctx->baked) hipFree(ctx->baked);
   if (ctx->d_stats) hipFree(ctx->d_stats);
+  if (ctx->d_counter) hipFree(ctx->d_counter);
   if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return AARBF_OK;
@@ -142,6 +145,13 @@ int aarbf_sync(aarbf_context *ctx) {
 
 const char *aarbf_last_error(const aarbf_context *ctx) {
   return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+int aarbf_set_render_mode(aarbf_context *ctx, int mode) {
+  if (!ctx) return AARBF_ERR_INVALID_ARGUMENT;
+  AARBF_REQUIRE(ctx, mode == 0 || mode == 1, "aarbf_set_render_mode: mode must be 0 or 1");
+  ctx->render_mode = mode;
+  return AARBF_OK;
 }
 
 int aarbf_set_stats(aarbf_context *ctx, int enable) {

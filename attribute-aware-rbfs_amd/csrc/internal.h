@@ -56,6 +56,9 @@ struct aarbf_context {
   std::string last_error;
   bool stats = false;
   uint64_t *d_stats = nullptr; // 8 counters
+  uint32_t *d_counter = nullptr; // work counter of the persistent render kernel
+  int render_mode = 0;           // 0 persistent segment-gather, 1 per-pixel point queries
+  int cus = 0;                   // compute units of the device
   // grow-only device scratch
   void *scratch = nullptr;
   size_t scratch_bytes = 0;

@@ -43,6 +43,7 @@ def main() -> int:
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", type=int, default=0, help="0 persistent segment-gather kernel, 1 point queries")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "render_traffic.json"),
                     help="HBM bytes per render launch measured with rocprofv3 --pmc")
@@ -68,8 +69,11 @@ def main() -> int:
     W, H, S = wl.width, wl.height, wl.spp
 
     ctx = aarbf.Context(0)
-    stream = torch.cuda.current_stream()
+    # one stream for the library's kernels and for the timing events
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     ctx.set_stream(stream)
+    ctx.set_render_mode(a.mode)
     torch.cuda.synchronize()
     tp = time.time()
     sc = aarbf.Scene.prepare(ctx, wl.xyzw, wl.colormap, wl.radiusmap, wl.densitymap, wl.rbf_radius,
@@ -172,7 +176,7 @@ def main() -> int:
             "workload": f"mixture-{wl.xyzw.shape[0]} delta tracking {W}x{H} {S} spp/frame, TAA, orbit of "
                         f"{wl.frames} frames",
             "particles": int(wl.xyzw.shape[0]), "width": W, "height": H, "spp": S, "particles_per_leaf": 1,
-            "bvh_branching": 8, "unit": wl.unit, "max_depth": aarbf.MAX_DEPTH,
+            "bvh_branching": 8, "unit": wl.unit, "render_mode": a.mode, "max_depth": aarbf.MAX_DEPTH,
             "exhausted_fraction": st["exhausted"] / samples, "preprocess_ms": round(prep_s * 1e3, 1),
             "parallelism": "single GPU",
         },

@@ -149,6 +149,10 @@ void *aarbf_get_stream(aarbf_context *ctx);
 int aarbf_sync(aarbf_context *ctx);
 const char *aarbf_last_error(const aarbf_context *ctx);
 const char *aarbf_status_string(int status);
+/* Render schedule: 0 (default) persistent waves with per-segment candidate
+ * gathering, 1 one lane per pixel with a BVH point query per step.  Both
+ * produce identical images; no counterpart in the reference. */
+int aarbf_set_render_mode(aarbf_context *ctx, int mode);
 /* Enable (1) or disable (0) render statistics; read them back (and reset
  * them) with aarbf_render_stats_read. */
 int aarbf_set_stats(aarbf_context *ctx, int enable);

@@ -141,3 +141,10 @@ def test_bvh_build_and_refit(gpu_ctx, K):
     for l in range(len(lv2)):
         assert np.array_equal(gpu_ctx.bvh_read_level(h, l), lv2[l])
     gpu_ctx.bvh_destroy(h)
+
+
+def test_single_hip_runtime(gpu_ctx):
+    """torch and the library share one HIP runtime in the process."""
+    maps = open("/proc/self/maps").read()
+    libs = {l.split()[-1] for l in maps.splitlines() if "libamdhip64" in l}
+    assert len(libs) == 1, libs

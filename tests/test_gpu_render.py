@@ -74,14 +74,19 @@ def test_render_matches_oracle(gpu_ctx, name, n, blue, ppl, K, vis, noise, spp):
     stbn_img = scenes.synthetic_stbn(seed=3) if blue else None
     stbn = gpu_ctx.stbn(stbn_img) if blue else 0
     light = (0.3, 0.2, 0.4)
-    img, acc = _gpu_render(gpu_ctx, sc, cam, W, H, unit, spp=spp, frame=5, accum=1, stbn=stbn, blue_noise=blue,
-                           show_noise=noise, visualize_attributes=vis, light=light)
     r = oracle.Render(op, orad, sc.aabb, *tfs, cam, W, H, unit, 0.05, frame_id=5, accum_id=1, blue_noise=blue,
                       stbn=stbn_img, show_noise=noise, visualize_attributes=vis, light=light)
     oimg, oacc, _ = r.run(spp=spp, threads=8)
-    _compare(acc, oacc, f"{name} accum")
-    _compare(img, oimg, f"{name} image")
     assert (oacc[..., :3] > 0).mean() > 0.05, "scene too empty to be a meaningful check"
+    for mode in (0, 1):  # persistent segment-gather kernel, per-pixel point-query kernel
+        gpu_ctx.set_render_mode(mode)
+        try:
+            img, acc = _gpu_render(gpu_ctx, sc, cam, W, H, unit, spp=spp, frame=5, accum=1, stbn=stbn,
+                                   blue_noise=blue, show_noise=noise, visualize_attributes=vis, light=light)
+        finally:
+            gpu_ctx.set_render_mode(0)
+        _compare(acc, oacc, f"{name} mode {mode} accum")
+        _compare(img, oimg, f"{name} mode {mode} image")
 
 
 def test_fused_spp_equals_consecutive_launches(gpu_ctx):
@@ -114,6 +119,26 @@ def test_uniform_cube_full_frame(gpu_ctx):
                       wl.unit, wl.rbf_radius, blue_noise=True, stbn=stbn_img)
     oimg, oacc, _ = r.run(spp=1, threads=16)
     _compare(img, oimg, "uniform cube image")
+
+
+def test_dense_cluster_uses_point_query_fallback(gpu_ctx):
+    """A cluster far denser than the per-segment candidate capacity: the
+    persistent kernel falls back to point queries there and must still
+    match the oracle bit for bit."""
+    W, H = 40, 40
+    rng = np.random.default_rng(12)
+    n = 6000
+    xyzw = np.empty((n, 4), np.float32)
+    xyzw[:, :3] = rng.normal(0.0, 0.05, (n, 3))
+    xyzw[:, 3] = rng.uniform(0, 1, n)
+    radii = rng.uniform(0.05, 0.1, n).astype(np.float32)
+    tfs = (scenes.ramp_table(256), scenes.constant_table(n=4), scenes.constant_table((0.02, 0, 0, 0), n=1))
+    sc, op, orad = _setup(gpu_ctx, xyzw, radii, tfs, 0.1)
+    cam = default_camera(W, H, look_from=(0.0, 0.2, 1.2))
+    r = oracle.Render(op, orad, sc.aabb, *tfs, cam, W, H, 0.01, 0.1, frame_id=1, accum_id=1, light=(0.1, 0.6, 0.3))
+    oimg, oacc, _ = r.run(spp=2, threads=8)
+    img, acc = _gpu_render(gpu_ctx, sc, cam, W, H, 0.01, spp=2, frame=1, accum=1, light=(0.1, 0.6, 0.3))
+    _compare(acc, oacc, "dense accum")
 
 
 def test_stats_count_samples(gpu_ctx):
