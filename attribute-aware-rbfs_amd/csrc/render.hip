@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void render_kernel(RenderArgs A) {
 //    particles crowd a short segment the lane falls back to point queries.
 // ---------------------------------------------------------------------------
 
-constexpr uint32_t kCand = 32;
+constexpr uint32_t kCand = 24;
 constexpr uint32_t kDense = 0xffffffffu;
 constexpr int kPBlock = 256;
 
@@ -421,6 +421,7 @@ __device__ uint32_t gather(const RenderArgs &A, const SegQuery &s, uint32_t *my,
   const uint32_t log2K = A.bvh.log2K, kmask = (1u << log2K) - 1u, top = A.bvh.top;
   const uint32_t *lofs = g_lvl_ofs, *lcnt = g_lvl_cnt;
   uint32_t level = top, idx = 0, n = 0;
+  if (STATS) c.queries++;
   for (;;) {
     const float4 *lv = N + lofs[level];
     if (STATS) c.nodes++;
@@ -460,6 +461,166 @@ __device__ uint32_t gather(const RenderArgs &A, const SegQuery &s, uint32_t *my,
   }
 }
 
+// ---- batched walk ------------------------------------------------------------
+// Depth-first like the stackless walk above, but a hit node tests all K of
+// its children with independent loads (one memory round trip per level
+// instead of one per child).  The hit mask of each level still to be
+// visited is kept in a per-lane LDS column (stk[level * kPBlock]); children
+// are taken lowest bit first, so leaves are still visited in ascending
+// order.  Level-1 hits hand their K leaves to `leaves(base, m)`.
+
+constexpr uint32_t kStackLevels = 16;
+
+template <bool STATS, class BoxTest, class Leaves>
+__device__ inline void walk_batched(const RenderArgs &A, uint16_t *stk, BoxTest test, Leaves leaves, Counters &c) {
+  const float4 *__restrict__ N = A.bvh.nodes;
+  const uint32_t log2K = A.bvh.log2K, K = 1u << log2K, top = A.bvh.top;
+  const float4 *root = N + g_lvl_ofs[top];
+  if (STATS) c.nodes++;
+  if (!test(root[0], root[1])) return;
+  if (top == 1) {
+    leaves(0u, min(K, g_lvl_cnt[0]));
+    return;
+  }
+  auto group = [&](uint32_t lvl, uint32_t base) -> uint32_t {
+    const float4 *lv = N + g_lvl_ofs[lvl] + 2 * (uint64_t)base;
+    uint32_t m = min(K, g_lvl_cnt[lvl] - base), mask = 0;
+    if (STATS) c.nodes += m;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k)
+      if (k < m && test(lv[2 * k], lv[2 * k + 1])) mask |= 1u << k;
+    for (uint32_t k = 8; k < m; ++k) // K = 16
+      if (test(lv[2 * k], lv[2 * k + 1])) mask |= 1u << k;
+    return mask;
+  };
+  uint32_t lvl = top - 1, base = 0;
+  uint32_t mask = group(lvl, 0);
+  for (;;) {
+    if (mask == 0) {
+      if (lvl == top - 1) return;
+      ++lvl;
+      mask = stk[lvl * kPBlock];
+      base = ((base >> log2K) >> log2K) << log2K;
+      continue;
+    }
+    uint32_t bit = (uint32_t)__ffs(mask) - 1u;
+    mask &= mask - 1u;
+    uint32_t node = base + bit;
+    if (lvl == 1) {
+      uint32_t lb = node << log2K;
+      if (leaves(lb, min(K, g_lvl_cnt[0] - lb))) return;
+      continue;
+    }
+    stk[lvl * kPBlock] = (uint16_t)mask;
+    --lvl;
+    base = node << log2K;
+    mask = group(lvl, base);
+  }
+}
+
+// Point query with the batched walk; same covering set, order and
+// arithmetic as point_query.
+template <bool STATS>
+__device__ inline void point_query_b(const RenderArgs &A, uint16_t *stk, float x, float y, float z, Payload &pl,
+                                     Counters &c) {
+  pl.count = 0;
+  pl.dens = 0.0f;
+  pl.r = pl.g = pl.b = 0.0f;
+  if (STATS) c.queries++;
+  auto test = [&](const float4 &lo, const float4 &hi) { return inside(lo, hi, x, y, z); };
+  auto particle = [&](uint32_t p, const float4 &q) {
+    float d = aa_dist3(q.x, q.y, q.z, x, y, z);
+    if (d < q.w) {
+      float4 k = A.pc[p];
+      float wa = aa_rbf(d, q.w) * k.x;
+      if (STATS) c.covering++;
+      pl.count += 1;
+      pl.dens += wa;
+      pl.r += k.y * wa;
+      pl.g += k.z * wa;
+      pl.b += k.w * wa;
+      if (!A.visAttr) pl.dens = fminf(pl.dens, 1.0f);
+    }
+  };
+  auto leaves = [&](uint32_t lb, uint32_t m) -> bool {
+    if (A.ppl == 1) {
+      float4 q[8];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k)
+        if (k < m) q[k] = A.pr[lb + k];
+      if (STATS) c.particles += m;
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k)
+        if (k < m) particle(lb + k, q[k]);
+      for (uint32_t k = 8; k < m; ++k) particle(lb + k, A.pr[lb + k]);
+      return false;
+    }
+    const float4 *L0 = A.bvh.nodes + g_lvl_ofs[0];
+    for (uint32_t k = 0; k < m; ++k) {
+      uint32_t leaf = lb + k;
+      if (STATS) c.leaves++;
+      if (!inside(L0[2 * leaf], L0[2 * leaf + 1], x, y, z)) continue;
+      uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
+      for (uint32_t p = p0; p < p1; ++p) {
+        if (STATS) c.particles++;
+        particle(p, A.pr[p]);
+      }
+    }
+    return false;
+  };
+  walk_batched<STATS>(A, stk, test, leaves, c);
+}
+
+// Segment gather with the batched walk (kDense on overflow).
+template <bool STATS>
+__device__ inline uint32_t gather_b(const RenderArgs &A, uint16_t *stk, const SegQuery &s, uint32_t *my,
+                                    Counters &c) {
+  uint32_t n = 0;
+  bool overflow = false;
+  if (STATS) c.queries++;
+  auto test = [&](const float4 &lo, const float4 &hi) { return seg_box(s, lo, hi); };
+  auto take = [&](uint32_t p, const float4 &q) -> bool {
+    if (seg_particle(s, q)) {
+      if (n == kCand) {
+        overflow = true;
+        return true;
+      }
+      my[n * kPBlock] = p;
+      ++n;
+    }
+    return false;
+  };
+  auto leaves = [&](uint32_t lb, uint32_t m) -> bool {
+    if (A.ppl == 1) {
+      float4 q[8];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k)
+        if (k < m) q[k] = A.pr[lb + k];
+      if (STATS) c.particles += m;
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k)
+        if (k < m && take(lb + k, q[k])) return true;
+      for (uint32_t k = 8; k < m; ++k)
+        if (take(lb + k, A.pr[lb + k])) return true;
+      return false;
+    }
+    const float4 *L0 = A.bvh.nodes + g_lvl_ofs[0];
+    for (uint32_t k = 0; k < m; ++k) {
+      uint32_t leaf = lb + k;
+      if (STATS) c.leaves++;
+      if (!seg_box(s, L0[2 * leaf], L0[2 * leaf + 1])) continue;
+      uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
+      for (uint32_t p = p0; p < p1; ++p) {
+        if (STATS) c.particles++;
+        if (take(p, A.pr[p])) return true;
+      }
+    }
+    return false;
+  };
+  walk_batched<STATS>(A, stk, test, leaves, c);
+  return overflow ? kDense : n;
+}
+
 // Point query restricted to the segment's candidates (same arithmetic and
 // order as point_query's leaf loop).
 template <bool STATS>
@@ -494,8 +655,10 @@ __global__ void zero_counter(uint32_t *c) { *c = 0; }
 template <bool STATS>
 __global__ __launch_bounds__(kPBlock) void render_persistent(RenderArgs A, uint32_t *__restrict__ counter) {
   __shared__ uint32_t cand[kCand * kPBlock];
+  __shared__ uint16_t stack_mem[kStackLevels * kPBlock];
   load_levels(A);
   uint32_t *my = cand + threadIdx.x;
+  uint16_t *stk = stack_mem + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   const uint32_t tilesX = (uint32_t)(A.W + 7) / 8, tilesY = (uint32_t)(A.H + 7) / 8;
@@ -632,7 +795,7 @@ __global__ __launch_bounds__(kPBlock) void render_persistent(RenderArgs A, uint3
           q.iz = safe_inv(rd.z);
           q.len = tb - t;
           q.m = 1e-5f * (1.0f + fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) + fabsf(tb));
-          uint32_t n = gather<STATS>(A, q, my, c);
+          uint32_t n = gather_b<STATS>(A, stk, q, my, c);
           if (n != kDense) {
             seg_t0 = t;
             seg_t1 = tb;
@@ -653,7 +816,7 @@ __global__ __launch_bounds__(kPBlock) void render_persistent(RenderArgs A, uint3
       if (live) {
         float x = o.x + t * rd.x, y = o.y + t * rd.y, z = o.z + t * rd.z;
         if (seg_n == kDense)
-          point_query<STATS>(A, x, y, z, pl, c);
+          point_query_b<STATS>(A, stk, x, y, z, pl, c);
         else
           scan<STATS>(A, my, seg_n, x, y, z, pl, c);
         if (marching) {
@@ -840,7 +1003,7 @@ int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg, const 
   A.accum = (float4 *)rg->accumBuffer;
   A.stats = ctx->d_stats;
   A.bvh = make_view(*bvh);
-  if (ctx->render_mode == 1) {
+  if (ctx->render_mode == 1 || bvh->num_levels > kStackLevels) {
     dim3 grid((W + 15) / 16, (H + 15) / 16);
     if (ctx->stats)
       render_kernel<true><<<grid, 256, 0, ctx->stream>>>(A);

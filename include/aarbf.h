@@ -129,8 +129,8 @@ typedef struct aarbf_raygen_data {
 typedef struct aarbf_render_stats {
   uint64_t samples;       /* ParticleRBFRayGen invocations */
   uint64_t steps;         /* tracker iterations (primary + shadow) */
-  uint64_t queries;       /* point queries that reached the BVH root box */
-  uint64_t nodes;         /* internal BVH node boxes tested */
+  uint64_t queries;       /* BVH walks: per-step point queries and per-segment gathers */
+  uint64_t nodes;         /* BVH node boxes tested (levels >= 1) */
   uint64_t leaves;        /* leaf boxes tested */
   uint64_t particles;     /* particle spheres tested */
   uint64_t covering;      /* particle spheres that covered the query point */
