@@ -1,0 +1,319 @@
+// render_common.h -- device code shared by the render kernels (render.hip,
+// wavefront.hip): kernel arguments, transfer-function sampling, the BVH
+// point query and segment walks, slab test, colour resolve.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+
+#include "aarbf_detmath.h"
+#include "internal.h"
+
+namespace aarbf {
+
+// Kernel arguments of the RBF render kernels (one launch).
+struct RenderArgs {
+  uint32_t frameID, accumID, spp;
+  uint32_t blue, showNoise, visAttr;
+  uint32_t max_depth;
+  int W, H;
+  float amb, unit;
+  float3 light, pos, d00, du, dv, lb, rt;
+  const float4 *pr; // baked x, y, z, radius (0 = never covers)
+  const float4 *pc; // baked alpha^3, rgb^2.2
+  uint32_t numParticles, ppl;
+  const float4 *cmap;
+  const float4 *dmap;
+  uint32_t cm_n, dm_n;
+  const uint8_t *stbn;
+  int stbn_w, stbn_h;
+  float4 *image, *accum;
+  uint64_t *stats;
+  BVHView bvh;
+};
+
+// Default schedule (wavefront.hip): rounds of gather and step kernels.
+int render_wavefront(aarbf_context *ctx, const RenderArgs &A);
+
+namespace {
+
+// Per-block LDS copy of the BVH level table (offsets and node counts), so
+// that the walk can index it by level without copying the kernel-argument
+// array to scratch memory.
+__shared__ uint32_t g_lvl_ofs[kMaxLevels];
+__shared__ uint32_t g_lvl_cnt[kMaxLevels];
+
+__device__ inline void load_levels(const RenderArgs &A) {
+  if (threadIdx.x < kMaxLevels) {
+    g_lvl_ofs[threadIdx.x] = A.bvh.level_ofs[threadIdx.x];
+    g_lvl_cnt[threadIdx.x] = A.bvh.counts[threadIdx.x];
+  }
+  __syncthreads();
+}
+
+struct Counters {
+  uint32_t samples, steps, queries, nodes, leaves, particles, covering, exhausted;
+};
+
+struct Payload {
+  uint32_t count;
+  float dens, r, g, b;
+};
+
+__device__ inline float4 tf4(const float4 *__restrict__ t, uint32_t n, float u) {
+  uint32_t j0, j1;
+  float f;
+  aa_tf_coords(u, n, &j0, &j1, &f);
+  float4 a = t[j0], b = t[j1];
+  return make_float4(aa_lerp_tex(a.x, b.x, f), aa_lerp_tex(a.y, b.y, f), aa_lerp_tex(a.z, b.z, f),
+                     aa_lerp_tex(a.w, b.w, f));
+}
+
+__device__ inline float tf_r(const float4 *__restrict__ t, uint32_t n, float u) {
+  uint32_t j0, j1;
+  float f;
+  aa_tf_coords(u, n, &j0, &j1, &f);
+  return aa_lerp_tex(t[j0].x, t[j1].x, f);
+}
+
+// Per-particle constants of the intersection/any-hit programs
+
+__device__ inline bool inside(const float4 &lo, const float4 &hi, float x, float y, float z) {
+  return (x >= lo.x) & (x <= hi.x) & (y >= lo.y) & (y <= hi.y) & (z >= lo.z) & (z <= hi.z);
+}
+
+// TraceRay point query (deviceCodeRBF.hlsl:90-107) + intersection
+// (':338-360') + any-hit (':362-389').
+template <bool STATS>
+__device__ inline void point_query(const RenderArgs &A, float x, float y, float z, Payload &pl, Counters &c) {
+  pl.count = 0;
+  pl.dens = 0.0f;
+  pl.r = pl.g = pl.b = 0.0f;
+  const float4 *__restrict__ N = A.bvh.nodes;
+  const uint32_t log2K = A.bvh.log2K, kmask = (1u << log2K) - 1u, top = A.bvh.top;
+  const uint32_t *lofs = g_lvl_ofs, *lcnt = g_lvl_cnt;
+  uint32_t level = top, idx = 0;
+  if (STATS) c.queries++;
+  for (;;) {
+    const float4 *lv = N + lofs[level];
+    float4 lo = lv[2 * idx], hi = lv[2 * idx + 1];
+    if (STATS) c.nodes++;
+    if (inside(lo, hi, x, y, z)) {
+      if (level != 1) {
+        --level;
+        idx <<= log2K;
+        continue;
+      }
+      uint32_t l0 = idx << log2K, l1 = min(l0 + kmask + 1u, lcnt[0]);
+      for (uint32_t leaf = l0; leaf < l1; ++leaf) {
+        if (A.ppl > 1) {
+          const float4 *lb = N + lofs[0];
+          if (STATS) c.leaves++;
+          if (!inside(lb[2 * leaf], lb[2 * leaf + 1], x, y, z)) continue;
+        }
+        uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
+        for (uint32_t p = p0; p < p1; ++p) {
+          float4 q = A.pr[p];
+          if (STATS) c.particles++;
+          float d = aa_dist3(q.x, q.y, q.z, x, y, z);
+          if (d < q.w) {
+            float4 k = A.pc[p];
+            float wa = aa_rbf(d, q.w) * k.x;
+            if (STATS) c.covering++;
+            pl.count += 1;
+            pl.dens += wa;
+            pl.r += k.y * wa;
+            pl.g += k.z * wa;
+            pl.b += k.w * wa;
+            if (!A.visAttr) pl.dens = fminf(pl.dens, 1.0f);
+          }
+        }
+      }
+    }
+    // next node in depth-first order
+    for (;;) {
+      if (level == top) return;
+      if ((idx & kmask) != kmask && idx + 1 < lcnt[level]) {
+        ++idx;
+        break;
+      }
+      ++level;
+      idx >>= log2K;
+    }
+  }
+}
+
+
+// aabbIntersection (sharedCode.h:151-185).
+__device__ inline void aabb_intersection(float3 o, float3 d, float tmin, float tmax, float3 lb, float3 rt,
+                                         float &te, float &tx) {
+  float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+  float t2 = (rt.x - o.x) * ix, t4 = (rt.y - o.y) * iy, t6 = (rt.z - o.z) * iz;
+  float t1 = (lb.x - o.x) * ix, t3 = (lb.y - o.y) * iy, t5 = (lb.z - o.z) * iz;
+  te = fmaxf(fmaxf(fminf(t1, t2), fminf(t3, t4)), fminf(t5, t6));
+  tx = fminf(fminf(fmaxf(t1, t2), fmaxf(t3, t4)), fmaxf(t5, t6));
+  te = fmaxf(te, tmin);
+  tx = fminf(tx, tmax);
+}
+
+__device__ inline float3 colour_of(const RenderArgs &A, const Payload &pl) {
+  float3 c;
+  if (A.visAttr) {
+    c.x = aa_powf(pl.r / pl.dens, 1.0f / 2.2f);
+    c.y = aa_powf(pl.g / pl.dens, 1.0f / 2.2f);
+    c.z = aa_powf(pl.b / pl.dens, 1.0f / 2.2f);
+  } else {
+    float4 t = tf4(A.cmap, A.cm_n, pl.dens);
+    c = make_float3(t.x, t.y, t.z);
+  }
+  return c;
+}
+
+// ParticleTracker::track (deviceCodeRBF.hlsl:58-203).  `i` is the step
+__device__ inline uint64_t wave_sum(uint32_t v) {
+  uint64_t s = v;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
+}
+
+// One lane per pixel, spp samples in sequence; 256-thread blocks cover
+// 16x16 pixels, each wave an 8x8 tile.
+
+struct SegQuery {
+  float ax, ay, az; // segment start
+  float dx, dy, dz; // unit direction
+  float ix, iy, iz; // 1/d with zero components replaced by +-1e30
+  float len;        // segment length
+  float m;          // conservative margin for rounding of o + t*d and |P-x|
+};
+
+__device__ inline float safe_inv(float d) { return 1.0f / (fabsf(d) < 1e-30f ? copysignf(1e-30f, d) : d); }
+
+__device__ inline bool seg_box(const SegQuery &s, float4 lo, float4 hi) {
+  if (!(lo.x <= hi.x)) return false; // empty node
+  float t1 = (lo.x - s.m - s.ax) * s.ix, t2 = (hi.x + s.m - s.ax) * s.ix;
+  float nr = fminf(t1, t2), fr = fmaxf(t1, t2);
+  t1 = (lo.y - s.m - s.ay) * s.iy;
+  t2 = (hi.y + s.m - s.ay) * s.iy;
+  nr = fmaxf(nr, fminf(t1, t2));
+  fr = fminf(fr, fmaxf(t1, t2));
+  t1 = (lo.z - s.m - s.az) * s.iz;
+  t2 = (hi.z + s.m - s.az) * s.iz;
+  nr = fmaxf(nr, fminf(t1, t2));
+  fr = fminf(fr, fmaxf(t1, t2));
+  return fmaxf(nr, 0.0f) <= fminf(fr, s.len);
+}
+
+
+// ---- batched walk ------------------------------------------------------------
+// Depth-first like the stackless walk above, but a hit node tests all K of
+// its children with independent loads (one memory round trip per level
+// instead of one per child).  The hit mask of each level still to be
+// visited is kept in a per-lane LDS column (stk[level * STRIDE]); children
+// are taken lowest bit first, so leaves are still visited in ascending
+// order.  Level-1 hits hand their K leaves to `leaves(base, m)`.
+
+constexpr uint32_t kStackLevels = 16;
+
+template <int STRIDE, bool STATS, class BoxTest, class Leaves>
+__device__ inline void walk_batched(const RenderArgs &A, uint16_t *stk, BoxTest test, Leaves leaves, Counters &c) {
+  const float4 *__restrict__ N = A.bvh.nodes;
+  const uint32_t log2K = A.bvh.log2K, K = 1u << log2K, top = A.bvh.top;
+  const float4 *root = N + g_lvl_ofs[top];
+  if (STATS) c.nodes++;
+  if (!test(root[0], root[1])) return;
+  if (top == 1) {
+    leaves(0u, min(K, g_lvl_cnt[0]));
+    return;
+  }
+  auto group = [&](uint32_t lvl, uint32_t base) -> uint32_t {
+    const float4 *lv = N + g_lvl_ofs[lvl] + 2 * (uint64_t)base;
+    uint32_t m = min(K, g_lvl_cnt[lvl] - base), mask = 0;
+    if (STATS) c.nodes += m;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k)
+      if (k < m && test(lv[2 * k], lv[2 * k + 1])) mask |= 1u << k;
+    for (uint32_t k = 8; k < m; ++k) // K = 16
+      if (test(lv[2 * k], lv[2 * k + 1])) mask |= 1u << k;
+    return mask;
+  };
+  uint32_t lvl = top - 1, base = 0;
+  uint32_t mask = group(lvl, 0);
+  for (;;) {
+    if (mask == 0) {
+      if (lvl == top - 1) return;
+      ++lvl;
+      mask = stk[lvl * STRIDE];
+      base = ((base >> log2K) >> log2K) << log2K;
+      continue;
+    }
+    uint32_t bit = (uint32_t)__ffs(mask) - 1u;
+    mask &= mask - 1u;
+    uint32_t node = base + bit;
+    if (lvl == 1) {
+      uint32_t lb = node << log2K;
+      if (leaves(lb, min(K, g_lvl_cnt[0] - lb))) return;
+      continue;
+    }
+    stk[lvl * STRIDE] = (uint16_t)mask;
+    --lvl;
+    base = node << log2K;
+    mask = group(lvl, base);
+  }
+}
+
+// Point query with the batched walk; same covering set, order and
+// arithmetic as point_query.
+template <int STRIDE, bool STATS>
+__device__ inline void point_query_b(const RenderArgs &A, uint16_t *stk, float x, float y, float z, Payload &pl,
+                                     Counters &c) {
+  pl.count = 0;
+  pl.dens = 0.0f;
+  pl.r = pl.g = pl.b = 0.0f;
+  if (STATS) c.queries++;
+  auto test = [&](const float4 &lo, const float4 &hi) { return inside(lo, hi, x, y, z); };
+  auto particle = [&](uint32_t p, const float4 &q) {
+    float d = aa_dist3(q.x, q.y, q.z, x, y, z);
+    if (d < q.w) {
+      float4 k = A.pc[p];
+      float wa = aa_rbf(d, q.w) * k.x;
+      if (STATS) c.covering++;
+      pl.count += 1;
+      pl.dens += wa;
+      pl.r += k.y * wa;
+      pl.g += k.z * wa;
+      pl.b += k.w * wa;
+      if (!A.visAttr) pl.dens = fminf(pl.dens, 1.0f);
+    }
+  };
+  auto leaves = [&](uint32_t lb, uint32_t m) -> bool {
+    if (A.ppl == 1) {
+      float4 q[8];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k)
+        if (k < m) q[k] = A.pr[lb + k];
+      if (STATS) c.particles += m;
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k)
+        if (k < m) particle(lb + k, q[k]);
+      for (uint32_t k = 8; k < m; ++k) particle(lb + k, A.pr[lb + k]);
+      return false;
+    }
+    const float4 *L0 = A.bvh.nodes + g_lvl_ofs[0];
+    for (uint32_t k = 0; k < m; ++k) {
+      uint32_t leaf = lb + k;
+      if (STATS) c.leaves++;
+      if (!inside(L0[2 * leaf], L0[2 * leaf + 1], x, y, z)) continue;
+      uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
+      for (uint32_t p = p0; p < p1; ++p) {
+        if (STATS) c.particles++;
+        particle(p, A.pr[p]);
+      }
+    }
+    return false;
+  };
+  walk_batched<STRIDE, STATS>(A, stk, test, leaves, c);
+}
+
+} // namespace
+} // namespace aarbf

@@ -1,0 +1,690 @@
+// wavefront.hip -- the default RBF render schedule: wavefront rounds of a
+// gather kernel and a step kernel over a pool of per-pixel ray states.
+//
+// Every pixel owns one ray slot; a slot walks through the pixel's spp
+// samples (primary ray, optional NEE shadow ray) exactly as
+// ParticleRBFRayGen does (deviceCodeRBF.hlsl:206-336), so images are bit for
+// bit those of the one-lane-per-pixel kernel.  What changes is how a
+// tracker step finds the particles covering its point:
+//
+//  * wf_gather (one lane per ray that needs it) walks the BVH once for the
+//    rest of the ray and keeps the kCand particles whose sphere -- widened
+//    by a rounding margin -- the ray enters first, with their conservative
+//    entry/exit parameters [tin, tout].  The segment ends where the first
+//    candidate that did not fit would enter, so all particles that can
+//    cover any step before the segment end are in the list.  The list is
+//    stored in ascending particle order plus a permutation by tin.
+//  * wf_step (one lane per active ray) advances its tracker for up to a
+//    step budget.  The active set {k : tin_k <= t <= tout_k} is kept as a
+//    bitmask updated by entry/exit events, so a step in empty space costs
+//    only the RNG draw, the logarithm and two comparisons; particles in the
+//    active set get the exact |P - x| < r test in ascending order, which
+//    reproduces the point query's covering set and summation order.
+//    Where more than kCand particles crowd the start of a segment the
+//    segment is marked dense and its steps use batched point queries.
+//
+// Rays that run past their segment, start a shadow ray or a new sample go
+// back to the gather queue; the host alternates the two kernels until no
+// ray is left.  Both kernels are dense (every lane has the same kind of
+// work), which is what the one-kernel schedule could not achieve.
+#include "render_common.h"
+
+namespace aarbf {
+namespace {
+
+constexpr uint32_t kCand = 16;
+constexpr int kGBlock = 128; // gather kernel block
+constexpr int kSBlock = 64;  // step kernel block (one wave)
+
+constexpr uint32_t F_SHADOW = 1u, F_PENDING = 2u, F_DENSE = 4u;
+
+// Ray-slot state, structure of arrays (one entry per slot).
+struct WFState {
+  float4 *od0;  // o.xyz, t
+  float4 *od1;  // d.xyz, tmax
+  float4 *acc;  // running mean of the pixel
+  float4 *alb;  // primary albedo
+  uint4 *u;     // trng, i, s | flags << 16, seg_n
+  float4 *f;    // LHS, RHS, shw, r2
+  float4 *seg;  // seg_t0, seg_end, rx, unused
+  uint32_t *cidx; // [kCand][nslots] candidate particle index (ascending)
+  float *ctin;    // [kCand][nslots]
+  float *ctout;   // [kCand][nslots]
+  uint8_t *ceo;   // [kCand][nslots] candidate positions ordered by tin
+  uint32_t nslots;
+  uint32_t tiles_x;
+};
+
+struct Lane {
+  float3 o, d;
+  float t, tmax;
+  float4 acc, alb;
+  uint32_t trng, i, s, flags, seg_n;
+  float LHS, RHS, shw, r2;
+  float seg_t0, seg_end, rx;
+};
+
+__device__ inline void load_lane(const WFState &S, uint32_t slot, Lane &L) {
+  float4 a = S.od0[slot], b = S.od1[slot];
+  L.o = make_float3(a.x, a.y, a.z);
+  L.t = a.w;
+  L.d = make_float3(b.x, b.y, b.z);
+  L.tmax = b.w;
+  L.acc = S.acc[slot];
+  L.alb = S.alb[slot];
+  uint4 u = S.u[slot];
+  L.trng = u.x;
+  L.i = u.y;
+  L.s = u.z & 0xffffu;
+  L.flags = u.z >> 16;
+  L.seg_n = u.w;
+  float4 f = S.f[slot];
+  L.LHS = f.x;
+  L.RHS = f.y;
+  L.shw = f.z;
+  L.r2 = f.w;
+  float4 g = S.seg[slot];
+  L.seg_t0 = g.x;
+  L.seg_end = g.y;
+  L.rx = g.z;
+}
+
+__device__ inline void save_lane(const WFState &S, uint32_t slot, const Lane &L) {
+  S.od0[slot] = make_float4(L.o.x, L.o.y, L.o.z, L.t);
+  S.od1[slot] = make_float4(L.d.x, L.d.y, L.d.z, L.tmax);
+  S.acc[slot] = L.acc;
+  S.alb[slot] = L.alb;
+  S.u[slot] = make_uint4(L.trng, L.i, (L.s & 0xffffu) | (L.flags << 16), L.seg_n);
+  S.f[slot] = make_float4(L.LHS, L.RHS, L.shw, L.r2);
+  S.seg[slot] = make_float4(L.seg_t0, L.seg_end, L.rx, 0.0f);
+}
+
+__device__ inline void slot_pixel(const WFState &S, uint32_t slot, int &px, int &py) {
+  uint32_t tile = slot >> 6, w = slot & 63u;
+  px = (int)((tile % S.tiles_x) * 8u + (w & 7u));
+  py = (int)((tile / S.tiles_x) * 8u + (w >> 3));
+}
+
+// Set-up of sample L.s (deviceCodeRBF.hlsl:207-280).  Returns true when the
+// primary ray enters the clip box and must be tracked.
+__device__ inline bool start_sample(const RenderArgs &A, int px, int py, Lane &L) {
+  const bool marching = A.blue != 0;
+  uint32_t fid = (A.frameID + L.s) & 0xffffu;
+  uint32_t rng = aa_rng_seed(fid, (uint32_t)px, (uint32_t)py, (uint32_t)A.W);
+  float sx = ((float)px + 0.5f) / (float)A.W;
+  float sy = ((float)py + 0.5f) / (float)A.H;
+  float3 dir;
+  dir.x = (A.d00.x + sx * A.du.x) + sy * A.dv.x;
+  dir.y = (A.d00.y + sx * A.du.y) + sy * A.dv.y;
+  dir.z = (A.d00.z + sx * A.du.z) + sy * A.dv.z;
+  float len = sqrtf((dir.x * dir.x + dir.y * dir.y) + dir.z * dir.z);
+  dir.x = dir.x / len;
+  dir.y = dir.y / len;
+  dir.z = dir.z / len;
+  float tenter, texit;
+  aabb_intersection(A.pos, dir, 0.0f, 10000.0f, A.lb, A.rt, tenter, texit);
+  float r0;
+  if (marching) {
+    int gx = (int)(fid % 8u), gy = (int)((fid / 8u) % 16u);
+    int cx = gx * 256 + px % 256, cy = gy * 256 + py % 256;
+    uint8_t v = 0;
+    if (cx < A.stbn_w && cy < A.stbn_h) v = A.stbn[4 * ((uint64_t)cy * A.stbn_w + cx)];
+    r0 = (float)v / 255.0f;
+  } else {
+    r0 = aa_lcg_next(&rng);
+  }
+  L.trng = marching ? 0u : rng;
+  float r1 = aa_lcg_next(&rng);
+  L.r2 = aa_lcg_next(&rng);
+  L.rx = r0;
+  L.i = 0;
+  L.alb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  L.shw = 0.0f;
+  L.flags = 0;
+  if (!(tenter < texit)) return false;
+  L.o = A.pos;
+  L.d = dir;
+  L.tmax = texit;
+  L.t = marching ? tenter + A.unit * r1 : 0.0f;
+  L.LHS = 0.0f;
+  L.RHS = marching ? -aa_logf(1.0f - r0 * 0.99f) : 0.0f;
+  return true;
+}
+
+// Shading, compositing over black and the running mean (':316-328').
+template <bool STATS>
+__device__ inline void finish_sample(const RenderArgs &A, Lane &L, float vis, Counters &c) {
+  float4 color = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (L.alb.w == 1.0f) {
+    color.x = (L.alb.x * vis) * (1.0f - A.amb) + L.alb.x * A.amb;
+    color.y = (L.alb.y * vis) * (1.0f - A.amb) + L.alb.y * A.amb;
+    color.z = (L.alb.z * vis) * (1.0f - A.amb) + L.alb.z * A.amb;
+    color.w = 1.0f;
+  }
+  if (STATS) {
+    c.samples++;
+    if (L.i >= A.max_depth) c.exhausted++;
+  }
+  float ra = color.w + 1.0f * (1.0f - color.w);
+  if (ra != 0.0f) {
+    color.x = (color.x * color.w + (0.0f * 1.0f) * (1.0f - color.w)) / ra;
+    color.y = (color.y * color.w + (0.0f * 1.0f) * (1.0f - color.w)) / ra;
+    color.z = (color.z * color.w + (0.0f * 1.0f) * (1.0f - color.w)) / ra;
+    color.w = ra;
+  }
+  uint32_t aid = (A.accumID + L.s) & 0xffffu;
+  float w1 = 1.0f / (float)aid;
+  float w0 = (float)((int)aid - 1) / (float)aid;
+  float4 fin;
+  fin.x = w1 * color.x + w0 * L.acc.x;
+  fin.y = w1 * color.y + w0 * L.acc.y;
+  fin.z = w1 * color.z + w0 * L.acc.z;
+  fin.w = w1 * color.w + w0 * L.acc.w;
+  L.acc = fin;
+  L.s += 1;
+}
+
+// Starts samples until one needs tracking (true) or the pixel is done, in
+// which case its outputs are written (false).
+template <bool STATS>
+__device__ inline bool next_ray(const RenderArgs &A, int px, int py, Lane &L, Counters &c) {
+  while (L.s < A.spp) {
+    if (start_sample(A, px, py, L)) {
+      L.seg_end = -AA_INF_F;
+      return true;
+    }
+    finish_sample<STATS>(A, L, 1.0f, c);
+  }
+  uint64_t ofs = (uint64_t)px + (uint64_t)A.W * py;
+  A.accum[ofs] = L.acc;
+  A.image[ofs] = A.showNoise ? make_float4(L.rx, L.rx, L.rx, 1.0f) : L.acc;
+  return false;
+}
+
+__device__ inline void push(uint32_t *list, uint32_t *count, uint32_t slot) {
+  // one atomic per wave for all lanes pushing to this list
+  uint64_t mask = __ballot(1);
+  int lane = threadIdx.x & 63;
+  int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  list[base + (uint32_t)__popcll(mask & lt)] = slot;
+}
+
+template <bool STATS>
+__device__ inline void flush_stats(const RenderArgs &A, const Counters &c) {
+  if (!STATS) return;
+  uint32_t v[8] = {c.samples, c.steps, c.queries, c.nodes, c.leaves, c.particles, c.covering, c.exhausted};
+  for (int k = 0; k < 8; ++k) {
+    uint64_t sum = wave_sum(v[k]);
+    if ((threadIdx.x & 63) == 0 && sum) atomicAdd((unsigned long long *)&A.stats[k], (unsigned long long)sum);
+  }
+}
+
+// ---- initial rays ------------------------------------------------------------
+template <bool STATS>
+__global__ __launch_bounds__(256) void wf_init(RenderArgs A, WFState S, uint32_t *listA, uint32_t *listG,
+                                               uint32_t *counts) {
+  uint32_t slot = blockIdx.x * 256u + threadIdx.x;
+  Counters c = {};
+  bool push_it = false;
+  if (slot < S.nslots) {
+    int px, py;
+    slot_pixel(S, slot, px, py);
+    if (px < A.W && py < A.H) {
+      Lane L;
+      L.acc = A.accum[(uint64_t)px + (uint64_t)A.W * py];
+      L.s = 0;
+      L.seg_n = 0;
+      L.seg_t0 = 0.0f;
+      if (next_ray<STATS>(A, px, py, L, c)) {
+        save_lane(S, slot, L);
+        push_it = true;
+      }
+    }
+  }
+  if (push_it) {
+    push(listA, &counts[0], slot);
+    push(listG, &counts[1], slot);
+  }
+  flush_stats<STATS>(A, c);
+}
+
+// ---- gather --------------------------------------------------------------------
+template <bool STATS>
+__global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, const uint32_t *__restrict__ listG,
+                                                     const uint32_t *__restrict__ countG) {
+  __shared__ uint32_t s_idx[kCand * kGBlock];
+  __shared__ float s_tin[kCand * kGBlock];
+  __shared__ float s_tout[kCand * kGBlock];
+  __shared__ uint16_t s_stack[kStackLevels * kGBlock];
+  load_levels(A);
+  const uint32_t n_g = *countG;
+  uint32_t gid = blockIdx.x * kGBlock + threadIdx.x;
+  Counters c = {};
+  if (gid < n_g) {
+    const uint32_t slot = listG[gid];
+    uint32_t *gi = s_idx + threadIdx.x;
+    float *gtin = s_tin + threadIdx.x, *gtout = s_tout + threadIdx.x;
+    uint16_t *stk = s_stack + threadIdx.x;
+    float4 a0 = S.od0[slot], a1 = S.od1[slot];
+    const float3 o = make_float3(a0.x, a0.y, a0.z), d = make_float3(a1.x, a1.y, a1.z);
+    const float t0 = a0.w, tmax = a1.w;
+    uint4 u = S.u[slot];
+    uint32_t flags = u.z >> 16;
+    SegQuery q;
+    q.ax = o.x + t0 * d.x;
+    q.ay = o.y + t0 * d.y;
+    q.az = o.z + t0 * d.z;
+    q.dx = d.x;
+    q.dy = d.y;
+    q.dz = d.z;
+    q.ix = safe_inv(d.x);
+    q.iy = safe_inv(d.y);
+    q.iz = safe_inv(d.z);
+    const float m = 2e-5f * (1.0f + fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) + fmaxf(fabsf(t0), fabsf(tmax)));
+    q.m = m;
+    q.len = tmax - t0 + 4.0f * m;
+    uint32_t n = 0, maxpos = 0;
+    float maxtin = -AA_INF_F, t_cut = AA_INF_F;
+    if (STATS) c.queries++;
+    // conservative entry/exit of the particle's sphere widened by m
+    auto take = [&](uint32_t p, const float4 &P) {
+      if (!(P.w > 0.0f)) return;
+      float vx = P.x - o.x, vy = P.y - o.y, vz = P.z - o.z;
+      float tc = vx * d.x + vy * d.y + vz * d.z;
+      float hx = vx - tc * d.x, hy = vy - tc * d.y, hz = vz - tc * d.z;
+      float rr = P.w + m;
+      float w2 = rr * rr - (hx * hx + hy * hy + hz * hz);
+      if (!(w2 >= 0.0f)) return;
+      float w = sqrtf(w2);
+      float tin = tc - w - m, tout = tc + w + m;
+      if (!(tout >= t0) || !(tin <= tmax) || !(tin < t_cut)) {
+        if (tout >= t0 && tin <= tmax) t_cut = fminf(t_cut, tin);
+        return;
+      }
+      if (n < kCand) {
+        gi[n * kGBlock] = p;
+        gtin[n * kGBlock] = tin;
+        gtout[n * kGBlock] = tout;
+        if (tin > maxtin) {
+          maxtin = tin;
+          maxpos = n;
+        }
+        ++n;
+        return;
+      }
+      if (tin < maxtin) { // evict the latest-entering candidate
+        t_cut = fminf(t_cut, maxtin);
+        gi[maxpos * kGBlock] = p;
+        gtin[maxpos * kGBlock] = tin;
+        gtout[maxpos * kGBlock] = tout;
+        maxtin = -AA_INF_F;
+        for (uint32_t k = 0; k < kCand; ++k) {
+          float v = gtin[k * kGBlock];
+          if (v > maxtin) {
+            maxtin = v;
+            maxpos = k;
+          }
+        }
+      } else {
+        t_cut = fminf(t_cut, tin);
+      }
+      q.len = fminf(tmax, t_cut + 4.0f * m) - t0;
+    };
+    auto test = [&](const float4 &lo, const float4 &hi) { return seg_box(q, lo, hi); };
+    auto leaves = [&](uint32_t lb, uint32_t mcount) -> bool {
+      if (A.ppl == 1) {
+        float4 P[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k)
+          if (k < mcount) P[k] = A.pr[lb + k];
+        if (STATS) c.particles += mcount;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k)
+          if (k < mcount) take(lb + k, P[k]);
+        for (uint32_t k = 8; k < mcount; ++k) take(lb + k, A.pr[lb + k]);
+        return false;
+      }
+      const float4 *L0 = A.bvh.nodes + g_lvl_ofs[0];
+      for (uint32_t k = 0; k < mcount; ++k) {
+        uint32_t leaf = lb + k;
+        if (STATS) c.leaves++;
+        if (!seg_box(q, L0[2 * leaf], L0[2 * leaf + 1])) continue;
+        uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
+        for (uint32_t p = p0; p < p1; ++p) {
+          if (STATS) c.particles++;
+          take(p, A.pr[p]);
+        }
+      }
+      return false;
+    };
+    walk_batched<kGBlock, STATS>(A, stk, test, leaves, c);
+
+    float seg_end;
+    if (t_cut <= t0 + 2.0f * A.unit) { // crowded: point queries for a short stretch
+      flags |= F_DENSE;
+      seg_end = t0 + 4.0f * A.unit;
+      n = 0;
+    } else {
+      flags &= ~F_DENSE;
+      seg_end = t_cut;
+      // ascending particle order (insertion sort, n <= kCand)
+      for (uint32_t a = 1; a < n; ++a) {
+        uint32_t pi = gi[a * kGBlock];
+        float ti = gtin[a * kGBlock], to = gtout[a * kGBlock];
+        uint32_t b = a;
+        while (b > 0 && gi[(b - 1) * kGBlock] > pi) {
+          gi[b * kGBlock] = gi[(b - 1) * kGBlock];
+          gtin[b * kGBlock] = gtin[(b - 1) * kGBlock];
+          gtout[b * kGBlock] = gtout[(b - 1) * kGBlock];
+          --b;
+        }
+        gi[b * kGBlock] = pi;
+        gtin[b * kGBlock] = ti;
+        gtout[b * kGBlock] = to;
+      }
+      // entry order: position of rank r in tin order (ties by position)
+      for (uint32_t k = 0; k < n; ++k) {
+        float tk = gtin[k * kGBlock];
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < n; ++j) {
+          float tj = gtin[j * kGBlock];
+          r += (tj < tk) || (tj == tk && j < k);
+        }
+        S.ceo[(uint64_t)r * S.nslots + slot] = (uint8_t)k;
+        S.cidx[(uint64_t)k * S.nslots + slot] = gi[k * kGBlock];
+        S.ctin[(uint64_t)k * S.nslots + slot] = tk;
+        S.ctout[(uint64_t)k * S.nslots + slot] = gtout[k * kGBlock];
+      }
+    }
+    S.u[slot] = make_uint4(u.x, u.y, (u.z & 0xffffu) | (flags << 16), n);
+    float4 g = S.seg[slot];
+    S.seg[slot] = make_float4(t0, seg_end, g.z, g.w);
+  }
+  flush_stats<STATS>(A, c);
+}
+
+// ---- step ------------------------------------------------------------------------
+template <bool STATS>
+__global__ __launch_bounds__(kSBlock) void wf_step(RenderArgs A, WFState S, const uint32_t *__restrict__ listA,
+                                                   const uint32_t *__restrict__ countA, uint32_t *__restrict__ outA,
+                                                   uint32_t *__restrict__ outG, uint32_t *__restrict__ counts,
+                                                   uint32_t step_cap) {
+  __shared__ uint32_t s_idx[kCand * kSBlock];
+  __shared__ float s_tin[kCand * kSBlock];
+  __shared__ float s_tout[kCand * kSBlock];
+  __shared__ uint8_t s_eo[kCand * kSBlock];
+  __shared__ uint16_t s_stack[kStackLevels * kSBlock];
+  load_levels(A);
+  const bool marching = A.blue != 0;
+  const uint32_t n_a = *countA;
+  uint32_t gid = blockIdx.x * kSBlock + threadIdx.x;
+  Counters c = {};
+  int outcome = 0; // 0 nothing, 1 keep active, 2 active + gather
+  uint32_t slot = 0;
+  if (gid < n_a) {
+    slot = listA[gid];
+    int px, py;
+    slot_pixel(S, slot, px, py);
+    Lane L;
+    load_lane(S, slot, L);
+    uint32_t *li = s_idx + threadIdx.x;
+    float *ltin = s_tin + threadIdx.x, *ltout = s_tout + threadIdx.x;
+    uint8_t *leo = s_eo + threadIdx.x;
+    uint16_t *stk = s_stack + threadIdx.x;
+    uint32_t n = (L.flags & F_DENSE) ? 0u : L.seg_n;
+    for (uint32_t k = 0; k < n; ++k) {
+      li[k * kSBlock] = S.cidx[(uint64_t)k * S.nslots + slot];
+      ltin[k * kSBlock] = S.ctin[(uint64_t)k * S.nslots + slot];
+      ltout[k * kSBlock] = S.ctout[(uint64_t)k * S.nslots + slot];
+      leo[k * kSBlock] = S.ceo[(uint64_t)k * S.nslots + slot];
+    }
+    // active set at the current t
+    uint32_t act = 0, eptr = 0;
+    float t_enter = AA_INF_F, t_exit = AA_INF_F;
+    auto enter = [&](float t) {
+      while (eptr < n) {
+        uint32_t k = leo[eptr * kSBlock];
+        if (!(ltin[k * kSBlock] <= t)) break;
+        float to = ltout[k * kSBlock];
+        if (t <= to) {
+          act |= 1u << k;
+          t_exit = fminf(t_exit, to);
+        }
+        ++eptr;
+      }
+      t_enter = eptr < n ? ltin[leo[eptr * kSBlock] * kSBlock] : AA_INF_F;
+    };
+    auto leave = [&](float t) {
+      t_exit = AA_INF_F;
+      uint32_t m = act;
+      while (m) {
+        uint32_t k = (uint32_t)__ffs(m) - 1u;
+        m &= m - 1u;
+        float to = ltout[k * kSBlock];
+        if (to < t)
+          act &= ~(1u << k);
+        else
+          t_exit = fminf(t_exit, to);
+      }
+    };
+    enter(L.t);
+
+    uint32_t steps = 0;
+    for (;;) {
+      bool pending = (L.flags & F_PENDING) != 0;
+      bool end = false;
+      if (!pending) {
+        if (!(L.i < A.max_depth)) {
+          end = true;
+        } else if (steps >= step_cap) {
+          outcome = 1;
+          break;
+        } else {
+          ++steps;
+          if (STATS) c.steps++;
+          if (marching) {
+            L.t = L.t + A.unit;
+          } else {
+            float xi = aa_lcg_next(&L.trng);
+            L.t = L.t - aa_logf(1.0f - xi) * A.unit;
+          }
+          if (L.t >= L.tmax) end = true;
+          else L.flags |= F_PENDING;
+        }
+      }
+      if (!end) {
+        // the point query at L.t
+        if (!(L.t < L.seg_end)) {
+          outcome = 2;
+          break;
+        }
+        Payload pl;
+        float x = L.o.x + L.t * L.d.x, y = L.o.y + L.t * L.d.y, z = L.o.z + L.t * L.d.z;
+        if (L.flags & F_DENSE) {
+          point_query_b<kSBlock, STATS>(A, stk, x, y, z, pl, c);
+        } else {
+          if (L.t >= t_enter) enter(L.t);
+          if (L.t > t_exit) leave(L.t);
+          pl.count = 0;
+          pl.dens = 0.0f;
+          pl.r = pl.g = pl.b = 0.0f;
+          uint32_t m = act;
+          while (m) {
+            uint32_t k = (uint32_t)__ffs(m) - 1u;
+            m &= m - 1u;
+            uint32_t p = li[k * kSBlock];
+            float4 qq = A.pr[p];
+            if (STATS) c.particles++;
+            float dd = aa_dist3(qq.x, qq.y, qq.z, x, y, z);
+            if (dd < qq.w) {
+              float4 kk = A.pc[p];
+              float wa = aa_rbf(dd, qq.w) * kk.x;
+              if (STATS) c.covering++;
+              pl.count += 1;
+              pl.dens += wa;
+              pl.r += kk.y * wa;
+              pl.g += kk.z * wa;
+              pl.b += kk.w * wa;
+              if (!A.visAttr) pl.dens = fminf(pl.dens, 1.0f);
+            }
+          }
+        }
+        L.flags &= ~F_PENDING;
+        const bool shadow = (L.flags & F_SHADOW) != 0;
+        if (marching) {
+          if (pl.count > 0 && pl.dens > 0.0f) {
+            float D = tf_r(A.dmap, A.dm_n, pl.dens);
+            L.LHS += D;
+            if (L.LHS > L.RHS && !shadow) {
+              float3 col = colour_of(A, pl);
+              L.alb = make_float4(col.x, col.y, col.z, 1.0f);
+              end = true;
+            } else if (shadow) {
+              L.shw += 1.0f - aa_expf(-D);
+              if (L.shw >= 1.0f) {
+                L.shw = 1.0f;
+                end = true;
+              }
+            }
+          }
+        } else if (pl.count > 0) {
+          float D = tf_r(A.dmap, A.dm_n, pl.dens);
+          float xi2 = aa_lcg_next(&L.trng);
+          if (xi2 < D) {
+            if (shadow) {
+              L.shw = 1.0f;
+            } else {
+              float3 col = colour_of(A, pl);
+              L.alb = make_float4(col.x, col.y, col.z, 1.0f);
+            }
+            end = true;
+          }
+        }
+        if (!end) {
+          ++L.i;
+          continue;
+        }
+      }
+      // ---- the ray ended
+      L.flags &= ~F_PENDING;
+      if (!(L.flags & F_SHADOW) && L.alb.w > 0.0f && A.amb != 1.0f) {
+        // NEE shadow ray (':288-311'); the step counter carries over
+        float3 so = make_float3(A.pos.x + L.d.x * L.t, A.pos.y + L.d.y * L.t, A.pos.z + L.d.z * L.t);
+        float ste, stx;
+        aabb_intersection(so, A.light, 0.0f, 10000.0f, A.lb, A.rt, ste, stx);
+        L.o = so;
+        L.d = A.light;
+        L.tmax = stx;
+        L.flags = F_SHADOW;
+        L.shw = 0.0f;
+        L.LHS = 0.0f;
+        L.t = marching ? -A.unit * L.r2 : 0.0f;
+        L.seg_end = -AA_INF_F;
+        outcome = 2;
+        break;
+      }
+      float vis = (L.flags & F_SHADOW) ? 1.0f - L.shw : 1.0f;
+      finish_sample<STATS>(A, L, vis, c);
+      if (next_ray<STATS>(A, px, py, L, c)) {
+        outcome = 2;
+      } else {
+        outcome = 0;
+      }
+      break;
+    }
+    if (outcome) save_lane(S, slot, L);
+  }
+  // compaction: lanes with an outcome append to the next lists
+  if (outcome) push(outA, &counts[0], slot);
+  if (outcome == 2) push(outG, &counts[1], slot);
+  flush_stats<STATS>(A, c);
+}
+
+__global__ void wf_zero(uint32_t *c, int n) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
+}
+
+} // namespace
+
+// Host driver of the wavefront rounds.  Grows the slot state in the
+// context's scratch pool; reads the two queue lengths back after each round.
+int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
+  const uint32_t tiles_x = (uint32_t)(A.W + 7) / 8, tiles_y = (uint32_t)(A.H + 7) / 8;
+  const uint64_t nslots = (uint64_t)tiles_x * tiles_y * 64u;
+  AARBF_REQUIRE(ctx, nslots < (1ull << 31), "aarbf_render_rbf: image too large");
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  size_t o_od0 = take(16 * nslots), o_od1 = take(16 * nslots), o_acc = take(16 * nslots), o_alb = take(16 * nslots),
+         o_u = take(16 * nslots), o_f = take(16 * nslots), o_seg = take(16 * nslots),
+         o_cidx = take(4 * kCand * nslots), o_ctin = take(4 * kCand * nslots), o_ctout = take(4 * kCand * nslots),
+         o_ceo = take(kCand * nslots), o_l0 = take(4 * nslots), o_l1 = take(4 * nslots), o_l2 = take(4 * nslots),
+         o_l3 = take(4 * nslots), o_cnt = take(64);
+  char *base = (char *)scratch(ctx, off);
+  if (!base) return set_error(ctx, AARBF_ERR_OUT_OF_MEMORY, "aarbf_render_rbf: wavefront state");
+  WFState S;
+  S.od0 = (float4 *)(base + o_od0);
+  S.od1 = (float4 *)(base + o_od1);
+  S.acc = (float4 *)(base + o_acc);
+  S.alb = (float4 *)(base + o_alb);
+  S.u = (uint4 *)(base + o_u);
+  S.f = (float4 *)(base + o_f);
+  S.seg = (float4 *)(base + o_seg);
+  S.cidx = (uint32_t *)(base + o_cidx);
+  S.ctin = (float *)(base + o_ctin);
+  S.ctout = (float *)(base + o_ctout);
+  S.ceo = (uint8_t *)(base + o_ceo);
+  S.nslots = (uint32_t)nslots;
+  S.tiles_x = tiles_x;
+  uint32_t *lists[4] = {(uint32_t *)(base + o_l0), (uint32_t *)(base + o_l1), (uint32_t *)(base + o_l2),
+                        (uint32_t *)(base + o_l3)};
+  uint32_t *cnt = (uint32_t *)(base + o_cnt); // [0..1] current A, G ; [2..3] next A, G
+  const bool st = ctx->stats;
+  wf_zero<<<1, 64, 0, ctx->stream>>>(cnt, 16);
+  uint32_t init_blocks = (uint32_t)((nslots + 255) / 256);
+  if (st)
+    wf_init<true><<<init_blocks, 256, 0, ctx->stream>>>(A, S, lists[0], lists[1], cnt);
+  else
+    wf_init<false><<<init_blocks, 256, 0, ctx->stream>>>(A, S, lists[0], lists[1], cnt);
+  AARBF_CHECK_LAUNCH(ctx, "wf_init");
+  uint32_t h[2];
+  AARBF_HIP(ctx, hipMemcpyAsync(h, cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
+  AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  int cur = 0; // lists[cur] = A, lists[cur+1] = G; the other pair receives
+  uint32_t nA = h[0], nG = h[1];
+  while (nA > 0) {
+    uint32_t *lA = lists[cur], *lG = lists[cur + 1], *nA2 = lists[2 - cur], *nG2 = lists[3 - cur];
+    uint32_t *cA = cnt + (cur ? 2 : 0), *cN = cnt + (cur ? 0 : 2);
+    if (nG > 0) {
+      uint32_t blocks = (nG + kGBlock - 1) / kGBlock;
+      if (st)
+        wf_gather<true><<<blocks, kGBlock, 0, ctx->stream>>>(A, S, lG, cA + 1);
+      else
+        wf_gather<false><<<blocks, kGBlock, 0, ctx->stream>>>(A, S, lG, cA + 1);
+    }
+    wf_zero<<<1, 64, 0, ctx->stream>>>(cN, 2);
+    // fewer live rays -> more steps per round, so the tail costs few rounds
+    uint32_t cap = (uint32_t)std::min<uint64_t>(16384, std::max<uint64_t>(256, 256ull * nslots / (4ull * nA)));
+    uint32_t blocks = (nA + kSBlock - 1) / kSBlock;
+    if (st)
+      wf_step<true><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, cap);
+    else
+      wf_step<false><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, cap);
+    AARBF_CHECK_LAUNCH(ctx, "wf_step");
+    AARBF_HIP(ctx, hipMemcpyAsync(h, cN, 8, hipMemcpyDeviceToHost, ctx->stream));
+    AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    nA = h[0];
+    nG = h[1];
+    cur = 2 - cur;
+  }
+  return AARBF_OK;
+}
+
+} // namespace aarbf
