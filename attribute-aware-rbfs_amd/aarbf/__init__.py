@@ -228,6 +228,14 @@ class Context:
         self.check(self.lib.aarbf_bvh_read_level(self.h, bvh, level, _fptr(out)), "aarbf_bvh_read_level")
         return out
 
+    def bvh_read_groups(self, bvh: int, level: int) -> np.ndarray:
+        """Raw quantised groups of `level` as uint32 [n, 20]."""
+        counts = self.bvh_levels(bvh)
+        out = np.empty((counts[level], 20), dtype=np.uint32)
+        self.check(self.lib.aarbf_bvh_read_groups(self.h, bvh, level, out.ctypes.data_as(C.POINTER(C.c_uint32))),
+                   "aarbf_bvh_read_groups")
+        return out
+
     # -- render --------------------------------------------------------------------
     def render(self, rg: RayGenData, pc: PushConstants, width: int, height: int, spp: int = 1):
         self.check(self.lib.aarbf_render_rbf_spp(self.h, C.byref(rg), C.byref(pc), width, height, spp),

@@ -110,6 +110,7 @@ SIGNATURES = {
     "aarbf_bvh_destroy": (C.c_int, [P, u64]),
     "aarbf_bvh_levels": (C.c_int, [P, u64, C.POINTER(u32), C.POINTER(u32), u32]),
     "aarbf_bvh_read_level": (C.c_int, [P, u64, u32, FP]),
+    "aarbf_bvh_read_groups": (C.c_int, [P, u64, u32, C.POINTER(u32)]),
     "aarbf_render_rbf": (C.c_int, [P, C.POINTER(RayGenData), C.POINTER(PushConstants), C.c_int, C.c_int]),
     "aarbf_render_rbf_spp": (C.c_int, [P, C.POINTER(RayGenData), C.POINTER(PushConstants), C.c_int,
                                        C.c_int, u32]),

@@ -114,6 +114,70 @@ __global__ __launch_bounds__(256) void bvh_level(const float4 *__restrict__ chil
   out[2 * (uint64_t)j + 1] = H;
 }
 
+// Dequantised bound: lo + q * step (two roundings; the traversal evaluates
+// exactly the same expression).
+__device__ inline float deq(float lo, float step, uint32_t q) { return lo + (float)q * step; }
+
+// One QGroup per node of level l >= 1 from the float boxes of levels l and
+// l-1.  Child bounds are rounded outwards and then corrected until the
+// dequantised box contains the float box.
+__global__ __launch_bounds__(256) void bvh_qgroups(const float4 *__restrict__ parent, uint32_t n_parent,
+                                                   const float4 *__restrict__ child, uint32_t n_child, uint32_t K,
+                                                   QGroup *__restrict__ out) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_parent) return;
+  float4 P0 = parent[2 * (uint64_t)j], P1 = parent[2 * (uint64_t)j + 1];
+  const bool pempty = !(P0.x <= P1.x);
+  float plo[3] = {P0.x, P0.y, P0.z}, phi[3] = {P1.x, P1.y, P1.z}, step[3];
+  for (int a = 0; a < 3; ++a) {
+    if (pempty) {
+      plo[a] = 0.0f;
+      step[a] = 1.0f;
+      continue;
+    }
+    float sa = fmaxf((phi[a] - plo[a]) / 255.0f, 1.17549435e-38f);
+    while (deq(plo[a], sa, 255u) < phi[a]) sa = sa * 1.0000005f + 1e-37f;
+    step[a] = sa;
+  }
+  uint32_t qb[6][8];
+  for (uint32_t k = 0; k < 8; ++k) {
+    uint64_t c = (uint64_t)j * K + k;
+    bool empty = pempty || k >= K || c >= n_child;
+    float4 C0 = make_float4(0, 0, 0, 0), C1 = C0;
+    if (!empty) {
+      C0 = child[2 * c];
+      C1 = child[2 * c + 1];
+      empty = !(C0.x <= C1.x);
+    }
+    if (empty) {
+      for (int a = 0; a < 3; ++a) {
+        qb[a][k] = 255u;
+        qb[3 + a][k] = 0u;
+      }
+      continue;
+    }
+    float clo[3] = {C0.x, C0.y, C0.z}, chi[3] = {C1.x, C1.y, C1.z};
+    for (int a = 0; a < 3; ++a) {
+      float fl = floorf((clo[a] - plo[a]) / step[a]);
+      uint32_t ql = fl <= 0.0f ? 0u : (fl >= 255.0f ? 255u : (uint32_t)fl);
+      while (ql > 0u && deq(plo[a], step[a], ql) > clo[a]) --ql;
+      float fh = ceilf((chi[a] - plo[a]) / step[a]);
+      uint32_t qh = fh <= 0.0f ? 0u : (fh >= 255.0f ? 255u : (uint32_t)fh);
+      while (qh < 255u && deq(plo[a], step[a], qh) < chi[a]) ++qh;
+      qb[a][k] = ql;
+      qb[3 + a][k] = qh;
+    }
+  }
+  QGroup g;
+  g.a = make_float4(plo[0], plo[1], plo[2], step[0]);
+  g.b = make_float4(step[1], step[2], 0.0f, 0.0f);
+  for (int f = 0; f < 6; ++f) {
+    g.q[2 * f] = qb[f][0] | (qb[f][1] << 8) | (qb[f][2] << 16) | (qb[f][3] << 24);
+    g.q[2 * f + 1] = qb[f][4] | (qb[f][5] << 8) | (qb[f][6] << 16) | (qb[f][7] << 24);
+  }
+  out[j] = g;
+}
+
 int build_levels(aarbf_context *ctx, BVH *b, const float *aabbs) {
   uint32_t n0 = b->counts[0];
   bvh_leaves<<<(n0 + 255) / 256, 256, 0, ctx->stream>>>(aabbs, n0, b->nodes + b->level_ofs[0]);
@@ -122,6 +186,13 @@ int build_levels(aarbf_context *ctx, BVH *b, const float *aabbs) {
     bvh_level<<<(n + 255) / 256, 256, 0, ctx->stream>>>(b->nodes + b->level_ofs[l - 1], b->counts[l - 1], b->K,
                                                         b->nodes + b->level_ofs[l], n);
   }
+  if (b->groups)
+    for (uint32_t l = 1; l < b->num_levels; ++l) {
+      uint32_t n = b->counts[l];
+      bvh_qgroups<<<(n + 255) / 256, 256, 0, ctx->stream>>>(b->nodes + b->level_ofs[l], n,
+                                                            b->nodes + b->level_ofs[l - 1], b->counts[l - 1], b->K,
+                                                            b->groups + b->group_ofs[l]);
+    }
   AARBF_CHECK_LAUNCH(ctx, "bvh build");
   return AARBF_OK;
 }
@@ -177,13 +248,24 @@ int aarbf_bvh_build(aarbf_context *ctx, uint64_t aabbs, uint32_t numAABBs, uint3
   b->num_levels = L;
   b->total_nodes = total;
   hipError_t e = hipMalloc((void **)&b->nodes, sizeof(float4) * 2 * total);
+  if (e == hipSuccess && branching <= 8) {
+    uint64_t g = 0;
+    for (uint32_t l = 1; l < L; ++l) {
+      b->group_ofs[l] = g;
+      g += b->counts[l];
+    }
+    b->total_groups = g;
+    e = hipMalloc((void **)&b->groups, sizeof(QGroup) * g);
+  }
   if (e != hipSuccess) {
+    if (b->nodes) hipFree(b->nodes);
     delete b;
     return hip_error(ctx, e, "aarbf_bvh_build");
   }
   int st = build_levels(ctx, b, (const float *)aabbs);
   if (st) {
     hipFree(b->nodes);
+    if (b->groups) hipFree(b->groups);
     delete b;
     return st;
   }
@@ -206,6 +288,7 @@ int aarbf_bvh_destroy(aarbf_context *ctx, uint64_t handle) {
   AARBF_REQUIRE(ctx, b, "aarbf_bvh_destroy: unknown handle");
   hipStreamSynchronize(ctx->stream);
   hipFree(b->nodes);
+  if (b->groups) hipFree(b->groups);
   delete b;
   ctx->bvhs.erase(handle);
   return AARBF_OK;
@@ -217,6 +300,17 @@ int aarbf_bvh_levels(aarbf_context *ctx, uint64_t handle, uint32_t *num_levels, 
   AARBF_REQUIRE(ctx, b && num_levels, "aarbf_bvh_levels: unknown handle");
   *num_levels = b->num_levels;
   for (uint32_t l = 0; counts && l < b->num_levels && l < max_levels; ++l) counts[l] = b->counts[l];
+  return AARBF_OK;
+}
+
+int aarbf_bvh_read_groups(aarbf_context *ctx, uint64_t handle, uint32_t level, uint32_t *groups) {
+  if (!ctx) return AARBF_ERR_INVALID_ARGUMENT;
+  BVH *b = get_bvh(ctx, handle);
+  AARBF_REQUIRE(ctx, b && groups && level >= 1 && level < b->num_levels, "aarbf_bvh_read_groups: bad arguments");
+  AARBF_REQUIRE(ctx, b->groups, "aarbf_bvh_read_groups: no quantised groups (branching > 8)");
+  AARBF_HIP(ctx, hipMemcpyAsync(groups, b->groups + b->group_ofs[level], sizeof(QGroup) * b->counts[level],
+                                hipMemcpyDeviceToHost, ctx->stream));
+  AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return AARBF_OK;
 }
 

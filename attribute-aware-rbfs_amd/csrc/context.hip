@@ -49,11 +49,13 @@ BVH *get_bvh(aarbf_context *ctx, uint64_t h) {
 BVHView make_view(const BVH &b) {
   BVHView v{};
   v.nodes = b.nodes;
+  v.groups = b.groups;
   v.log2K = b.log2K;
   v.top = b.num_levels - 1;
   for (uint32_t l = 0; l < b.num_levels; ++l) {
     v.counts[l] = b.counts[l];
     v.level_ofs[l] = (uint32_t)b.level_ofs[l];
+    v.group_ofs[l] = (uint32_t)b.group_ofs[l];
   }
   return v;
 }
@@ -118,6 +120,7 @@ int aarbf_context_destroy(aarbf_context *ctx) {
   for (uint64_t h : ctx->bvhs) {
     BVH *b = reinterpret_cast<BVH *>(h);
     hipFree(b->nodes);
+    if (b->groups) hipFree(b->groups);
     delete b;
   }
   if (ctx->scratch) hipFree(ctx->scratch);

@@ -29,6 +29,17 @@ constexpr int kMaxLevels = 32;
 // Hilbert-packed BVH: level 0 holds one (widened) box per leaf, node j of
 // level l+1 bounds nodes [j*K, j*K+K) of level l.  Every node is two float4
 // (lo.xyz, hi.xyz) in one allocation; level_ofs is in float4 units.
+//
+// For K <= 8 the internal nodes are also stored quantised: node j of level
+// l >= 1 owns one QGroup holding its own box corner and per-axis step and
+// its K children's boxes as 8-bit offsets, rounded outwards (80 bytes per
+// group instead of K * 32 bytes of float boxes).
+struct QGroup {
+  float4 a;       // lo.x, lo.y, lo.z, step.x
+  float4 b;       // step.y, step.z, 0, 0
+  uint32_t q[12]; // bytes: lo.x[8], lo.y[8], lo.z[8], hi.x[8], hi.y[8], hi.z[8]; lo.x > hi.x marks an empty child
+};
+
 struct BVH {
   uint32_t K = 8, log2K = 3;
   uint32_t num_levels = 0; // including level 0
@@ -36,15 +47,20 @@ struct BVH {
   uint64_t level_ofs[kMaxLevels] = {};
   float4 *nodes = nullptr;
   uint64_t total_nodes = 0;
+  QGroup *groups = nullptr;            // K <= 8 only
+  uint64_t group_ofs[kMaxLevels] = {}; // first group of level l (l >= 1)
+  uint64_t total_groups = 0;
 };
 
 // Device view of a BVH passed by value to kernels.
 struct BVHView {
   const float4 *nodes;
+  const QGroup *groups;
   uint32_t log2K;
   uint32_t top; // root level
   uint32_t counts[kMaxLevels];
   uint32_t level_ofs[kMaxLevels]; // in float4 units (< 2^32 for <= 2^30 nodes)
+  uint32_t group_ofs[kMaxLevels];
 };
 
 } // namespace aarbf

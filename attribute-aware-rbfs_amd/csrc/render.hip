@@ -278,7 +278,7 @@ int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg, const 
   A.accum = (float4 *)rg->accumBuffer;
   A.stats = ctx->d_stats;
   A.bvh = make_view(*bvh);
-  if (ctx->render_mode == 1 || bvh->num_levels > kStackLevels) {
+  if (ctx->render_mode == 1 || bvh->num_levels > kStackLevels || !bvh->groups) {
     dim3 grid((W + 15) / 16, (H + 15) / 16);
     if (ctx->stats)
       render_kernel<true><<<grid, 256, 0, ctx->stream>>>(A);

@@ -42,11 +42,13 @@ namespace {
 // array to scratch memory.
 __shared__ uint32_t g_lvl_ofs[kMaxLevels];
 __shared__ uint32_t g_lvl_cnt[kMaxLevels];
+__shared__ uint32_t g_lvl_qofs[kMaxLevels];
 
 __device__ inline void load_levels(const RenderArgs &A) {
   if (threadIdx.x < kMaxLevels) {
     g_lvl_ofs[threadIdx.x] = A.bvh.level_ofs[threadIdx.x];
     g_lvl_cnt[threadIdx.x] = A.bvh.counts[threadIdx.x];
+    g_lvl_qofs[threadIdx.x] = A.bvh.group_ofs[threadIdx.x];
   }
   __syncthreads();
 }
@@ -260,6 +262,122 @@ __device__ inline void walk_batched(const RenderArgs &A, uint16_t *stk, BoxTest 
     base = node << log2K;
     mask = group(lvl, base);
   }
+}
+
+// ---- quantised walk ------------------------------------------------------------
+// The same depth-first order over the QGroup copy of the tree (K <= 8): a
+// visited node costs one 80-byte group load, and its children's boxes are
+// dequantised (lo + q * step, rounded outwards at build time) before the
+// box test.  Level-0 hits are handed to `leaves(base, mask)` all at once.
+
+__device__ inline uint32_t qbyte(const QGroup &g, uint32_t field, uint32_t k) {
+  return (g.q[2 * field + (k >> 2)] >> ((k & 3u) * 8u)) & 255u;
+}
+
+template <int STRIDE, bool STATS, class BoxTest, class Leaves>
+__device__ inline void walk_q(const RenderArgs &A, uint16_t *stk, BoxTest test, Leaves leaves, Counters &c) {
+  const QGroup *__restrict__ Q = A.bvh.groups;
+  const uint32_t log2K = A.bvh.log2K, K = 1u << log2K, top = A.bvh.top;
+  const float4 *root = A.bvh.nodes + g_lvl_ofs[top];
+  if (STATS) c.nodes++;
+  if (!test(root[0], root[1])) return;
+  auto group = [&](uint32_t lvl, uint32_t node) -> uint32_t {
+    const QGroup g = Q[g_lvl_qofs[lvl] + node];
+    uint32_t first = node << log2K;
+    uint32_t m = min(K, g_lvl_cnt[lvl - 1] - first), mask = 0;
+    if (STATS) c.nodes += m;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+      uint32_t lx = qbyte(g, 0, k), hx = qbyte(g, 3, k);
+      if (k < m && lx <= hx) {
+        float4 lo, hi;
+        lo.x = g.a.x + (float)lx * g.a.w;
+        lo.y = g.a.y + (float)qbyte(g, 1, k) * g.b.x;
+        lo.z = g.a.z + (float)qbyte(g, 2, k) * g.b.y;
+        hi.x = g.a.x + (float)hx * g.a.w;
+        hi.y = g.a.y + (float)qbyte(g, 4, k) * g.b.x;
+        hi.z = g.a.z + (float)qbyte(g, 5, k) * g.b.y;
+        lo.w = hi.w = 0.0f;
+        if (test(lo, hi)) mask |= 1u << k;
+      }
+    }
+    return mask;
+  };
+  uint32_t lvl = top - 1, base = 0;
+  uint32_t mask = group(top, 0);
+  for (;;) {
+    if (mask == 0) {
+      if (lvl == top - 1) return;
+      ++lvl;
+      mask = stk[lvl * STRIDE];
+      base = ((base >> log2K) >> log2K) << log2K;
+      continue;
+    }
+    if (lvl == 0) {
+      if (leaves(base, mask)) return;
+      mask = 0;
+      continue;
+    }
+    uint32_t bit = (uint32_t)__ffs(mask) - 1u;
+    mask &= mask - 1u;
+    uint32_t node = base + bit;
+    stk[lvl * STRIDE] = (uint16_t)mask;
+    base = node << log2K;
+    mask = group(lvl, node);
+    --lvl;
+  }
+}
+
+// Point query over the quantised tree: leaves whose (dequantised) box holds
+// the point get the exact test in ascending order, as in point_query.
+template <int STRIDE, bool STATS>
+__device__ inline void point_query_q(const RenderArgs &A, uint16_t *stk, float x, float y, float z, Payload &pl,
+                                     Counters &c) {
+  pl.count = 0;
+  pl.dens = 0.0f;
+  pl.r = pl.g = pl.b = 0.0f;
+  if (STATS) c.queries++;
+  auto test = [&](const float4 &lo, const float4 &hi) { return inside(lo, hi, x, y, z); };
+  auto particle = [&](uint32_t p, const float4 &q) {
+    float d = aa_dist3(q.x, q.y, q.z, x, y, z);
+    if (d < q.w) {
+      float4 k = A.pc[p];
+      float wa = aa_rbf(d, q.w) * k.x;
+      if (STATS) c.covering++;
+      pl.count += 1;
+      pl.dens += wa;
+      pl.r += k.y * wa;
+      pl.g += k.z * wa;
+      pl.b += k.w * wa;
+      if (!A.visAttr) pl.dens = fminf(pl.dens, 1.0f);
+    }
+  };
+  auto leaves = [&](uint32_t lb, uint32_t mask) -> bool {
+    if (A.ppl == 1) {
+      float4 q[8];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k)
+        if ((mask >> k) & 1u) q[k] = A.pr[lb + k];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k)
+        if ((mask >> k) & 1u) {
+          if (STATS) c.particles++;
+          particle(lb + k, q[k]);
+        }
+      return false;
+    }
+    while (mask) {
+      uint32_t leaf = lb + (uint32_t)__ffs(mask) - 1u;
+      mask &= mask - 1u;
+      uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
+      for (uint32_t p = p0; p < p1; ++p) {
+        if (STATS) c.particles++;
+        particle(p, A.pr[p]);
+      }
+    }
+    return false;
+  };
+  walk_q<STRIDE, STATS>(A, stk, test, leaves, c);
 }
 
 // Point query with the batched walk; same covering set, order and

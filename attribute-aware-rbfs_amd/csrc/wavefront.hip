@@ -335,24 +335,23 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
       q.len = fminf(tmax, t_cut + 4.0f * m) - t0;
     };
     auto test = [&](const float4 &lo, const float4 &hi) { return seg_box(q, lo, hi); };
-    auto leaves = [&](uint32_t lb, uint32_t mcount) -> bool {
+    auto leaves = [&](uint32_t lb, uint32_t mask) -> bool {
       if (A.ppl == 1) {
         float4 P[8];
 #pragma unroll
         for (uint32_t k = 0; k < 8; ++k)
-          if (k < mcount) P[k] = A.pr[lb + k];
-        if (STATS) c.particles += mcount;
+          if ((mask >> k) & 1u) P[k] = A.pr[lb + k];
 #pragma unroll
         for (uint32_t k = 0; k < 8; ++k)
-          if (k < mcount) take(lb + k, P[k]);
-        for (uint32_t k = 8; k < mcount; ++k) take(lb + k, A.pr[lb + k]);
+          if ((mask >> k) & 1u) {
+            if (STATS) c.particles++;
+            take(lb + k, P[k]);
+          }
         return false;
       }
-      const float4 *L0 = A.bvh.nodes + g_lvl_ofs[0];
-      for (uint32_t k = 0; k < mcount; ++k) {
-        uint32_t leaf = lb + k;
-        if (STATS) c.leaves++;
-        if (!seg_box(q, L0[2 * leaf], L0[2 * leaf + 1])) continue;
+      while (mask) {
+        uint32_t leaf = lb + (uint32_t)__ffs(mask) - 1u;
+        mask &= mask - 1u;
         uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
         for (uint32_t p = p0; p < p1; ++p) {
           if (STATS) c.particles++;
@@ -361,7 +360,7 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
       }
       return false;
     };
-    walk_batched<kGBlock, STATS>(A, stk, test, leaves, c);
+    walk_q<kGBlock, STATS>(A, stk, test, leaves, c);
 
     float seg_end;
     if (t_cut <= t0 + 2.0f * A.unit) { // crowded: point queries for a short stretch
@@ -505,7 +504,7 @@ __global__ __launch_bounds__(kSBlock) void wf_step(RenderArgs A, WFState S, cons
         Payload pl;
         float x = L.o.x + L.t * L.d.x, y = L.o.y + L.t * L.d.y, z = L.o.z + L.t * L.d.z;
         if (L.flags & F_DENSE) {
-          point_query_b<kSBlock, STATS>(A, stk, x, y, z, pl, c);
+          point_query_q<kSBlock, STATS>(A, stk, x, y, z, pl, c);
         } else {
           if (L.t >= t_enter) enter(L.t);
           if (L.t > t_exit) leave(L.t);

@@ -240,6 +240,11 @@ int aarbf_bvh_destroy(aarbf_context *ctx, uint64_t handle);
 int aarbf_bvh_levels(aarbf_context *ctx, uint64_t handle, uint32_t *num_levels,
                      uint32_t *counts, uint32_t max_levels);
 int aarbf_bvh_read_level(aarbf_context *ctx, uint64_t handle, uint32_t level, float *boxes);
+/* Quantised copy of level `level` (>= 1, branching <= 8): 80 bytes per node
+ * (20 uint32): lo.xyz and step.x as floats, step.y, step.z, 2 unused, then
+ * 48 bytes of child bounds -- lo.x[8], lo.y[8], lo.z[8], hi.x[8], hi.y[8],
+ * hi.z[8] -- dequantised as lo + q * step; lo.x > hi.x marks no child. */
+int aarbf_bvh_read_groups(aarbf_context *ctx, uint64_t handle, uint32_t level, uint32_t *groups);
 
 /* ---- rendering ---------------------------------------------------------- */
 

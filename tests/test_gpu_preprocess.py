@@ -148,3 +148,43 @@ def test_single_hip_runtime(gpu_ctx):
     maps = open("/proc/self/maps").read()
     libs = {l.split()[-1] for l in maps.splitlines() if "libamdhip64" in l}
     assert len(libs) == 1, libs
+
+
+def _dequantise(groups):
+    """Child boxes [n, 8, 2, 3] of raw groups (lo + q * step in float32)."""
+    f = groups[:, :8].view(np.float32)
+    lo = f[:, 0:3]
+    step = np.stack([f[:, 3], f[:, 4], f[:, 5]], 1)
+    qb = groups[:, 8:20].copy().view(np.uint8).reshape(-1, 6, 8).astype(np.float32)
+    qlo, qhi = qb[:, 0:3, :], qb[:, 3:6, :]
+    dlo = (lo[:, :, None] + (qlo * step[:, :, None]).astype(np.float32)).astype(np.float32)
+    dhi = (lo[:, :, None] + (qhi * step[:, :, None]).astype(np.float32)).astype(np.float32)
+    empty = qb[:, 0, :] > qb[:, 3, :]
+    return dlo.transpose(0, 2, 1), dhi.transpose(0, 2, 1), empty
+
+
+@pytest.mark.parametrize("K", [2, 4, 8])
+def test_quantised_groups_contain_children(gpu_ctx, K):
+    rng = np.random.default_rng(40 + K)
+    n = 30000
+    a = rng.normal(0, 1, (n, 6)).astype(np.float32)
+    a[:, 3:] = a[:, :3] + rng.uniform(0, 0.01, (n, 3)).astype(np.float32)
+    a[rng.integers(0, n, 30)] = np.nan
+    h = gpu_ctx.bvh_build(_dev(a.reshape(-1)), K)
+    counts = gpu_ctx.bvh_levels(h)
+    for l in range(1, len(counts)):
+        child = gpu_ctx.bvh_read_level(h, l - 1)
+        dlo, dhi, empty = _dequantise(gpu_ctx.bvh_read_groups(h, l))
+        for j in range(counts[l]):
+            for k in range(K):
+                c = j * K + k
+                if c >= counts[l - 1]:
+                    assert empty[j, k]
+                    continue
+                clo, chi = child[c, :3], child[c, 4:7]
+                if not (clo[0] <= chi[0]):
+                    assert empty[j, k]
+                    continue
+                assert not empty[j, k]
+                assert np.all(dlo[j, k] <= clo) and np.all(dhi[j, k] >= chi), (l, j, k)
+    gpu_ctx.bvh_destroy(h)
