@@ -191,6 +191,23 @@ struct SegQuery {
 
 __device__ inline float safe_inv(float d) { return 1.0f / (fabsf(d) < 1e-30f ? copysignf(1e-30f, d) : d); }
 
+// Slab test that also returns where the segment enters the (widened) box,
+// relative to the segment start.
+__device__ inline bool seg_box_t(const SegQuery &s, float4 lo, float4 hi, float &tnear) {
+  float t1 = (lo.x - s.m - s.ax) * s.ix, t2 = (hi.x + s.m - s.ax) * s.ix;
+  float nr = fminf(t1, t2), fr = fmaxf(t1, t2);
+  t1 = (lo.y - s.m - s.ay) * s.iy;
+  t2 = (hi.y + s.m - s.ay) * s.iy;
+  nr = fmaxf(nr, fminf(t1, t2));
+  fr = fminf(fr, fmaxf(t1, t2));
+  t1 = (lo.z - s.m - s.az) * s.iz;
+  t2 = (hi.z + s.m - s.az) * s.iz;
+  nr = fmaxf(nr, fminf(t1, t2));
+  fr = fminf(fr, fmaxf(t1, t2));
+  tnear = fmaxf(nr, 0.0f);
+  return tnear <= fminf(fr, s.len);
+}
+
 __device__ inline bool seg_box(const SegQuery &s, float4 lo, float4 hi) {
   if (!(lo.x <= hi.x)) return false; // empty node
   float t1 = (lo.x - s.m - s.ax) * s.ix, t2 = (hi.x + s.m - s.ax) * s.ix;

@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
   __shared__ uint32_t s_idx[kCand * kGBlock];
   __shared__ float s_tin[kCand * kGBlock];
   __shared__ float s_tout[kCand * kGBlock];
-  __shared__ uint16_t s_stack[kStackLevels * kGBlock];
+  __shared__ uint32_t s_stack[kStackLevels * kGBlock];
   load_levels(A);
   const uint32_t n_g = *countG;
   uint32_t gid = blockIdx.x * kGBlock + threadIdx.x;
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
     const uint32_t slot = listG[gid];
     uint32_t *gi = s_idx + threadIdx.x;
     float *gtin = s_tin + threadIdx.x, *gtout = s_tout + threadIdx.x;
-    uint16_t *stk = s_stack + threadIdx.x;
+    uint32_t *stk = s_stack + threadIdx.x;
     float4 a0 = S.od0[slot], a1 = S.od1[slot];
     const float3 o = make_float3(a0.x, a0.y, a0.z), d = make_float3(a1.x, a1.y, a1.z);
     const float t0 = a0.w, tmax = a1.w;
@@ -334,33 +334,117 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
       }
       q.len = fminf(tmax, t_cut + 4.0f * m) - t0;
     };
-    auto test = [&](const float4 &lo, const float4 &hi) { return seg_box(q, lo, hi); };
-    auto leaves = [&](uint32_t lb, uint32_t mask) -> bool {
+    // Front-to-back walk over the quantised tree: the hit children of a
+    // node are visited in order of where the segment enters their box, so
+    // the list fills with the nearest candidates first and t_cut prunes the
+    // rest of the tree early.  The per-level visiting order is kept as
+    // nibbles (child + 1) in an LDS word.
+    const QGroup *__restrict__ Qg = A.bvh.groups;
+    const uint32_t log2K = A.bvh.log2K, K = 1u << log2K, top = A.bvh.top;
+    auto ordered = [&](uint32_t lvl, uint32_t node) -> uint32_t {
+      const QGroup g = Qg[g_lvl_qofs[lvl] + node];
+      uint32_t first = node << log2K;
+      uint32_t mcnt = min(K, g_lvl_cnt[lvl - 1] - first);
+      if (STATS) c.nodes += mcnt;
+      float key[8];
+      uint32_t id[8];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k) {
+        key[k] = AA_INF_F;
+        id[k] = k;
+        uint32_t lx = qbyte(g, 0, k), hx = qbyte(g, 3, k);
+        if (k < mcnt && lx <= hx) {
+          float4 lo, hi;
+          lo.x = g.a.x + (float)lx * g.a.w;
+          lo.y = g.a.y + (float)qbyte(g, 1, k) * g.b.x;
+          lo.z = g.a.z + (float)qbyte(g, 2, k) * g.b.y;
+          hi.x = g.a.x + (float)hx * g.a.w;
+          hi.y = g.a.y + (float)qbyte(g, 4, k) * g.b.x;
+          hi.z = g.a.z + (float)qbyte(g, 5, k) * g.b.y;
+          float tn;
+          if (seg_box_t(q, lo, hi, tn)) key[k] = tn;
+        }
+      }
+      // Batcher's odd-even merge sort network for 8 keys
+#define AARBF_CSWAP(i, j)                                                                                    \
+  if (key[j] < key[i]) {                                                                                     \
+    float tk = key[i];                                                                                       \
+    key[i] = key[j];                                                                                         \
+    key[j] = tk;                                                                                             \
+    uint32_t ti = id[i];                                                                                     \
+    id[i] = id[j];                                                                                           \
+    id[j] = ti;                                                                                              \
+  }
+      AARBF_CSWAP(0, 1) AARBF_CSWAP(2, 3) AARBF_CSWAP(4, 5) AARBF_CSWAP(6, 7)
+      AARBF_CSWAP(0, 2) AARBF_CSWAP(1, 3) AARBF_CSWAP(4, 6) AARBF_CSWAP(5, 7)
+      AARBF_CSWAP(1, 2) AARBF_CSWAP(5, 6) AARBF_CSWAP(0, 4) AARBF_CSWAP(3, 7)
+      AARBF_CSWAP(1, 5) AARBF_CSWAP(2, 6) AARBF_CSWAP(1, 4) AARBF_CSWAP(3, 6)
+      AARBF_CSWAP(2, 4) AARBF_CSWAP(3, 5) AARBF_CSWAP(3, 4)
+#undef AARBF_CSWAP
+      uint32_t ord = 0, cnt = 0;
+#pragma unroll
+      for (uint32_t r = 0; r < 8; ++r)
+        if (key[r] < AA_INF_F) {
+          ord |= (id[r] + 1u) << (4u * cnt);
+          ++cnt;
+        }
+      return ord;
+    };
+    auto leaf_batch = [&](uint32_t lb, uint32_t ord) {
       if (A.ppl == 1) {
         float4 P[8];
+        uint32_t pid[8];
 #pragma unroll
-        for (uint32_t k = 0; k < 8; ++k)
-          if ((mask >> k) & 1u) P[k] = A.pr[lb + k];
+        for (uint32_t r = 0; r < 8; ++r) {
+          uint32_t nib = (ord >> (4u * r)) & 15u;
+          pid[r] = nib ? lb + nib - 1u : 0xffffffffu;
+          if (nib) P[r] = A.pr[pid[r]];
+        }
 #pragma unroll
-        for (uint32_t k = 0; k < 8; ++k)
-          if ((mask >> k) & 1u) {
+        for (uint32_t r = 0; r < 8; ++r)
+          if (pid[r] != 0xffffffffu) {
             if (STATS) c.particles++;
-            take(lb + k, P[k]);
+            take(pid[r], P[r]);
           }
-        return false;
+        return;
       }
-      while (mask) {
-        uint32_t leaf = lb + (uint32_t)__ffs(mask) - 1u;
-        mask &= mask - 1u;
+      for (; ord; ord >>= 4) {
+        uint32_t leaf = lb + (ord & 15u) - 1u;
         uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
         for (uint32_t p = p0; p < p1; ++p) {
           if (STATS) c.particles++;
           take(p, A.pr[p]);
         }
       }
-      return false;
     };
-    walk_q<kGBlock, STATS>(A, stk, test, leaves, c);
+    {
+      const float4 *root = A.bvh.nodes + g_lvl_ofs[top];
+      if (STATS) c.nodes++;
+      if (seg_box(q, root[0], root[1])) {
+        uint32_t lvl = top - 1, base = 0;
+        uint32_t ord = ordered(top, 0);
+        for (;;) {
+          if (lvl == 0) { // a whole leaf group, in entry order
+            leaf_batch(base, ord);
+            ord = 0;
+          }
+          uint32_t nib = ord & 15u;
+          if (nib == 0) {
+            if (lvl == top - 1) break;
+            ++lvl;
+            ord = stk[lvl * kGBlock];
+            base = ((base >> log2K) >> log2K) << log2K;
+            continue;
+          }
+          ord >>= 4;
+          uint32_t node = base + nib - 1u;
+          stk[lvl * kGBlock] = ord;
+          base = node << log2K;
+          ord = ordered(lvl, node);
+          --lvl;
+        }
+      }
+    }
 
     float seg_end;
     if (t_cut <= t0 + 2.0f * A.unit) { // crowded: point queries for a short stretch
