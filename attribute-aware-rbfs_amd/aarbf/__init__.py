@@ -134,6 +134,12 @@ class Context:
         self.check(self.lib.aarbf_render_stats_read(self.h, C.byref(s), 1 if reset else 0), "stats")
         return {n: int(getattr(s, n)) for n, _ in RenderStats._fields_}
 
+    def read_histogram(self, reset: bool = True) -> list:
+        """Gather walk-cost histogram: entry b = gathers testing [2^b, 2^(b+1)) nodes."""
+        h = (C.c_uint64 * 24)()
+        self.check(self.lib.aarbf_render_histogram(self.h, h, 1 if reset else 0), "aarbf_render_histogram")
+        return [int(v) for v in h]
+
     # -- pre-process (hostCode.cpp:183-250) ---------------------------------
     def spacetime_aabb(self, particles, aabb=None) -> np.ndarray:
         a = np.array(aabb if aabb is not None else [1e38, 1e38, 1e38, -1e38, -1e38, -1e38], dtype=np.float32)

@@ -180,6 +180,14 @@ int aarbf_render_stats_read(aarbf_context *ctx, aarbf_render_stats *out, int res
   return AARBF_OK;
 }
 
+int aarbf_render_histogram(aarbf_context *ctx, uint64_t *out, int reset) {
+  if (!ctx || !out) return AARBF_ERR_INVALID_ARGUMENT;
+  AARBF_HIP(ctx, hipMemcpyAsync(out, ctx->d_stats + 8, sizeof(uint64_t) * 24, hipMemcpyDeviceToHost, ctx->stream));
+  AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (reset) AARBF_HIP(ctx, hipMemsetAsync(ctx->d_stats + 8, 0, sizeof(uint64_t) * 24, ctx->stream));
+  return AARBF_OK;
+}
+
 // ---- transfer functions (hostCode.cpp:355-365,524-572) --------------------
 
 int aarbf_tf_create(aarbf_context *ctx, const float *rgba, uint32_t n, uint64_t *handle) {

@@ -290,6 +290,7 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
     uint32_t n = 0, maxpos = 0;
     float maxtin = -AA_INF_F, t_cut = AA_INF_F;
     if (STATS) c.queries++;
+    const uint32_t nodes0 = c.nodes;
     // conservative entry/exit of the particle's sphere widened by m
     auto take = [&](uint32_t p, const float4 &P) {
       if (!(P.w > 0.0f)) return;
@@ -446,6 +447,11 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
       }
     }
 
+    if (STATS) {
+      uint32_t cost = c.nodes - nodes0;
+      uint32_t b = cost ? 31u - (uint32_t)__clz(cost) : 0u;
+      atomicAdd((unsigned long long *)&A.stats[8 + min(b, 23u)], 1ull);
+    }
     float seg_end;
     if (t_cut <= t0 + 2.0f * A.unit) { // crowded: point queries for a short stretch
       flags |= F_DENSE;

@@ -116,7 +116,9 @@ def main() -> int:
     frame(0)
     torch.cuda.synchronize()
     st = ctx.read_stats(reset=True)
+    hist = ctx.read_histogram(reset=True)
     ctx.set_stats(False)
+    log("gather walk cost histogram (log2 nodes -> count): " + str({b: v for b, v in enumerate(hist) if v}))
     samples = max(1, st["samples"])
     # bytes per launch: node/leaf boxes 32 B, particle position+radius 16 B,
     # colour payload 16 B per covering particle, accum read+write and image write 48 B per pixel

@@ -157,6 +157,10 @@ int aarbf_set_render_mode(aarbf_context *ctx, int mode);
  * them) with aarbf_render_stats_read. */
 int aarbf_set_stats(aarbf_context *ctx, int enable);
 int aarbf_render_stats_read(aarbf_context *ctx, aarbf_render_stats *out, int reset);
+/* Walk-cost histogram of the segment gathers since the last reset (with
+ * statistics enabled): out[b] (b < 24) counts gathers that tested between
+ * 2^b and 2^(b+1)-1 BVH nodes.  No counterpart in the reference. */
+int aarbf_render_histogram(aarbf_context *ctx, uint64_t *out, int reset);
 /* Writes sizes and offsets of the structs above into out[0..n) and returns
  * how many there are; bindings in other languages use it to check their
  * struct mirrors.  No counterpart in the reference. */
