@@ -90,10 +90,10 @@ int aarbf_context_create(int device, aarbf_context **out) {
   if (!ctx) return AARBF_ERR_OUT_OF_MEMORY;
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&ctx->d_stats, sizeof(uint64_t) * 8) != hipSuccess ||
+      hipMalloc(&ctx->d_stats, sizeof(uint64_t) * 32) != hipSuccess ||
       hipMalloc(&ctx->d_counter, sizeof(uint32_t) * 4) != hipSuccess ||
       hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
-      hipMemset(ctx->d_stats, 0, sizeof(uint64_t) * 8) != hipSuccess) {
+      hipMemset(ctx->d_stats, 0, sizeof(uint64_t) * 32) != hipSuccess) {
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return AARBF_ERR_HIP;
