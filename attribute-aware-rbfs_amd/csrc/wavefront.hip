@@ -37,6 +37,10 @@ constexpr int kGBlock = 128; // gather kernel block
 constexpr int kSBlock = 64;  // step kernel block (one wave)
 
 constexpr uint32_t F_SHADOW = 1u, F_PENDING = 2u, F_DENSE = 4u;
+// F_HEAD: the segment is the pixel's head list (shared by the primary rays
+// of all its samples, which follow the same line); F_HEADGATHER: the next
+// gather builds that head list from the ray's first possible position.
+constexpr uint32_t F_HEAD = 8u, F_HEADGATHER = 16u;
 
 // Ray-slot state, structure of arrays (one entry per slot).
 struct WFState {
@@ -51,6 +55,10 @@ struct WFState {
   float *ctin;    // [kCand][nslots]
   float *ctout;   // [kCand][nslots]
   uint8_t *ceo;   // [kCand][nslots] candidate positions ordered by tin
+  uint32_t *hidx; // head list of the pixel's primary ray, same layout
+  float *htin, *htout;
+  uint8_t *heo;
+  float4 *head;   // head t0, head segment end, n (bits), 1 = valid | 2 = dense (bits)
   uint32_t nslots;
   uint32_t tiles_x;
 };
@@ -146,6 +154,7 @@ __device__ inline bool start_sample(const RenderArgs &A, int px, int py, Lane &L
   L.d = dir;
   L.tmax = texit;
   L.t = marching ? tenter + A.unit * r1 : 0.0f;
+  L.seg_t0 = marching ? tenter : 0.0f; // where any sample's primary ray can start
   L.LHS = 0.0f;
   L.RHS = marching ? -aa_logf(1.0f - r0 * 0.99f) : 0.0f;
   return true;
@@ -186,19 +195,32 @@ __device__ inline void finish_sample(const RenderArgs &A, Lane &L, float vis, Co
 
 // Starts samples until one needs tracking (true) or the pixel is done, in
 // which case its outputs are written (false).
+// Returns 2 when the new ray needs a gather, 1 when it can step at once
+// (head list of the pixel already built), 0 when the pixel is done.
 template <bool STATS>
-__device__ inline bool next_ray(const RenderArgs &A, int px, int py, Lane &L, Counters &c) {
+__device__ inline int next_ray(const RenderArgs &A, const WFState &S, uint32_t slot, int px, int py, Lane &L,
+                               Counters &c) {
   while (L.s < A.spp) {
     if (start_sample(A, px, py, L)) {
+      float4 h = S.head[slot];
+      uint32_t hf = __float_as_uint(h.w);
+      if (hf & 1u) {
+        L.flags |= F_HEAD | ((hf & 2u) ? F_DENSE : 0u);
+        L.seg_t0 = h.x;
+        L.seg_end = h.y;
+        L.seg_n = __float_as_uint(h.z);
+        return 1;
+      }
+      L.flags |= F_HEADGATHER;
       L.seg_end = -AA_INF_F;
-      return true;
+      return 2;
     }
     finish_sample<STATS>(A, L, 1.0f, c);
   }
   uint64_t ofs = (uint64_t)px + (uint64_t)A.W * py;
   A.accum[ofs] = L.acc;
   A.image[ofs] = A.showNoise ? make_float4(L.rx, L.rx, L.rx, 1.0f) : L.acc;
-  return false;
+  return 0;
 }
 
 __device__ inline void push(uint32_t *list, uint32_t *count, uint32_t slot) {
@@ -229,7 +251,7 @@ __global__ __launch_bounds__(256) void wf_init(RenderArgs A, WFState S, uint32_t
                                                uint32_t *counts) {
   uint32_t slot = blockIdx.x * 256u + threadIdx.x;
   Counters c = {};
-  bool push_it = false;
+  bool push_a = false, push_g = false;
   if (slot < S.nslots) {
     int px, py;
     slot_pixel(S, slot, px, py);
@@ -239,16 +261,17 @@ __global__ __launch_bounds__(256) void wf_init(RenderArgs A, WFState S, uint32_t
       L.s = 0;
       L.seg_n = 0;
       L.seg_t0 = 0.0f;
-      if (next_ray<STATS>(A, px, py, L, c)) {
+      S.head[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      int r = next_ray<STATS>(A, S, slot, px, py, L, c);
+      if (r) {
         save_lane(S, slot, L);
-        push_it = true;
+        push_a = true;
+        push_g = r == 2;
       }
     }
   }
-  if (push_it) {
-    push(listA, &counts[0], slot);
-    push(listG, &counts[1], slot);
-  }
+  if (push_a) push(listA, &counts[0], slot);
+  if (push_g) push(listG, &counts[1], slot);
   flush_stats<STATS>(A, c);
 }
 
@@ -271,9 +294,10 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
     uint32_t *stk = s_stack + threadIdx.x;
     float4 a0 = S.od0[slot], a1 = S.od1[slot];
     const float3 o = make_float3(a0.x, a0.y, a0.z), d = make_float3(a1.x, a1.y, a1.z);
-    const float t0 = a0.w, tmax = a1.w;
     uint4 u = S.u[slot];
     uint32_t flags = u.z >> 16;
+    const bool head = (flags & F_HEADGATHER) != 0;
+    const float t0 = head ? S.seg[slot].x : a0.w, tmax = a1.w;
     SegQuery q;
     q.ax = o.x + t0 * d.x;
     q.ay = o.y + t0 * d.y;
@@ -459,6 +483,9 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
       n = 0;
     } else {
       flags &= ~F_DENSE;
+      uint32_t *oidx = head ? S.hidx : S.cidx;
+      float *otin = head ? S.htin : S.ctin, *otout = head ? S.htout : S.ctout;
+      uint8_t *oeo = head ? S.heo : S.ceo;
       seg_end = t_cut;
       // ascending particle order (insertion sort, n <= kCand)
       for (uint32_t a = 1; a < n; ++a) {
@@ -483,11 +510,18 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
           float tj = gtin[j * kGBlock];
           r += (tj < tk) || (tj == tk && j < k);
         }
-        S.ceo[(uint64_t)r * S.nslots + slot] = (uint8_t)k;
-        S.cidx[(uint64_t)k * S.nslots + slot] = gi[k * kGBlock];
-        S.ctin[(uint64_t)k * S.nslots + slot] = tk;
-        S.ctout[(uint64_t)k * S.nslots + slot] = gtout[k * kGBlock];
+        oeo[(uint64_t)r * S.nslots + slot] = (uint8_t)k;
+        oidx[(uint64_t)k * S.nslots + slot] = gi[k * kGBlock];
+        otin[(uint64_t)k * S.nslots + slot] = tk;
+        otout[(uint64_t)k * S.nslots + slot] = gtout[k * kGBlock];
       }
+    }
+    if (head) {
+      flags = (flags & ~F_HEADGATHER) | F_HEAD;
+      S.head[slot] = make_float4(t0, seg_end, __uint_as_float(n),
+                                 __uint_as_float(1u | ((flags & F_DENSE) ? 2u : 0u)));
+    } else {
+      flags &= ~F_HEAD;
     }
     S.u[slot] = make_uint4(u.x, u.y, (u.z & 0xffffu) | (flags << 16), n);
     float4 g = S.seg[slot];
@@ -525,11 +559,17 @@ __global__ __launch_bounds__(kSBlock) void wf_step(RenderArgs A, WFState S, cons
     uint8_t *leo = s_eo + threadIdx.x;
     uint16_t *stk = s_stack + threadIdx.x;
     uint32_t n = (L.flags & F_DENSE) ? 0u : L.seg_n;
-    for (uint32_t k = 0; k < n; ++k) {
-      li[k * kSBlock] = S.cidx[(uint64_t)k * S.nslots + slot];
-      ltin[k * kSBlock] = S.ctin[(uint64_t)k * S.nslots + slot];
-      ltout[k * kSBlock] = S.ctout[(uint64_t)k * S.nslots + slot];
-      leo[k * kSBlock] = S.ceo[(uint64_t)k * S.nslots + slot];
+    {
+      const bool hd = (L.flags & F_HEAD) != 0;
+      const uint32_t *iidx = hd ? S.hidx : S.cidx;
+      const float *itin = hd ? S.htin : S.ctin, *itout = hd ? S.htout : S.ctout;
+      const uint8_t *ieo = hd ? S.heo : S.ceo;
+      for (uint32_t k = 0; k < n; ++k) {
+        li[k * kSBlock] = iidx[(uint64_t)k * S.nslots + slot];
+        ltin[k * kSBlock] = itin[(uint64_t)k * S.nslots + slot];
+        ltout[k * kSBlock] = itout[(uint64_t)k * S.nslots + slot];
+        leo[k * kSBlock] = ieo[(uint64_t)k * S.nslots + slot];
+      }
     }
     // active set at the current t
     uint32_t act = 0, eptr = 0;
@@ -678,11 +718,7 @@ __global__ __launch_bounds__(kSBlock) void wf_step(RenderArgs A, WFState S, cons
       }
       float vis = (L.flags & F_SHADOW) ? 1.0f - L.shw : 1.0f;
       finish_sample<STATS>(A, L, vis, c);
-      if (next_ray<STATS>(A, px, py, L, c)) {
-        outcome = 2;
-      } else {
-        outcome = 0;
-      }
+      outcome = next_ray<STATS>(A, S, slot, px, py, L, c);
       break;
     }
     if (outcome) save_lane(S, slot, L);
@@ -714,7 +750,8 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
   size_t o_od0 = take(16 * nslots), o_od1 = take(16 * nslots), o_acc = take(16 * nslots), o_alb = take(16 * nslots),
          o_u = take(16 * nslots), o_f = take(16 * nslots), o_seg = take(16 * nslots),
          o_cidx = take(4 * kCand * nslots), o_ctin = take(4 * kCand * nslots), o_ctout = take(4 * kCand * nslots),
-         o_ceo = take(kCand * nslots), o_l0 = take(4 * nslots), o_l1 = take(4 * nslots), o_l2 = take(4 * nslots),
+         o_ceo = take(kCand * nslots), o_hidx = take(4 * kCand * nslots), o_htin = take(4 * kCand * nslots),
+         o_htout = take(4 * kCand * nslots), o_heo = take(kCand * nslots), o_head = take(16 * nslots), o_l0 = take(4 * nslots), o_l1 = take(4 * nslots), o_l2 = take(4 * nslots),
          o_l3 = take(4 * nslots), o_cnt = take(64);
   char *base = (char *)scratch(ctx, off);
   if (!base) return set_error(ctx, AARBF_ERR_OUT_OF_MEMORY, "aarbf_render_rbf: wavefront state");
@@ -730,6 +767,11 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
   S.ctin = (float *)(base + o_ctin);
   S.ctout = (float *)(base + o_ctout);
   S.ceo = (uint8_t *)(base + o_ceo);
+  S.hidx = (uint32_t *)(base + o_hidx);
+  S.htin = (float *)(base + o_htin);
+  S.htout = (float *)(base + o_htout);
+  S.heo = (uint8_t *)(base + o_heo);
+  S.head = (float4 *)(base + o_head);
   S.nslots = (uint32_t)nslots;
   S.tiles_x = tiles_x;
   uint32_t *lists[4] = {(uint32_t *)(base + o_l0), (uint32_t *)(base + o_l1), (uint32_t *)(base + o_l2),
