@@ -531,25 +531,37 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
 }
 
 // ---- step ------------------------------------------------------------------------
-template <bool STATS>
+// DENSE = false: rays whose segment has a candidate list; rays on a dense
+// segment are handed to the dense queue untouched.  DENSE = true: the dense
+// queue, stepped with point queries over the quantised tree.  Keeping the
+// two apart keeps the BVH walk's registers out of the common kernel.
+template <bool STATS, bool DENSE>
 __global__ __launch_bounds__(kSBlock) void wf_step(RenderArgs A, WFState S, const uint32_t *__restrict__ listA,
                                                    const uint32_t *__restrict__ countA, uint32_t *__restrict__ outA,
                                                    uint32_t *__restrict__ outG, uint32_t *__restrict__ counts,
+                                                   uint32_t *__restrict__ outD, uint32_t *__restrict__ countD,
                                                    uint32_t step_cap) {
-  __shared__ uint32_t s_idx[kCand * kSBlock];
-  __shared__ float s_tin[kCand * kSBlock];
-  __shared__ float s_tout[kCand * kSBlock];
-  __shared__ uint8_t s_eo[kCand * kSBlock];
-  __shared__ uint16_t s_stack[kStackLevels * kSBlock];
+  constexpr uint32_t NC = DENSE ? 1u : kCand;
+  constexpr uint32_t NS = DENSE ? kStackLevels : 1u;
+  __shared__ uint32_t s_idx[NC * kSBlock];
+  __shared__ float s_tin[NC * kSBlock];
+  __shared__ float s_tout[NC * kSBlock];
+  __shared__ uint8_t s_eo[NC * kSBlock];
+  __shared__ uint16_t s_stack[NS * kSBlock];
   load_levels(A);
   const bool marching = A.blue != 0;
   const uint32_t n_a = *countA;
   uint32_t gid = blockIdx.x * kSBlock + threadIdx.x;
   Counters c = {};
-  int outcome = 0; // 0 nothing, 1 keep active, 2 active + gather
+  int outcome = 0; // 0 nothing, 1 keep active, 2 active + gather, 3 dense queue
   uint32_t slot = 0;
   if (gid < n_a) {
     slot = listA[gid];
+    if (!DENSE && (S.u[slot].z >> 16) & F_DENSE) {
+      outcome = 3;
+    }
+  }
+  if (gid < n_a && outcome == 0) {
     int px, py;
     slot_pixel(S, slot, px, py);
     Lane L;
@@ -633,7 +645,7 @@ __global__ __launch_bounds__(kSBlock) void wf_step(RenderArgs A, WFState S, cons
         }
         Payload pl;
         float x = L.o.x + L.t * L.d.x, y = L.o.y + L.t * L.d.y, z = L.o.z + L.t * L.d.z;
-        if (L.flags & F_DENSE) {
+        if (DENSE) {
           point_query_q<kSBlock, STATS>(A, stk, x, y, z, pl, c);
         } else {
           if (L.t >= t_enter) enter(L.t);
@@ -724,8 +736,9 @@ __global__ __launch_bounds__(kSBlock) void wf_step(RenderArgs A, WFState S, cons
     if (outcome) save_lane(S, slot, L);
   }
   // compaction: lanes with an outcome append to the next lists
-  if (outcome) push(outA, &counts[0], slot);
+  if (outcome == 1 || outcome == 2) push(outA, &counts[0], slot);
   if (outcome == 2) push(outG, &counts[1], slot);
+  if (outcome == 3) push(outD, countD, slot);
   flush_stats<STATS>(A, c);
 }
 
@@ -752,7 +765,7 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
          o_cidx = take(4 * kCand * nslots), o_ctin = take(4 * kCand * nslots), o_ctout = take(4 * kCand * nslots),
          o_ceo = take(kCand * nslots), o_hidx = take(4 * kCand * nslots), o_htin = take(4 * kCand * nslots),
          o_htout = take(4 * kCand * nslots), o_heo = take(kCand * nslots), o_head = take(16 * nslots), o_l0 = take(4 * nslots), o_l1 = take(4 * nslots), o_l2 = take(4 * nslots),
-         o_l3 = take(4 * nslots), o_cnt = take(64);
+         o_l3 = take(4 * nslots), o_l4 = take(4 * nslots), o_cnt = take(64);
   char *base = (char *)scratch(ctx, off);
   if (!base) return set_error(ctx, AARBF_ERR_OUT_OF_MEMORY, "aarbf_render_rbf: wavefront state");
   WFState S;
@@ -776,7 +789,8 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
   S.tiles_x = tiles_x;
   uint32_t *lists[4] = {(uint32_t *)(base + o_l0), (uint32_t *)(base + o_l1), (uint32_t *)(base + o_l2),
                         (uint32_t *)(base + o_l3)};
-  uint32_t *cnt = (uint32_t *)(base + o_cnt); // [0..1] current A, G ; [2..3] next A, G
+  uint32_t *listD = (uint32_t *)(base + o_l4);
+  uint32_t *cnt = (uint32_t *)(base + o_cnt); // [0..1] and [2..3]: (A, G) of the two list pairs; [4]: dense
   const bool st = ctx->stats;
   wf_zero<<<1, 64, 0, ctx->stream>>>(cnt, 16);
   uint32_t init_blocks = (uint32_t)((nslots + 255) / 256);
@@ -801,13 +815,19 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
         wf_gather<false><<<blocks, kGBlock, 0, ctx->stream>>>(A, S, lG, cA + 1);
     }
     wf_zero<<<1, 64, 0, ctx->stream>>>(cN, 2);
+    wf_zero<<<1, 64, 0, ctx->stream>>>(cnt + 4, 1);
     // fewer live rays -> more steps per round, so the tail costs few rounds
     uint32_t cap = (uint32_t)std::min<uint64_t>(16384, std::max<uint64_t>(256, 256ull * nslots / (4ull * nA)));
     uint32_t blocks = (nA + kSBlock - 1) / kSBlock;
-    if (st)
-      wf_step<true><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, cap);
-    else
-      wf_step<false><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, cap);
+    if (st) {
+      wf_step<true, false><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+      wf_step<true, true><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, nullptr, nullptr,
+                                                               cap);
+    } else {
+      wf_step<false, false><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+      wf_step<false, true><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, nullptr, nullptr,
+                                                                cap);
+    }
     AARBF_CHECK_LAUNCH(ctx, "wf_step");
     AARBF_HIP(ctx, hipMemcpyAsync(h, cN, 8, hipMemcpyDeviceToHost, ctx->stream));
     AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
