@@ -126,6 +126,13 @@ class Context:
         """0: persistent segment-gather kernel (default); 1: per-pixel point queries."""
         self.check(self.lib.aarbf_set_render_mode(self.h, int(mode)), "aarbf_set_render_mode")
 
+    TUNE_STEP_WAVES = 1
+    TUNE_STEP_CAP = 2
+
+    def set_tuning(self, param: int, value: int):
+        """Schedule parameters of the default renderer (images do not change)."""
+        self.check(self.lib.aarbf_set_tuning(self.h, int(param), int(value)), "aarbf_set_tuning")
+
     def set_stats(self, enable: bool):
         self.check(self.lib.aarbf_set_stats(self.h, 1 if enable else 0), "aarbf_set_stats")
 

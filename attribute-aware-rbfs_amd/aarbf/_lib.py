@@ -87,6 +87,7 @@ SIGNATURES = {
     "aarbf_status_string": (C.c_char_p, [C.c_int]),
     "aarbf_set_stats": (C.c_int, [P, C.c_int]),
     "aarbf_set_render_mode": (C.c_int, [P, C.c_int]),
+    "aarbf_set_tuning": (C.c_int, [P, C.c_int, C.c_int]),
     "aarbf_render_stats_read": (C.c_int, [P, C.POINTER(RenderStats), C.c_int]),
     "aarbf_render_histogram": (C.c_int, [P, C.POINTER(u64), C.c_int]),
     "aarbf_import_particles": (C.c_int, [C.c_char_p, C.POINTER(P)]),

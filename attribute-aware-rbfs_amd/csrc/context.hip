@@ -157,6 +157,22 @@ int aarbf_set_render_mode(aarbf_context *ctx, int mode) {
   return AARBF_OK;
 }
 
+int aarbf_set_tuning(aarbf_context *ctx, int param, int value) {
+  if (!ctx) return AARBF_ERR_INVALID_ARGUMENT;
+  switch (param) {
+  case AARBF_TUNE_STEP_WAVES:
+    AARBF_REQUIRE(ctx, value == 0 || (value >= 4 && value <= 6), "aarbf_set_tuning: step waves must be 0, 4, 5 or 6");
+    ctx->tune_step_waves = value;
+    return AARBF_OK;
+  case AARBF_TUNE_STEP_CAP:
+    AARBF_REQUIRE(ctx, value >= 16 && value <= 65536, "aarbf_set_tuning: step cap must be in [16, 65536]");
+    ctx->tune_step_cap = value;
+    return AARBF_OK;
+  default:
+    return set_error(ctx, AARBF_ERR_INVALID_ARGUMENT, "aarbf_set_tuning: unknown parameter");
+  }
+}
+
 int aarbf_set_stats(aarbf_context *ctx, int enable) {
   if (!ctx) return AARBF_ERR_INVALID_ARGUMENT;
   ctx->stats = enable != 0;

@@ -75,6 +75,8 @@ struct aarbf_context {
   uint32_t *d_counter = nullptr; // work counter of the persistent render kernel
   int render_mode = 0;           // 0 persistent segment-gather, 1 per-pixel point queries
   int cus = 0;                   // compute units of the device
+  int tune_step_waves = 0;       // occupancy target of the step kernel (0 = compiler's choice)
+  int tune_step_cap = 256;       // tracker steps per ray per wavefront round
   // grow-only device scratch
   void *scratch = nullptr;
   size_t scratch_bytes = 0;

@@ -535,8 +535,8 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
 // segment are handed to the dense queue untouched.  DENSE = true: the dense
 // queue, stepped with point queries over the quantised tree.  Keeping the
 // two apart keeps the BVH walk's registers out of the common kernel.
-template <bool STATS, bool DENSE>
-__global__ __launch_bounds__(kSBlock) void wf_step(RenderArgs A, WFState S, const uint32_t *__restrict__ listA,
+template <bool STATS, bool DENSE, int WPE>
+__global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S, const uint32_t *__restrict__ listA,
                                                    const uint32_t *__restrict__ countA, uint32_t *__restrict__ outA,
                                                    uint32_t *__restrict__ outG, uint32_t *__restrict__ counts,
                                                    uint32_t *__restrict__ outD, uint32_t *__restrict__ countD,
@@ -817,16 +817,29 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
     wf_zero<<<1, 64, 0, ctx->stream>>>(cN, 2);
     wf_zero<<<1, 64, 0, ctx->stream>>>(cnt + 4, 1);
     // fewer live rays -> more steps per round, so the tail costs few rounds
-    uint32_t cap = (uint32_t)std::min<uint64_t>(16384, std::max<uint64_t>(256, 256ull * nslots / (4ull * nA)));
+    const uint64_t base_cap = (uint64_t)ctx->tune_step_cap;
+    uint32_t cap = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(base_cap, base_cap * nslots / (4ull * nA)));
     uint32_t blocks = (nA + kSBlock - 1) / kSBlock;
     if (st) {
-      wf_step<true, false><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
-      wf_step<true, true><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, nullptr, nullptr,
-                                                               cap);
+      wf_step<true, false, 1><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+      wf_step<true, true, 1><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, nullptr,
+                                                                  nullptr, cap);
     } else {
-      wf_step<false, false><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
-      wf_step<false, true><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, nullptr, nullptr,
-                                                                cap);
+      switch (ctx->tune_step_waves) {
+      case 4:
+        wf_step<false, false, 4><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+        break;
+      case 5:
+        wf_step<false, false, 5><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+        break;
+      case 6:
+        wf_step<false, false, 6><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+        break;
+      default:
+        wf_step<false, false, 1><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+      }
+      wf_step<false, true, 1><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, nullptr,
+                                                                   nullptr, cap);
     }
     AARBF_CHECK_LAUNCH(ctx, "wf_step");
     AARBF_HIP(ctx, hipMemcpyAsync(h, cN, 8, hipMemcpyDeviceToHost, ctx->stream));

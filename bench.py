@@ -43,7 +43,9 @@ def main() -> int:
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mode", type=int, default=0, help="0 persistent segment-gather kernel, 1 point queries")
+    ap.add_argument("--mode", type=int, default=0, help="0 wavefront gather/step kernels, 1 point queries")
+    ap.add_argument("--step-waves", type=int, default=0)
+    ap.add_argument("--step-cap", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "render_traffic.json"),
                     help="HBM bytes per render launch measured with rocprofv3 --pmc")
@@ -74,6 +76,8 @@ def main() -> int:
     torch.cuda.set_stream(stream)
     ctx.set_stream(stream)
     ctx.set_render_mode(a.mode)
+    ctx.set_tuning(ctx.TUNE_STEP_WAVES, a.step_waves)
+    ctx.set_tuning(ctx.TUNE_STEP_CAP, a.step_cap)
     torch.cuda.synchronize()
     tp = time.time()
     sc = aarbf.Scene.prepare(ctx, wl.xyzw, wl.colormap, wl.radiusmap, wl.densitymap, wl.rbf_radius,

@@ -153,6 +153,14 @@ const char *aarbf_status_string(int status);
  * gathering, 1 one lane per pixel with a BVH point query per step.  Both
  * produce identical images; no counterpart in the reference. */
 int aarbf_set_render_mode(aarbf_context *ctx, int mode);
+/* Schedule parameters of the default renderer (none changes the image):
+ * AARBF_TUNE_STEP_WAVES: occupancy target of the step kernel in waves per
+ * SIMD (0 = compiler's choice, or 4..6); AARBF_TUNE_STEP_CAP: tracker steps
+ * per ray per wavefront round (16..65536, default 256).  No counterpart in
+ * the reference. */
+#define AARBF_TUNE_STEP_WAVES 1
+#define AARBF_TUNE_STEP_CAP 2
+int aarbf_set_tuning(aarbf_context *ctx, int param, int value);
 /* Enable (1) or disable (0) render statistics; read them back (and reset
  * them) with aarbf_render_stats_read. */
 int aarbf_set_stats(aarbf_context *ctx, int enable);
