@@ -76,7 +76,7 @@ struct aarbf_context {
   int render_mode = 0;           // 0 persistent segment-gather, 1 per-pixel point queries
   int cus = 0;                   // compute units of the device
   int tune_step_waves = 0;       // occupancy target of the step kernel (0 = compiler's choice)
-  int tune_step_cap = 256;       // tracker steps per ray per wavefront round
+  int tune_step_cap = 4096;      // tracker steps per ray per wavefront round
   // grow-only device scratch
   void *scratch = nullptr;
   size_t scratch_bytes = 0;

@@ -474,7 +474,7 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
     if (STATS) {
       uint32_t cost = c.nodes - nodes0;
       uint32_t b = cost ? 31u - (uint32_t)__clz(cost) : 0u;
-      atomicAdd((unsigned long long *)&A.stats[8 + min(b, 23u)], 1ull);
+      atomicAdd((unsigned long long *)&A.stats[8 + min(b, 17u)], 1ull);
     }
     float seg_end;
     if (t_cut <= t0 + 2.0f * A.unit) { // crowded: point queries for a short stretch
@@ -553,6 +553,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
   const uint32_t n_a = *countA;
   uint32_t gid = blockIdx.x * kSBlock + threadIdx.x;
   Counters c = {};
+  uint64_t simt[3] = {0, 0, 0};
   int outcome = 0; // 0 nothing, 1 keep active, 2 active + gather, 3 dense queue
   uint32_t slot = 0;
   if (gid < n_a) {
@@ -616,6 +617,15 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
 
     uint32_t steps = 0;
     for (;;) {
+      if (STATS) { // SIMT accounting: iterations, lanes in them, iterations touching candidates
+        uint64_t live = __ballot(1);
+        if ((uint32_t)__ffsll((unsigned long long)live) - 1u == (threadIdx.x & 63u)) {
+          simt[0] += 1;
+          simt[1] += (uint32_t)__popcll(live);
+        }
+        uint64_t touch = __ballot(act != 0 || (L.flags & F_PENDING) != 0 && DENSE);
+        if (touch && (uint32_t)__ffsll((unsigned long long)live) - 1u == (threadIdx.x & 63u)) simt[2] += 1;
+      }
       bool pending = (L.flags & F_PENDING) != 0;
       bool end = false;
       if (!pending) {
@@ -740,6 +750,13 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
   if (outcome == 2) push(outG, &counts[1], slot);
   if (outcome == 3) push(outD, countD, slot);
   flush_stats<STATS>(A, c);
+  if (STATS)
+    for (int k = 0; k < 3; ++k) {
+      uint64_t sum = simt[k];
+      for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+      if ((threadIdx.x & 63) == 0 && sum)
+        atomicAdd((unsigned long long *)&A.stats[(DENSE ? 29 : 26) + k], (unsigned long long)sum);
+    }
 }
 
 __global__ void wf_zero(uint32_t *c, int n) {

@@ -45,7 +45,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mode", type=int, default=0, help="0 wavefront gather/step kernels, 1 point queries")
     ap.add_argument("--step-waves", type=int, default=0)
-    ap.add_argument("--step-cap", type=int, default=256)
+    ap.add_argument("--step-cap", type=int, default=4096)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "render_traffic.json"),
                     help="HBM bytes per render launch measured with rocprofv3 --pmc")
@@ -122,7 +122,11 @@ def main() -> int:
     st = ctx.read_stats(reset=True)
     hist = ctx.read_histogram(reset=True)
     ctx.set_stats(False)
-    log("gather walk cost histogram (log2 nodes -> count): " + str({b: v for b, v in enumerate(hist) if v}))
+    log("gather walk cost histogram (log2 nodes -> count): " + str({b: v for b, v in enumerate(hist[:18]) if v}))
+    for name, o in (("step", 18), ("dense", 21)):
+        if hist[o]:
+            log(f"{name} kernel: {hist[o]} wave iterations, lane utilisation {hist[o + 1] / (64 * hist[o]):.3f}, "
+                f"iterations testing candidates {hist[o + 2] / hist[o]:.3f}")
     samples = max(1, st["samples"])
     # bytes per launch: node/leaf boxes 32 B, particle position+radius 16 B,
     # colour payload 16 B per covering particle, accum read+write and image write 48 B per pixel

@@ -156,7 +156,7 @@ int aarbf_set_render_mode(aarbf_context *ctx, int mode);
 /* Schedule parameters of the default renderer (none changes the image):
  * AARBF_TUNE_STEP_WAVES: occupancy target of the step kernel in waves per
  * SIMD (0 = compiler's choice, or 4..6); AARBF_TUNE_STEP_CAP: tracker steps
- * per ray per wavefront round (16..65536, default 256).  No counterpart in
+ * per ray per wavefront round (16..65536, default 4096).  No counterpart in
  * the reference. */
 #define AARBF_TUNE_STEP_WAVES 1
 #define AARBF_TUNE_STEP_CAP 2
@@ -166,8 +166,11 @@ int aarbf_set_tuning(aarbf_context *ctx, int param, int value);
 int aarbf_set_stats(aarbf_context *ctx, int enable);
 int aarbf_render_stats_read(aarbf_context *ctx, aarbf_render_stats *out, int reset);
 /* Walk-cost histogram of the segment gathers since the last reset (with
- * statistics enabled): out[b] (b < 24) counts gathers that tested between
- * 2^b and 2^(b+1)-1 BVH nodes.  No counterpart in the reference. */
+ * statistics enabled): out[b] (b < 18) counts gathers that tested between
+ * 2^b and 2^(b+1)-1 BVH nodes; out[18..20] = step-kernel wave iterations,
+ * lanes active in them, iterations in which some lane tested candidates;
+ * out[21..23] the same for the dense-segment kernel.  No counterpart in
+ * the reference. */
 int aarbf_render_histogram(aarbf_context *ctx, uint64_t *out, int reset);
 /* Writes sizes and offsets of the structs above into out[0..n) and returns
  * how many there are; bindings in other languages use it to check their
