@@ -600,6 +600,10 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
       }
       t_enter = eptr < n ? ltin[leo[eptr * kSBlock] * kSBlock] : AA_INF_F;
     };
+    // A draw that lands before lim with no active particle is a step in
+    // empty space: the query at it covers nothing, so only t and i move.
+    const float lim0 = fminf(L.tmax, L.seg_end);
+    float lim = lim0;
     auto leave = [&](float t) {
       t_exit = AA_INF_F;
       uint32_t m = act;
@@ -614,6 +618,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
       }
     };
     enter(L.t);
+    lim = fminf(lim0, t_enter);
 
     uint32_t steps = 0;
     for (;;) {
@@ -626,7 +631,27 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
         uint64_t touch = __ballot(act != 0 || (L.flags & F_PENDING) != 0 && DENSE);
         if (touch && (uint32_t)__ffsll((unsigned long long)live) - 1u == (threadIdx.x & 63u)) simt[2] += 1;
       }
-      bool pending = (L.flags & F_PENDING) != 0;
+      const bool pending = (L.flags & F_PENDING) != 0;
+      // The next free-flight draw, computed by every lane (nearly all of
+      // them step in each iteration) and committed only where a step is
+      // taken.  Marching steps a fixed distance and draws nothing.
+      uint32_t ns = L.trng;
+      float tn;
+      if (marching) {
+        tn = L.t + A.unit;
+      } else {
+        float xi = aa_lcg_next(&ns);
+        tn = L.t - aa_log_step(1.0f - xi) * A.unit;
+      }
+      const bool can = !pending && L.i < A.max_depth && steps < step_cap;
+      if (!DENSE && can && act == 0u && tn < lim) {
+        L.trng = ns;
+        L.t = tn;
+        ++L.i;
+        ++steps;
+        if (STATS) c.steps++;
+        continue;
+      }
       bool end = false;
       if (!pending) {
         if (!(L.i < A.max_depth)) {
@@ -637,12 +662,8 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
         } else {
           ++steps;
           if (STATS) c.steps++;
-          if (marching) {
-            L.t = L.t + A.unit;
-          } else {
-            float xi = aa_lcg_next(&L.trng);
-            L.t = L.t - aa_logf(1.0f - xi) * A.unit;
-          }
+          L.trng = ns;
+          L.t = tn;
           if (L.t >= L.tmax) end = true;
           else L.flags |= F_PENDING;
         }
@@ -658,7 +679,10 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
         if (DENSE) {
           point_query_q<kSBlock, STATS>(A, stk, x, y, z, pl, c);
         } else {
-          if (L.t >= t_enter) enter(L.t);
+          if (L.t >= t_enter) {
+            enter(L.t);
+            lim = fminf(lim0, t_enter);
+          }
           if (L.t > t_exit) leave(L.t);
           pl.count = 0;
           pl.dens = 0.0f;

@@ -99,6 +99,35 @@ AA_FN float aa_logf(float x) {
   return AA_FMAF(ef, 0.693145751953125f, r);
 }
 
+/* aa_logf restricted to the arguments a tracker step can produce, 1 - xi
+ * with xi an LCG draw in [0, 1]: 0 or a normal number in [2^-24, 1].  Same
+ * operations as aa_logf on that domain, without its branches (the result
+ * is identical bit for bit; checked exhaustively over the domain). */
+AA_FN float aa_log_step(float x) {
+  uint32_t u = aa_f2u(x);
+  int e = (int)(u >> 23) - 127;
+  uint32_t mb = (u & 0x007fffffu) | 0x3f800000u;
+  const int hi = mb > 0x3fb504f3u;
+  mb = hi ? mb - 0x00800000u : mb;
+  e = hi ? e + 1 : e;
+  float f = aa_u2f(mb) - 1.0f;
+  float q = -0.07866667956113815f;
+  q = AA_FMAF(q, f, 0.12821750342845917f);
+  q = AA_FMAF(q, f, -0.131016343832016f);
+  q = AA_FMAF(q, f, 0.1418900042772293f);
+  q = AA_FMAF(q, f, -0.1662759929895401f);
+  q = AA_FMAF(q, f, 0.2000196874141693f);
+  q = AA_FMAF(q, f, -0.2500077188014984f);
+  q = AA_FMAF(q, f, 0.33333322405815125f);
+  q = AA_FMAF(q, f, -0.4999999701976776f);
+  float f2 = f * f;
+  /* x = 0: e = -inf carries -inf through both fused terms */
+  float ef = u == 0u ? -AA_INF_F : (float)e;
+  float lo = AA_FMAF(ef, 1.4286068203094172e-06f, f2 * q);
+  float r = lo + f;
+  return AA_FMAF(ef, 0.693145751953125f, r);
+}
+
 /* e^x.  k = rint(x / ln2), r = x - k*ln2 (two-part, fused), e^r = 1 + r +
  * r^2 * P(r) with P a degree-5 fit on |r| <= ln2/2, then scaled by 2^k
  * through the exponent field (two factors when 2^k is outside the normal

@@ -945,6 +945,18 @@ ORC_EXPORT void orc_math(int which, const float *x, const float *y, float *out, 
   }
 }
 
+/* Number of arguments of the tracker-step logarithm's domain (0 and every
+ * binary32 in [2^-24, 1]) where aa_log_step and aa_logf differ bitwise. */
+ORC_EXPORT uint64_t orc_log_step_mismatches(void) {
+  uint64_t bad = 0;
+  if (aa_f2u(aa_log_step(0.0f)) != aa_f2u(aa_logf(0.0f))) ++bad;
+  for (uint32_t u = 0x33800000u; u <= 0x3f800000u; ++u) {
+    float x = aa_u2f(u);
+    if (aa_f2u(aa_log_step(x)) != aa_f2u(aa_logf(x))) ++bad;
+  }
+  return bad;
+}
+
 /* get_rng(frame, pixel, dims).state (rng.h:63-70) and the first k draws. */
 ORC_EXPORT uint32_t orc_rng_stream(uint32_t frame, uint32_t px, uint32_t py, uint32_t width, float *draws,
                                    uint32_t k) {
