@@ -27,7 +27,18 @@
 // back to the gather queue; the host alternates the two kernels until no
 // ray is left.  Both kernels are dense (every lane has the same kind of
 // work), which is what the one-kernel schedule could not achieve.
+//
+// Slot k renders the samples of pixel k (tile order) one after another,
+// which keeps the rays of a wave on neighbouring pixels.  Samples are
+// claimed through per-pixel counters, so a slot whose pixel has no samples
+// left helps pixels that still have some (visited in order through a
+// shared pointer); without this the pixels with the longest paths would
+// leave the last rounds nearly empty.  Each sample writes its colour to a
+// per-sample buffer and wf_fold forms the running mean of every pixel in
+// sample order, exactly as the sequential loop does.
 #include "render_common.h"
+
+#include <algorithm>
 
 namespace aarbf {
 namespace {
@@ -35,6 +46,7 @@ namespace {
 constexpr uint32_t kCand = 16;
 constexpr int kGBlock = 128; // gather kernel block
 constexpr int kSBlock = 64;  // step kernel block (one wave)
+constexpr uint64_t kPassSamples = 64; // samples per colour-buffer pass
 
 constexpr uint32_t F_SHADOW = 1u, F_PENDING = 2u, F_DENSE = 4u;
 // F_HEAD: the segment is the pixel's head list (shared by the primary rays
@@ -46,7 +58,7 @@ constexpr uint32_t F_HEAD = 8u, F_HEADGATHER = 16u;
 struct WFState {
   float4 *od0;  // o.xyz, t
   float4 *od1;  // d.xyz, tmax
-  float4 *acc;  // running mean of the pixel
+  uint32_t *pix; // pixel (tile order) of the slot's sample
   float4 *alb;  // primary albedo
   uint4 *u;     // trng, i, s | flags << 16, seg_n
   float4 *f;    // LHS, RHS, shw, r2
@@ -55,18 +67,24 @@ struct WFState {
   float *ctin;    // [kCand][nslots]
   float *ctout;   // [kCand][nslots]
   uint8_t *ceo;   // [kCand][nslots] candidate positions ordered by tin
-  uint32_t *hidx; // head list of the pixel's primary ray, same layout
+  uint32_t *hidx; // [kCand][npix] head list of the pixel's primary ray
   float *htin, *htout;
   uint8_t *heo;
-  float4 *head;   // head t0, head segment end, n (bits), 1 = valid | 2 = dense (bits)
-  uint32_t nslots;
+  float4 *head;   // [npix] head t0, head segment end, n (bits), 1 = valid | 2 = dense (bits)
+  float4 *col;    // [s1 - s0][npix] colour of each sample, w = its noise value
+  uint32_t *next_s; // [npix] next unclaimed sample of each pixel
+  uint32_t *visit;  // next pixel for slots that help
+  uint32_t nslots; // == npix: slot k owns pixel k
+  uint32_t npix;  // pixels in tile order (8x8 tiles, padded)
   uint32_t tiles_x;
+  uint32_t s0, s1; // samples of this pass
 };
 
 struct Lane {
   float3 o, d;
   float t, tmax;
-  float4 acc, alb;
+  uint32_t pix;
+  float4 alb;
   uint32_t trng, i, s, flags, seg_n;
   float LHS, RHS, shw, r2;
   float seg_t0, seg_end, rx;
@@ -78,7 +96,7 @@ __device__ inline void load_lane(const WFState &S, uint32_t slot, Lane &L) {
   L.t = a.w;
   L.d = make_float3(b.x, b.y, b.z);
   L.tmax = b.w;
-  L.acc = S.acc[slot];
+  L.pix = S.pix[slot];
   L.alb = S.alb[slot];
   uint4 u = S.u[slot];
   L.trng = u.x;
@@ -100,15 +118,15 @@ __device__ inline void load_lane(const WFState &S, uint32_t slot, Lane &L) {
 __device__ inline void save_lane(const WFState &S, uint32_t slot, const Lane &L) {
   S.od0[slot] = make_float4(L.o.x, L.o.y, L.o.z, L.t);
   S.od1[slot] = make_float4(L.d.x, L.d.y, L.d.z, L.tmax);
-  S.acc[slot] = L.acc;
+  S.pix[slot] = L.pix;
   S.alb[slot] = L.alb;
   S.u[slot] = make_uint4(L.trng, L.i, (L.s & 0xffffu) | (L.flags << 16), L.seg_n);
   S.f[slot] = make_float4(L.LHS, L.RHS, L.shw, L.r2);
   S.seg[slot] = make_float4(L.seg_t0, L.seg_end, L.rx, 0.0f);
 }
 
-__device__ inline void slot_pixel(const WFState &S, uint32_t slot, int &px, int &py) {
-  uint32_t tile = slot >> 6, w = slot & 63u;
+__device__ inline void pix_xy(const WFState &S, uint32_t pix, int &px, int &py) {
+  uint32_t tile = pix >> 6, w = pix & 63u;
   px = (int)((tile % S.tiles_x) * 8u + (w & 7u));
   py = (int)((tile / S.tiles_x) * 8u + (w >> 3));
 }
@@ -160,9 +178,11 @@ __device__ inline bool start_sample(const RenderArgs &A, int px, int py, Lane &L
   return true;
 }
 
-// Shading, compositing over black and the running mean (':316-328').
+// Shading and compositing over black (':316-324'); the colour goes to the
+// sample's entry of the colour buffer, the running mean (':325-328') is
+// formed by wf_fold.
 template <bool STATS>
-__device__ inline void finish_sample(const RenderArgs &A, Lane &L, float vis, Counters &c) {
+__device__ inline void finish_sample(const RenderArgs &A, const WFState &S, Lane &L, float vis, Counters &c) {
   float4 color = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (L.alb.w == 1.0f) {
     color.x = (L.alb.x * vis) * (1.0f - A.amb) + L.alb.x * A.amb;
@@ -181,28 +201,60 @@ __device__ inline void finish_sample(const RenderArgs &A, Lane &L, float vis, Co
     color.z = (color.z * color.w + (0.0f * 1.0f) * (1.0f - color.w)) / ra;
     color.w = ra;
   }
-  uint32_t aid = (A.accumID + L.s) & 0xffffu;
-  float w1 = 1.0f / (float)aid;
-  float w0 = (float)((int)aid - 1) / (float)aid;
-  float4 fin;
-  fin.x = w1 * color.x + w0 * L.acc.x;
-  fin.y = w1 * color.y + w0 * L.acc.y;
-  fin.z = w1 * color.z + w0 * L.acc.z;
-  fin.w = w1 * color.w + w0 * L.acc.w;
-  L.acc = fin;
-  L.s += 1;
+  // ra is 1 for any alpha in {0, 1}, so only rgb is kept; w holds the
+  // sample's noise value for the showNoise image
+  S.col[(uint64_t)(L.s - S.s0) * S.npix + L.pix] = make_float4(color.x, color.y, color.z, L.rx);
 }
 
-// Starts samples until one needs tracking (true) or the pixel is done, in
-// which case its outputs are written (false).
-// Returns 2 when the new ray needs a gather, 1 when it can step at once
-// (head list of the pixel already built), 0 when the pixel is done.
+constexpr uint32_t kNoPix = 0xffffffffu;
+
+// Claims the next sample for a slot: first those of the pixel it is on
+// (its own, pixel == slot, or the one it helps), then, visiting pixels in
+// order, those of the first pixel with samples left.  Every sample is
+// claimed exactly once (atomic per-pixel counters).  Returns false when
+// the slot finds no work (all pixels visited, or after kProbes pixels
+// without samples left: the owners finish them).
+__device__ inline bool take_item(const WFState &S, Lane &L) {
+  constexpr int kProbes = 32;
+  uint32_t p = L.pix;
+  for (int probe = 0; probe < kProbes; ++probe) {
+    if (p != kNoPix) {
+      uint32_t s = atomicAdd(&S.next_s[p], 1u);
+      if (s < S.s1) {
+        L.pix = p;
+        L.s = s;
+        return true;
+      }
+    }
+    // next pixel to visit, one atomic per wave for the lanes here
+    uint64_t mask = __ballot(1);
+    int lane = threadIdx.x & 63;
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(S.visit, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t q = base + (uint32_t)__popcll(mask & lt);
+    if (q >= S.npix) break;
+    p = __hip_atomic_load(&S.next_s[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < S.s1 ? q : kNoPix;
+  }
+  L.pix = kNoPix;
+  return false;
+}
+
+// Starts samples until one needs tracking.  Returns 2 when the new ray
+// needs a gather, 1 when it can step at once (the pixel's head list is
+// built), 0 when the slot has no work left.  The head list is built by the
+// pixel's sample 0; a later sample that starts before it is ready gathers
+// its own list.
 template <bool STATS>
-__device__ inline int next_ray(const RenderArgs &A, const WFState &S, uint32_t slot, int px, int py, Lane &L,
-                               Counters &c) {
-  while (L.s < A.spp) {
+__device__ inline int next_ray(const RenderArgs &A, const WFState &S, Lane &L, Counters &c) {
+  for (;;) {
+    if (!take_item(S, L)) return 0;
+    int px, py;
+    pix_xy(S, L.pix, px, py);
     if (start_sample(A, px, py, L)) {
-      float4 h = S.head[slot];
+      float4 h = S.head[L.pix];
       uint32_t hf = __float_as_uint(h.w);
       if (hf & 1u) {
         L.flags |= F_HEAD | ((hf & 2u) ? F_DENSE : 0u);
@@ -211,16 +263,12 @@ __device__ inline int next_ray(const RenderArgs &A, const WFState &S, uint32_t s
         L.seg_n = __float_as_uint(h.z);
         return 1;
       }
-      L.flags |= F_HEADGATHER;
+      if (L.s == 0) L.flags |= F_HEADGATHER;
       L.seg_end = -AA_INF_F;
       return 2;
     }
-    finish_sample<STATS>(A, L, 1.0f, c);
+    finish_sample<STATS>(A, S, L, 1.0f, c);
   }
-  uint64_t ofs = (uint64_t)px + (uint64_t)A.W * py;
-  A.accum[ofs] = L.acc;
-  A.image[ofs] = A.showNoise ? make_float4(L.rx, L.rx, L.rx, 1.0f) : L.acc;
-  return 0;
 }
 
 __device__ inline void push(uint32_t *list, uint32_t *count, uint32_t slot) {
@@ -246,6 +294,23 @@ __device__ inline void flush_stats(const RenderArgs &A, const Counters &c) {
 }
 
 // ---- initial rays ------------------------------------------------------------
+// Head lists are invalid until a sample 0 builds them (start of a call).
+__global__ __launch_bounds__(256) void wf_head_reset(WFState S) {
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p < S.npix) S.head[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Per pass: every pixel on the image has samples s0..s1-1 to claim, the
+// padding pixels of partial tiles none.
+__global__ __launch_bounds__(256) void wf_pass_reset(RenderArgs A, WFState S) {
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p >= S.npix) return;
+  int px, py;
+  pix_xy(S, p, px, py);
+  S.next_s[p] = (px < A.W && py < A.H) ? S.s0 : S.s1;
+}
+
+// Slot k starts on its own pixel k.
 template <bool STATS>
 __global__ __launch_bounds__(256) void wf_init(RenderArgs A, WFState S, uint32_t *listA, uint32_t *listG,
                                                uint32_t *counts) {
@@ -253,26 +318,48 @@ __global__ __launch_bounds__(256) void wf_init(RenderArgs A, WFState S, uint32_t
   Counters c = {};
   bool push_a = false, push_g = false;
   if (slot < S.nslots) {
-    int px, py;
-    slot_pixel(S, slot, px, py);
-    if (px < A.W && py < A.H) {
-      Lane L;
-      L.acc = A.accum[(uint64_t)px + (uint64_t)A.W * py];
-      L.s = 0;
-      L.seg_n = 0;
-      L.seg_t0 = 0.0f;
-      S.head[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      int r = next_ray<STATS>(A, S, slot, px, py, L, c);
-      if (r) {
-        save_lane(S, slot, L);
-        push_a = true;
-        push_g = r == 2;
-      }
+    Lane L;
+    L.pix = slot;
+    L.seg_n = 0;
+    L.seg_t0 = 0.0f;
+    int r = next_ray<STATS>(A, S, L, c);
+    if (r) {
+      save_lane(S, slot, L);
+      push_a = true;
+      push_g = r == 2;
     }
   }
   if (push_a) push(listA, &counts[0], slot);
   if (push_g) push(listG, &counts[1], slot);
   flush_stats<STATS>(A, c);
+}
+
+// Running mean of every pixel over the pass's samples, in sample order
+// (deviceCodeRBF.hlsl:325-335).
+__global__ __launch_bounds__(256) void wf_fold(RenderArgs A, WFState S) {
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p >= S.npix) return;
+  int px, py;
+  pix_xy(S, p, px, py);
+  if (px >= A.W || py >= A.H) return;
+  uint64_t ofs = (uint64_t)px + (uint64_t)A.W * py;
+  float4 acc = A.accum[ofs];
+  float rx = 0.0f;
+  for (uint32_t s = S.s0; s < S.s1; ++s) {
+    float4 col = S.col[(uint64_t)(s - S.s0) * S.npix + p];
+    rx = col.w;
+    uint32_t aid = (A.accumID + s) & 0xffffu;
+    float w1 = 1.0f / (float)aid;
+    float w0 = (float)((int)aid - 1) / (float)aid;
+    float4 fin;
+    fin.x = w1 * col.x + w0 * acc.x;
+    fin.y = w1 * col.y + w0 * acc.y;
+    fin.z = w1 * col.z + w0 * acc.z;
+    fin.w = w1 * 1.0f + w0 * acc.w;
+    acc = fin;
+  }
+  A.accum[ofs] = acc;
+  A.image[ofs] = A.showNoise ? make_float4(rx, rx, rx, 1.0f) : acc;
 }
 
 // ---- gather --------------------------------------------------------------------
@@ -297,6 +384,7 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
     uint4 u = S.u[slot];
     uint32_t flags = u.z >> 16;
     const bool head = (flags & F_HEADGATHER) != 0;
+    const uint32_t pix = S.pix[slot];
     const float t0 = head ? S.seg[slot].x : a0.w, tmax = a1.w;
     SegQuery q;
     q.ax = o.x + t0 * d.x;
@@ -486,6 +574,7 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
       uint32_t *oidx = head ? S.hidx : S.cidx;
       float *otin = head ? S.htin : S.ctin, *otout = head ? S.htout : S.ctout;
       uint8_t *oeo = head ? S.heo : S.ceo;
+      const uint64_t row = head ? pix : slot, stride = head ? S.npix : S.nslots;
       seg_end = t_cut;
       // ascending particle order (insertion sort, n <= kCand)
       for (uint32_t a = 1; a < n; ++a) {
@@ -510,15 +599,15 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
           float tj = gtin[j * kGBlock];
           r += (tj < tk) || (tj == tk && j < k);
         }
-        oeo[(uint64_t)r * S.nslots + slot] = (uint8_t)k;
-        oidx[(uint64_t)k * S.nslots + slot] = gi[k * kGBlock];
-        otin[(uint64_t)k * S.nslots + slot] = tk;
-        otout[(uint64_t)k * S.nslots + slot] = gtout[k * kGBlock];
+        oeo[r * stride + row] = (uint8_t)k;
+        oidx[k * stride + row] = gi[k * kGBlock];
+        otin[k * stride + row] = tk;
+        otout[k * stride + row] = gtout[k * kGBlock];
       }
     }
     if (head) {
       flags = (flags & ~F_HEADGATHER) | F_HEAD;
-      S.head[slot] = make_float4(t0, seg_end, __uint_as_float(n),
+      S.head[pix] = make_float4(t0, seg_end, __uint_as_float(n),
                                  __uint_as_float(1u | ((flags & F_DENSE) ? 2u : 0u)));
     } else {
       flags &= ~F_HEAD;
@@ -563,8 +652,6 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
     }
   }
   if (gid < n_a && outcome == 0) {
-    int px, py;
-    slot_pixel(S, slot, px, py);
     Lane L;
     load_lane(S, slot, L);
     uint32_t *li = s_idx + threadIdx.x;
@@ -577,11 +664,12 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
       const uint32_t *iidx = hd ? S.hidx : S.cidx;
       const float *itin = hd ? S.htin : S.ctin, *itout = hd ? S.htout : S.ctout;
       const uint8_t *ieo = hd ? S.heo : S.ceo;
+      const uint64_t row = hd ? L.pix : slot, stride = hd ? S.npix : S.nslots;
       for (uint32_t k = 0; k < n; ++k) {
-        li[k * kSBlock] = iidx[(uint64_t)k * S.nslots + slot];
-        ltin[k * kSBlock] = itin[(uint64_t)k * S.nslots + slot];
-        ltout[k * kSBlock] = itout[(uint64_t)k * S.nslots + slot];
-        leo[k * kSBlock] = ieo[(uint64_t)k * S.nslots + slot];
+        li[k * kSBlock] = iidx[k * stride + row];
+        ltin[k * kSBlock] = itin[k * stride + row];
+        ltout[k * kSBlock] = itout[k * stride + row];
+        leo[k * kSBlock] = ieo[k * stride + row];
       }
     }
     // active set at the current t
@@ -763,8 +851,8 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
         break;
       }
       float vis = (L.flags & F_SHADOW) ? 1.0f - L.shw : 1.0f;
-      finish_sample<STATS>(A, L, vis, c);
-      outcome = next_ray<STATS>(A, S, slot, px, py, L, c);
+      finish_sample<STATS>(A, S, L, vis, c);
+      outcome = next_ray<STATS>(A, S, L, c);
       break;
     }
     if (outcome) save_lane(S, slot, L);
@@ -787,59 +875,18 @@ __global__ void wf_zero(uint32_t *c, int n) {
   if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
 }
 
+// Clears the list counts and the visiting pointer.
+__global__ void wf_reset_counts(uint32_t *c) {
+  if (threadIdx.x < 16) c[threadIdx.x] = 0u;
+}
+
 } // namespace
 
-// Host driver of the wavefront rounds.  Grows the slot state in the
-// context's scratch pool; reads the two queue lengths back after each round.
-int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
-  const uint32_t tiles_x = (uint32_t)(A.W + 7) / 8, tiles_y = (uint32_t)(A.H + 7) / 8;
-  const uint64_t nslots = (uint64_t)tiles_x * tiles_y * 64u;
-  AARBF_REQUIRE(ctx, nslots < (1ull << 31), "aarbf_render_rbf: image too large");
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    size_t o = off;
-    off += (bytes + 255) & ~(size_t)255;
-    return o;
-  };
-  size_t o_od0 = take(16 * nslots), o_od1 = take(16 * nslots), o_acc = take(16 * nslots), o_alb = take(16 * nslots),
-         o_u = take(16 * nslots), o_f = take(16 * nslots), o_seg = take(16 * nslots),
-         o_cidx = take(4 * kCand * nslots), o_ctin = take(4 * kCand * nslots), o_ctout = take(4 * kCand * nslots),
-         o_ceo = take(kCand * nslots), o_hidx = take(4 * kCand * nslots), o_htin = take(4 * kCand * nslots),
-         o_htout = take(4 * kCand * nslots), o_heo = take(kCand * nslots), o_head = take(16 * nslots), o_l0 = take(4 * nslots), o_l1 = take(4 * nslots), o_l2 = take(4 * nslots),
-         o_l3 = take(4 * nslots), o_l4 = take(4 * nslots), o_cnt = take(64);
-  char *base = (char *)scratch(ctx, off);
-  if (!base) return set_error(ctx, AARBF_ERR_OUT_OF_MEMORY, "aarbf_render_rbf: wavefront state");
-  WFState S;
-  S.od0 = (float4 *)(base + o_od0);
-  S.od1 = (float4 *)(base + o_od1);
-  S.acc = (float4 *)(base + o_acc);
-  S.alb = (float4 *)(base + o_alb);
-  S.u = (uint4 *)(base + o_u);
-  S.f = (float4 *)(base + o_f);
-  S.seg = (float4 *)(base + o_seg);
-  S.cidx = (uint32_t *)(base + o_cidx);
-  S.ctin = (float *)(base + o_ctin);
-  S.ctout = (float *)(base + o_ctout);
-  S.ceo = (uint8_t *)(base + o_ceo);
-  S.hidx = (uint32_t *)(base + o_hidx);
-  S.htin = (float *)(base + o_htin);
-  S.htout = (float *)(base + o_htout);
-  S.heo = (uint8_t *)(base + o_heo);
-  S.head = (float4 *)(base + o_head);
-  S.nslots = (uint32_t)nslots;
-  S.tiles_x = tiles_x;
-  uint32_t *lists[4] = {(uint32_t *)(base + o_l0), (uint32_t *)(base + o_l1), (uint32_t *)(base + o_l2),
-                        (uint32_t *)(base + o_l3)};
-  uint32_t *listD = (uint32_t *)(base + o_l4);
-  uint32_t *cnt = (uint32_t *)(base + o_cnt); // [0..1] and [2..3]: (A, G) of the two list pairs; [4]: dense
+// The gather / step rounds of one pass, until no ray is left.
+static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S, uint32_t *const lists[4],
+                      uint32_t *listD, uint32_t *cnt) {
   const bool st = ctx->stats;
-  wf_zero<<<1, 64, 0, ctx->stream>>>(cnt, 16);
-  uint32_t init_blocks = (uint32_t)((nslots + 255) / 256);
-  if (st)
-    wf_init<true><<<init_blocks, 256, 0, ctx->stream>>>(A, S, lists[0], lists[1], cnt);
-  else
-    wf_init<false><<<init_blocks, 256, 0, ctx->stream>>>(A, S, lists[0], lists[1], cnt);
-  AARBF_CHECK_LAUNCH(ctx, "wf_init");
+  const uint64_t nslots = S.nslots;
   uint32_t h[2];
   AARBF_HIP(ctx, hipMemcpyAsync(h, cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
   AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -888,6 +935,81 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
     nA = h[0];
     nG = h[1];
     cur = 2 - cur;
+  }
+  return AARBF_OK;
+}
+
+
+// Host driver of the wavefront rounds.  Grows the slot state in the
+// context's scratch pool; reads the two queue lengths back after each round.
+// Samples are rendered in passes of at most kPassSamples (the colour buffer
+// holds one pass), each pass folded into accum before the next.
+int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
+  const uint32_t tiles_x = (uint32_t)(A.W + 7) / 8, tiles_y = (uint32_t)(A.H + 7) / 8;
+  const uint64_t npix = (uint64_t)tiles_x * tiles_y * 64u;
+  const uint64_t nslots = npix;
+  AARBF_REQUIRE(ctx, npix < (1ull << 28), "aarbf_render_rbf: image too large");
+  const uint64_t chunk = std::min<uint64_t>(A.spp, kPassSamples);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  size_t o_od0 = take(16 * nslots), o_od1 = take(16 * nslots), o_pix = take(4 * nslots), o_alb = take(16 * nslots),
+         o_u = take(16 * nslots), o_f = take(16 * nslots), o_seg = take(16 * nslots),
+         o_cidx = take(4 * kCand * nslots), o_ctin = take(4 * kCand * nslots), o_ctout = take(4 * kCand * nslots),
+         o_ceo = take(kCand * nslots), o_hidx = take(4 * kCand * npix), o_htin = take(4 * kCand * npix),
+         o_htout = take(4 * kCand * npix), o_heo = take(kCand * npix), o_head = take(16 * npix),
+         o_col = take(16 * npix * chunk), o_next = take(4 * npix), o_l0 = take(4 * nslots), o_l1 = take(4 * nslots), o_l2 = take(4 * nslots),
+         o_l3 = take(4 * nslots), o_l4 = take(4 * nslots), o_cnt = take(64);
+  char *base = (char *)scratch(ctx, off);
+  if (!base) return set_error(ctx, AARBF_ERR_OUT_OF_MEMORY, "aarbf_render_rbf: wavefront state");
+  WFState S;
+  S.od0 = (float4 *)(base + o_od0);
+  S.od1 = (float4 *)(base + o_od1);
+  S.pix = (uint32_t *)(base + o_pix);
+  S.alb = (float4 *)(base + o_alb);
+  S.u = (uint4 *)(base + o_u);
+  S.f = (float4 *)(base + o_f);
+  S.seg = (float4 *)(base + o_seg);
+  S.cidx = (uint32_t *)(base + o_cidx);
+  S.ctin = (float *)(base + o_ctin);
+  S.ctout = (float *)(base + o_ctout);
+  S.ceo = (uint8_t *)(base + o_ceo);
+  S.hidx = (uint32_t *)(base + o_hidx);
+  S.htin = (float *)(base + o_htin);
+  S.htout = (float *)(base + o_htout);
+  S.heo = (uint8_t *)(base + o_heo);
+  S.head = (float4 *)(base + o_head);
+  S.col = (float4 *)(base + o_col);
+  S.next_s = (uint32_t *)(base + o_next);
+  S.nslots = (uint32_t)nslots;
+  S.npix = (uint32_t)npix;
+  S.tiles_x = tiles_x;
+  uint32_t *lists[4] = {(uint32_t *)(base + o_l0), (uint32_t *)(base + o_l1), (uint32_t *)(base + o_l2),
+                        (uint32_t *)(base + o_l3)};
+  uint32_t *listD = (uint32_t *)(base + o_l4);
+  // [0..1] and [2..3]: (A, G) counts of the two list pairs; [4]: dense; [8]: visiting pointer
+  uint32_t *cnt = (uint32_t *)(base + o_cnt);
+  S.visit = cnt + 8;
+  const bool st = ctx->stats;
+  const uint32_t pix_blocks = (uint32_t)((npix + 255) / 256), init_blocks = (uint32_t)((nslots + 255) / 256);
+  wf_head_reset<<<pix_blocks, 256, 0, ctx->stream>>>(S);
+  for (uint32_t s0 = 0; s0 < A.spp; s0 += (uint32_t)chunk) {
+    S.s0 = s0;
+    S.s1 = (uint32_t)std::min<uint64_t>(A.spp, s0 + chunk);
+    wf_reset_counts<<<1, 64, 0, ctx->stream>>>(cnt);
+    wf_pass_reset<<<pix_blocks, 256, 0, ctx->stream>>>(A, S);
+    if (st)
+      wf_init<true><<<init_blocks, 256, 0, ctx->stream>>>(A, S, lists[0], lists[1], cnt);
+    else
+      wf_init<false><<<init_blocks, 256, 0, ctx->stream>>>(A, S, lists[0], lists[1], cnt);
+    AARBF_CHECK_LAUNCH(ctx, "wf_init");
+    int rc = run_rounds(ctx, A, S, lists, listD, cnt);
+    if (rc != AARBF_OK) return rc;
+    wf_fold<<<pix_blocks, 256, 0, ctx->stream>>>(A, S);
+    AARBF_CHECK_LAUNCH(ctx, "wf_fold");
   }
   return AARBF_OK;
 }
