@@ -658,20 +658,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
     float *ltin = s_tin + threadIdx.x, *ltout = s_tout + threadIdx.x;
     uint8_t *leo = s_eo + threadIdx.x;
     uint16_t *stk = s_stack + threadIdx.x;
-    uint32_t n = (L.flags & F_DENSE) ? 0u : L.seg_n;
-    {
-      const bool hd = (L.flags & F_HEAD) != 0;
-      const uint32_t *iidx = hd ? S.hidx : S.cidx;
-      const float *itin = hd ? S.htin : S.ctin, *itout = hd ? S.htout : S.ctout;
-      const uint8_t *ieo = hd ? S.heo : S.ceo;
-      const uint64_t row = hd ? L.pix : slot, stride = hd ? S.npix : S.nslots;
-      for (uint32_t k = 0; k < n; ++k) {
-        li[k * kSBlock] = iidx[k * stride + row];
-        ltin[k * kSBlock] = itin[k * stride + row];
-        ltout[k * kSBlock] = itout[k * stride + row];
-        leo[k * kSBlock] = ieo[k * stride + row];
-      }
-    }
+    uint32_t n = 0;
     // active set at the current t
     uint32_t act = 0, eptr = 0;
     float t_enter = AA_INF_F, t_exit = AA_INF_F;
@@ -690,8 +677,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
     };
     // A draw that lands before lim with no active particle is a step in
     // empty space: the query at it covers nothing, so only t and i move.
-    const float lim0 = fminf(L.tmax, L.seg_end);
-    float lim = lim0;
+    float lim0 = 0.0f, lim = 0.0f;
     auto leave = [&](float t) {
       t_exit = AA_INF_F;
       uint32_t m = act;
@@ -705,8 +691,28 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
           t_exit = fminf(t_exit, to);
       }
     };
-    enter(L.t);
-    lim = fminf(lim0, t_enter);
+    // the candidate list of the lane's ray into LDS, active set at L.t
+    auto setup = [&]() {
+      n = (L.flags & F_DENSE) ? 0u : L.seg_n;
+      const bool hd = (L.flags & F_HEAD) != 0;
+      const uint32_t *iidx = hd ? S.hidx : S.cidx;
+      const float *itin = hd ? S.htin : S.ctin, *itout = hd ? S.htout : S.ctout;
+      const uint8_t *ieo = hd ? S.heo : S.ceo;
+      const uint64_t row = hd ? L.pix : slot, stride = hd ? S.npix : S.nslots;
+      for (uint32_t k = 0; k < n; ++k) {
+        li[k * kSBlock] = iidx[k * stride + row];
+        ltin[k * kSBlock] = itin[k * stride + row];
+        ltout[k * kSBlock] = itout[k * stride + row];
+        leo[k * kSBlock] = ieo[k * stride + row];
+      }
+      act = 0;
+      eptr = 0;
+      t_exit = AA_INF_F;
+      lim0 = fminf(L.tmax, L.seg_end);
+      enter(L.t);
+      lim = fminf(lim0, t_enter);
+    };
+    setup();
 
     uint32_t steps = 0;
     for (;;) {
@@ -853,6 +859,12 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
       float vis = (L.flags & F_SHADOW) ? 1.0f - L.shw : 1.0f;
       finish_sample<STATS>(A, S, L, vis, c);
       outcome = next_ray<STATS>(A, S, L, c);
+      if (!DENSE && outcome == 1 && !(L.flags & F_DENSE)) {
+        // the pixel's next sample can step at once: go on with it here
+        setup();
+        outcome = 0;
+        continue;
+      }
       break;
     }
     if (outcome) save_lane(S, slot, L);
