@@ -47,6 +47,7 @@ constexpr uint32_t kCand = 16;
 constexpr int kGBlock = 128; // gather kernel block
 constexpr int kSBlock = 64;  // step kernel block (one wave)
 constexpr uint64_t kPassSamples = 64; // samples per colour-buffer pass
+constexpr uint32_t kDmLds = 64;       // density-map texels cached in LDS
 
 constexpr uint32_t F_SHADOW = 1u, F_PENDING = 2u, F_DENSE = 4u;
 // F_HEAD: the segment is the pixel's head list (shared by the primary rays
@@ -637,7 +638,20 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
   __shared__ float s_tout[NC * kSBlock];
   __shared__ uint8_t s_eo[NC * kSBlock];
   __shared__ uint16_t s_stack[NS * kSBlock];
-  load_levels(A);
+  __shared__ float s_dm[kDmLds];
+  if (DENSE) load_levels(A);
+  // density map red channel, read at every accepted-candidate step
+  const bool dm_lds = A.dm_n <= kDmLds;
+  if (dm_lds)
+    for (uint32_t j = threadIdx.x; j < A.dm_n; j += kSBlock) s_dm[j] = A.dmap[j].x;
+  __syncthreads();
+  auto density = [&](float u) {
+    if (!dm_lds) return tf_r(A.dmap, A.dm_n, u);
+    uint32_t j0, j1;
+    float f;
+    aa_tf_coords(u, A.dm_n, &j0, &j1, &f);
+    return aa_lerp_tex(s_dm[j0], s_dm[j1], f);
+  };
   const bool marching = A.blue != 0;
   const uint32_t n_a = *countA;
   uint32_t gid = blockIdx.x * kSBlock + threadIdx.x;
@@ -786,11 +800,12 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
             uint32_t k = (uint32_t)__ffs(m) - 1u;
             m &= m - 1u;
             uint32_t p = li[k * kSBlock];
+            // both halves of the payload in one round trip
             float4 qq = A.pr[p];
+            float4 kk = A.pc[p];
             if (STATS) c.particles++;
             float dd = aa_dist3(qq.x, qq.y, qq.z, x, y, z);
             if (dd < qq.w) {
-              float4 kk = A.pc[p];
               float wa = aa_rbf(dd, qq.w) * kk.x;
               if (STATS) c.covering++;
               pl.count += 1;
@@ -806,7 +821,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
         const bool shadow = (L.flags & F_SHADOW) != 0;
         if (marching) {
           if (pl.count > 0 && pl.dens > 0.0f) {
-            float D = tf_r(A.dmap, A.dm_n, pl.dens);
+            float D = density(pl.dens);
             L.LHS += D;
             if (L.LHS > L.RHS && !shadow) {
               float3 col = colour_of(A, pl);
@@ -821,7 +836,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
             }
           }
         } else if (pl.count > 0) {
-          float D = tf_r(A.dmap, A.dm_n, pl.dens);
+          float D = density(pl.dens);
           float xi2 = aa_lcg_next(&L.trng);
           if (xi2 < D) {
             if (shadow) {
