@@ -147,6 +147,19 @@ class Context:
         self.check(self.lib.aarbf_render_histogram(self.h, h, 1 if reset else 0), "aarbf_render_histogram")
         return [int(v) for v in h]
 
+    TIMING_KERNELS = ("gather", "step", "dense_step", "setup_fold")
+
+    def set_timing(self, enable: bool):
+        """Bracket the render rounds' kernels with HIP events (see read_timing)."""
+        self.check(self.lib.aarbf_set_timing(self.h, 1 if enable else 0), "aarbf_set_timing")
+
+    def read_timing(self, reset: bool = True) -> dict:
+        """{kernel: (total ms, launches)} since the last reset."""
+        ms = (C.c_double * 4)()
+        n = (C.c_uint64 * 4)()
+        self.check(self.lib.aarbf_read_timing(self.h, ms, n, 1 if reset else 0), "aarbf_read_timing")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.TIMING_KERNELS)}
+
     # -- pre-process (hostCode.cpp:183-250) ---------------------------------
     def spacetime_aabb(self, particles, aabb=None) -> np.ndarray:
         a = np.array(aabb if aabb is not None else [1e38, 1e38, 1e38, -1e38, -1e38, -1e38], dtype=np.float32)

@@ -90,6 +90,8 @@ SIGNATURES = {
     "aarbf_set_tuning": (C.c_int, [P, C.c_int, C.c_int]),
     "aarbf_render_stats_read": (C.c_int, [P, C.POINTER(RenderStats), C.c_int]),
     "aarbf_render_histogram": (C.c_int, [P, C.POINTER(u64), C.c_int]),
+    "aarbf_set_timing": (C.c_int, [P, C.c_int]),
+    "aarbf_read_timing": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(u64), C.c_int]),
     "aarbf_import_particles": (C.c_int, [C.c_char_p, C.POINTER(P)]),
     "aarbf_timesteps_count": (u32, [P]),
     "aarbf_timesteps_get": (C.c_int, [P, u32, C.POINTER(FP), C.POINTER(u64), C.POINTER(C.c_char_p)]),

@@ -127,6 +127,8 @@ int aarbf_context_destroy(aarbf_context *ctx) {
   if (ctx->baked) hipFree(ctx->baked);
   if (ctx->d_stats) hipFree(ctx->d_stats);
   if (ctx->d_counter) hipFree(ctx->d_counter);
+  for (hipEvent_t &e : ctx->tev)
+    if (e) hipEventDestroy(e);
   if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return AARBF_OK;
@@ -193,6 +195,28 @@ int aarbf_render_stats_read(aarbf_context *ctx, aarbf_render_stats *out, int res
   out->covering = v[6];
   out->exhausted = v[7];
   if (reset) AARBF_HIP(ctx, hipMemsetAsync(ctx->d_stats, 0, sizeof(v), ctx->stream));
+  return AARBF_OK;
+}
+
+int aarbf_set_timing(aarbf_context *ctx, int enable) {
+  if (!ctx) return AARBF_ERR_INVALID_ARGUMENT;
+  if (enable)
+    for (hipEvent_t &e : ctx->tev)
+      if (!e) AARBF_HIP(ctx, hipEventCreate(&e));
+  ctx->timing = enable != 0;
+  return AARBF_OK;
+}
+
+int aarbf_read_timing(aarbf_context *ctx, double *ms, uint64_t *launches, int reset) {
+  if (!ctx || !ms || !launches) return AARBF_ERR_INVALID_ARGUMENT;
+  for (int k = 0; k < 4; ++k) {
+    ms[k] = ctx->tms[k];
+    launches[k] = ctx->tlaunch[k];
+    if (reset) {
+      ctx->tms[k] = 0.0;
+      ctx->tlaunch[k] = 0;
+    }
+  }
   return AARBF_OK;
 }
 

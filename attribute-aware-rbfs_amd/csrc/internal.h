@@ -75,8 +75,13 @@ struct aarbf_context {
   uint32_t *d_counter = nullptr; // work counter of the persistent render kernel
   int render_mode = 0;           // 0 persistent segment-gather, 1 per-pixel point queries
   int cus = 0;                   // compute units of the device
-  int tune_step_waves = 0;       // occupancy target of the step kernel (0 = compiler's choice)
+  int tune_step_waves = 0;       // occupancy target of the step kernel (0 = default, 4 waves per SIMD)
   int tune_step_cap = 4096;      // tracker steps per ray per wavefront round
+  // kernel timing (HIP events around each kernel class of the render rounds)
+  bool timing = false;
+  hipEvent_t tev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double tms[4] = {0, 0, 0, 0};   // gather, step, dense step, other
+  uint64_t tlaunch[4] = {0, 0, 0, 0};
   // grow-only device scratch
   void *scratch = nullptr;
   size_t scratch_bytes = 0;

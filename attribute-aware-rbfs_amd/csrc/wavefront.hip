@@ -43,7 +43,10 @@
 namespace aarbf {
 namespace {
 
-constexpr uint32_t kCand = 16;
+// Candidates per ray segment.  12 keeps the step kernel's LDS at 10 KB per
+// wave, so with <= 128 VGPRs four waves fit on each SIMD (16 gave longer
+// segments but three waves, and was slower).
+constexpr uint32_t kCand = 12;
 constexpr int kGBlock = 128; // gather kernel block
 constexpr int kSBlock = 64;  // step kernel block (one wave)
 constexpr uint64_t kPassSamples = 64; // samples per colour-buffer pass
@@ -922,6 +925,8 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
   while (nA > 0) {
     uint32_t *lA = lists[cur], *lG = lists[cur + 1], *nA2 = lists[2 - cur], *nG2 = lists[3 - cur];
     uint32_t *cA = cnt + (cur ? 2 : 0), *cN = cnt + (cur ? 0 : 2);
+    const bool tm = ctx->timing;
+    if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[0], ctx->stream));
     if (nG > 0) {
       uint32_t blocks = (nG + kGBlock - 1) / kGBlock;
       if (st)
@@ -931,34 +936,56 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
     }
     wf_zero<<<1, 64, 0, ctx->stream>>>(cN, 2);
     wf_zero<<<1, 64, 0, ctx->stream>>>(cnt + 4, 1);
+    if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[1], ctx->stream));
     // fewer live rays -> more steps per round, so the tail costs few rounds
     const uint64_t base_cap = (uint64_t)ctx->tune_step_cap;
     uint32_t cap = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(base_cap, base_cap * nslots / (4ull * nA)));
-    uint32_t blocks = (nA + kSBlock - 1) / kSBlock;
+    const uint32_t blocks = (nA + kSBlock - 1) / kSBlock;
     if (st) {
       wf_step<true, false, 1><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
-      wf_step<true, true, 1><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, nullptr,
-                                                                  nullptr, cap);
     } else {
       switch (ctx->tune_step_waves) {
-      case 4:
-        wf_step<false, false, 4><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
-        break;
       case 5:
         wf_step<false, false, 5><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
         break;
       case 6:
         wf_step<false, false, 6><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
         break;
-      default:
-        wf_step<false, false, 1><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+      default: // 4 waves per SIMD
+        wf_step<false, false, 4><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
       }
-      wf_step<false, true, 1><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, nullptr,
-                                                                   nullptr, cap);
     }
     AARBF_CHECK_LAUNCH(ctx, "wf_step");
+    if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[2], ctx->stream));
+    // the dense queue (filled by the step kernel) sizes its own grid
+    uint32_t nD = 0;
+    AARBF_HIP(ctx, hipMemcpyAsync(&nD, cnt + 4, 4, hipMemcpyDeviceToHost, ctx->stream));
+    AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (nD > 0) {
+      const uint32_t dblocks = (nD + kSBlock - 1) / kSBlock;
+      if (st)
+        wf_step<true, true, 1><<<dblocks, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, nullptr,
+                                                                     nullptr, cap);
+      else
+        wf_step<false, true, 1><<<dblocks, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, nullptr,
+                                                                      nullptr, cap);
+      AARBF_CHECK_LAUNCH(ctx, "wf_step (dense)");
+    }
+    if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[3], ctx->stream));
     AARBF_HIP(ctx, hipMemcpyAsync(h, cN, 8, hipMemcpyDeviceToHost, ctx->stream));
     AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (tm) {
+      float e[3];
+      for (int k = 0; k < 3; ++k) AARBF_HIP(ctx, hipEventElapsedTime(&e[k], ctx->tev[k], ctx->tev[k + 1]));
+      if (nG > 0) {
+        ctx->tms[0] += e[0];
+        ctx->tlaunch[0] += 1;
+      }
+      ctx->tms[1] += e[1];
+      ctx->tms[2] += e[2];
+      ctx->tlaunch[1] += 1;
+      ctx->tlaunch[2] += nD > 0;
+    }
     nA = h[0];
     nG = h[1];
     cur = 2 - cur;
@@ -1026,6 +1053,7 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
   for (uint32_t s0 = 0; s0 < A.spp; s0 += (uint32_t)chunk) {
     S.s0 = s0;
     S.s1 = (uint32_t)std::min<uint64_t>(A.spp, s0 + chunk);
+    if (ctx->timing) AARBF_HIP(ctx, hipEventRecord(ctx->tev[0], ctx->stream));
     wf_reset_counts<<<1, 64, 0, ctx->stream>>>(cnt);
     wf_pass_reset<<<pix_blocks, 256, 0, ctx->stream>>>(A, S);
     if (st)
@@ -1033,10 +1061,26 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
     else
       wf_init<false><<<init_blocks, 256, 0, ctx->stream>>>(A, S, lists[0], lists[1], cnt);
     AARBF_CHECK_LAUNCH(ctx, "wf_init");
+    if (ctx->timing) {
+      AARBF_HIP(ctx, hipEventRecord(ctx->tev[1], ctx->stream));
+      AARBF_HIP(ctx, hipEventSynchronize(ctx->tev[1]));
+      float e;
+      AARBF_HIP(ctx, hipEventElapsedTime(&e, ctx->tev[0], ctx->tev[1]));
+      ctx->tms[3] += e;
+      ctx->tlaunch[3] += 1;
+    }
     int rc = run_rounds(ctx, A, S, lists, listD, cnt);
     if (rc != AARBF_OK) return rc;
+    if (ctx->timing) AARBF_HIP(ctx, hipEventRecord(ctx->tev[0], ctx->stream));
     wf_fold<<<pix_blocks, 256, 0, ctx->stream>>>(A, S);
     AARBF_CHECK_LAUNCH(ctx, "wf_fold");
+    if (ctx->timing) {
+      AARBF_HIP(ctx, hipEventRecord(ctx->tev[1], ctx->stream));
+      AARBF_HIP(ctx, hipEventSynchronize(ctx->tev[1]));
+      float e;
+      AARBF_HIP(ctx, hipEventElapsedTime(&e, ctx->tev[0], ctx->tev[1]));
+      ctx->tms[3] += e;
+    }
   }
   return AARBF_OK;
 }

@@ -49,7 +49,12 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "render_traffic.json"),
                     help="HBM bytes per render launch measured with rocprofv3 --pmc")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one plain device-to-device copy of BYTES (the reference figure that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        return copy_calibration(a.copy_calibration)
 
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); the render path shards "
@@ -128,9 +133,15 @@ def main() -> int:
             log(f"{name} kernel: {hist[o]} wave iterations, lane utilisation {hist[o + 1] / (64 * hist[o]):.3f}, "
                 f"iterations testing candidates {hist[o + 2] / hist[o]:.3f}")
     samples = max(1, st["samples"])
-    # bytes per launch: node/leaf boxes 32 B, particle position+radius 16 B,
-    # colour payload 16 B per covering particle, accum read+write and image write 48 B per pixel
-    bytes_launch = 32 * st["nodes"] + 32 * st["leaves"] + 16 * st["particles"] + 16 * st["covering"] + 48 * W * H
+    # Algorithmic bytes of one render call (per sample: the BVH nodes the
+    # segment gathers test, at the quantised group's 80 B per 8 children;
+    # 16 B position + radius per particle tested, by a gather or a step;
+    # 16 B colour payload per covering particle; 32 B for the sample's
+    # colour write and read-back into the running mean), plus the accum
+    # read + write and the image write, 48 B per pixel.
+    node_bytes = 80 / 8
+    bytes_launch = (node_bytes * st["nodes"] + 32 * st["leaves"] + 16 * st["particles"] + 16 * st["covering"]
+                    + 32 * samples + 48 * W * H)
     log(f"stats per sample: steps {st['steps'] / samples:.0f} queries {st['queries'] / samples:.0f} "
         f"nodes {st['nodes'] / samples:.0f} particles {st['particles'] / samples:.0f} "
         f"covering {st['covering'] / samples:.1f} exhausted {st['exhausted'] / samples:.4f}")
@@ -141,6 +152,8 @@ def main() -> int:
 
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
+    ctx.set_timing(a.mode == 0)
+    ctx.read_timing(reset=True)
     torch.cuda.synchronize()
     w0 = time.time()
     t_start.record(stream)
@@ -154,6 +167,13 @@ def main() -> int:
     wall = time.time() - w0
     gpu_ms = t_start.elapsed_time(t_end)
     kern_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
+    # per-kernel times of the render rounds, from the library's HIP events
+    # on the same stream (mode 0)
+    ktime = ctx.read_timing(reset=True)
+    ctx.set_timing(False)
+    kernels = {k: {"ms_per_frame": round(ms / a.steps, 3), "launches_per_frame": round(n / a.steps, 2),
+                   "ms_per_launch": round(ms / n, 4) if n else None}
+               for k, (ms, n) in ktime.items()} if a.mode == 0 else None
     total_samples = W * H * S * a.steps
     msps = total_samples / (gpu_ms * 1e-3) / 1e6
     achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
@@ -193,9 +213,11 @@ def main() -> int:
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "render_kernel", "kernel_ms": round(kern_ms, 3),
+            "kernel": "aarbf_render_rbf_spp (wf_gather + wf_step + dense wf_step rounds, wf_fold)",
+            "kernel_ms": round(kern_ms, 3),
             "algorithmic_bytes_per_launch": int(bytes_launch),
             "algorithmic_bytes_per_sample": round(bytes_launch / samples, 1),
+            "kernels": kernels,
         },
         "cpu_baseline": cpu,
         "wall_s": round(wall, 3),
@@ -205,10 +227,21 @@ def main() -> int:
     return 0
 
 
+def copy_calibration(nbytes: int) -> int:
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copied {src.numel() * 4} bytes")
+    return 0
+
+
 def cpu_baseline(sc, wl, cam, threads):
     """The oracle (scalar C restatement of the reference shaders, std::thread
-    -style pthreads over pixels) on frame 0 at 1 spp on a fixed 1/64 pixel
-    subsample (every 8th pixel in x and y)."""
+    -style pthreads over pixels) on frame 0 at 2 spp on a fixed 1/16 pixel
+    subsample (every 4th pixel in x and y)."""
     import oracle
     p = sc.particles.cpu().numpy()
     r = sc.radii.cpu().numpy() if sc.radii is not None else None
@@ -219,14 +252,16 @@ def cpu_baseline(sc, wl, cam, threads):
     t0 = time.time()
     prep = R.prepared()
     setup = time.time() - t0
-    ys, xs = np.mgrid[0:H:8, 0:W:8]
+    ys, xs = np.mgrid[0:H:4, 0:W:4]
     pix = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.int32)
+    spp = 2
     t0 = time.time()
-    prep.render(pix, spp=1, threads=threads)
+    prep.render(pix, spp=spp, threads=threads)
     dt = time.time() - t0
-    log(f"cpu baseline: {len(pix)} samples in {dt:.2f} s on {threads} threads (scene set-up {setup:.1f} s)")
-    return {"value": round(len(pix) / dt / 1e6, 6), "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"frame 0, 1 spp, every 8th pixel in x and y ({len(pix)} samples of the {W}x{H} frame); "
+    n = len(pix) * spp
+    log(f"cpu baseline: {n} samples in {dt:.2f} s on {threads} threads (scene set-up {setup:.1f} s)")
+    return {"value": round(n / dt / 1e6, 6), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"frame 0, {spp} spp, every 4th pixel in x and y ({n} samples of the {W}x{H} frame); "
                       f"oracle/oracle.c -O2 -ffp-contract=off, {threads} threads, grid set-up {setup:.1f} s untimed"}
 
 

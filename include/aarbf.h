@@ -155,7 +155,7 @@ const char *aarbf_status_string(int status);
 int aarbf_set_render_mode(aarbf_context *ctx, int mode);
 /* Schedule parameters of the default renderer (none changes the image):
  * AARBF_TUNE_STEP_WAVES: occupancy target of the step kernel in waves per
- * SIMD (0 = compiler's choice, or 4..6); AARBF_TUNE_STEP_CAP: tracker steps
+ * SIMD (0 = default 4, or 4..6); AARBF_TUNE_STEP_CAP: tracker steps
  * per ray per wavefront round (16..65536, default 4096).  No counterpart in
  * the reference. */
 #define AARBF_TUNE_STEP_WAVES 1
@@ -172,6 +172,13 @@ int aarbf_render_stats_read(aarbf_context *ctx, aarbf_render_stats *out, int res
  * out[21..23] the same for the dense-segment kernel.  No counterpart in
  * the reference. */
 int aarbf_render_histogram(aarbf_context *ctx, uint64_t *out, int reset);
+/* Kernel timing of the default renderer: with timing enabled, HIP events on
+ * the context's stream bracket each kernel of the render rounds; ms[k] and
+ * launches[k] accumulate the time and launch count of the segment gathers
+ * (k = 0), the step kernel (1), the dense-segment step kernel (2) and the
+ * set-up / fold kernels (3).  No counterpart in the reference. */
+int aarbf_set_timing(aarbf_context *ctx, int enable);
+int aarbf_read_timing(aarbf_context *ctx, double *ms, uint64_t *launches, int reset);
 /* Writes sizes and offsets of the structs above into out[0..n) and returns
  * how many there are; bindings in other languages use it to check their
  * struct mirrors.  No counterpart in the reference. */
