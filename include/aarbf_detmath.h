@@ -185,6 +185,54 @@ AA_FN float aa_rbf(float d, float r) {
   return aa_expf(-0.5f * (q * q));
 }
 
+/* over (sharedCode.h:142-149): a composited over b. */
+AA_FN void aa_over(const float a[4], const float b[4], float out[4]) {
+  float ra = a[3] + b[3] * (1.0f - a[3]);
+  if (ra == 0.0f) {
+    out[0] = a[0];
+    out[1] = a[1];
+    out[2] = a[2];
+    out[3] = a[3];
+    return;
+  }
+  float r0 = (a[0] * a[3] + (b[0] * b[3]) * (1.0f - a[3])) / ra;
+  float r1 = (a[1] * a[3] + (b[1] * b[3]) * (1.0f - a[3])) / ra;
+  float r2 = (a[2] * a[3] + (b[2] * b[3]) * (1.0f - a[3])) / ra;
+  out[0] = r0;
+  out[1] = r1;
+  out[2] = r2;
+  out[3] = ra;
+}
+
+/* Whether the segment [tmin, tmax] of the ray o + t*d meets the box
+ * [l, h] (slab test with the reciprocal direction i = 1/d; an empty box,
+ * l.x > h.x, is never met).  This stands for the box test of a ray-tracing
+ * traversal, which reports the primitives of every box the ray segment
+ * meets. */
+AA_FN int aa_seg_box(float ox, float oy, float oz, float ix, float iy, float iz, float tmin, float tmax,
+                     float lx, float ly, float lz, float hx, float hy, float hz) {
+  if (!(lx <= hx)) return 0;
+  float x1 = (lx - ox) * ix, x2 = (hx - ox) * ix;
+  float y1 = (ly - oy) * iy, y2 = (hy - oy) * iy;
+  float z1 = (lz - oz) * iz, z2 = (hz - oz) * iz;
+  float tn = AA_FMAXF(AA_FMAXF(AA_FMINF(x1, x2), AA_FMINF(y1, y2)), AA_FMINF(z1, z2));
+  float tf = AA_FMINF(AA_FMINF(AA_FMAXF(x1, x2), AA_FMAXF(y1, y2)), AA_FMAXF(z1, z2));
+  tn = AA_FMAXF(tn, tmin);
+  tf = AA_FMINF(tf, tmax);
+  return tn <= tf;
+}
+
+/* ParticleSplatIntersection's test (deviceCodeSplat.hlsl:164-181) for a
+ * particle at c with radius r: the sample sits at t = |c - o| along the
+ * ray, and hits when it lies within r of c and tmin <= t < tmax. */
+AA_FN int aa_splat_hit(float cx, float cy, float cz, float r, float ox, float oy, float oz, float dx, float dy,
+                       float dz, float tmin, float tmax, float *t_out) {
+  float t = aa_dist3(cx, cy, cz, ox, oy, oz);
+  float fx = cx - (ox + t * dx), fy = cy - (oy + t * dy), fz = cz - (oz + t * dz);
+  *t_out = t;
+  return ((fx * fx + fy * fy) + fz * fz) < r * r && tmin <= t && t < tmax;
+}
+
 /* One LCG step (rng.h:52-57) and its [0,1] float (rng.h:59-61): the uint32
  * is converted with round-to-nearest-even, so states >= 0xFFFFFF80 give
  * exactly 1.0f. */

@@ -80,6 +80,9 @@ def lib() -> C.CDLL:
         L.orc_point_queries.argtypes = [C.POINTER(OrcParams), f32p, C.c_uint64, f32p]
         L.orc_light_direction.argtypes = [C.c_float, C.c_float, f32p]
         L.orc_taa_resolve.argtypes = [f32p, f32p, f32p, C.c_int, C.c_int, C.c_int]
+        L.orc_splat.restype = C.c_int
+        L.orc_splat.argtypes = [C.POINTER(OrcParams), f32p, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32),
+                                C.c_uint64, f32p]
         _lib = L
     return _lib
 
@@ -289,6 +292,23 @@ class Render:
                     self.h = None
 
         return _Prepared()
+
+    def splat(self, leaf_aabbs: np.ndarray, ppl: int = 1, pixels=None) -> np.ndarray:
+        """ParticleSplatRayGen over all pixels (or the given (x, y) pixels);
+        leaf_aabbs = the GenRBFBounds output the BVH was built from."""
+        W, H = self.p.width, self.p.height
+        image = np.zeros((H, W, 4), np.float32)
+        a = np.ascontiguousarray(leaf_aabbs, np.float32).reshape(-1)
+        nl = a.size // 6
+        if pixels is not None:
+            px = np.ascontiguousarray(np.asarray(pixels, dtype=np.int32).reshape(-1, 2))
+            st = lib().orc_splat(C.byref(self.p), _f(a), nl, ppl, px.ctypes.data_as(C.POINTER(C.c_int32)),
+                                 px.shape[0], _f(image))
+        else:
+            st = lib().orc_splat(C.byref(self.p), _f(a), nl, ppl, None, 0, _f(image))
+        if st != 0:
+            raise RuntimeError("orc_splat failed")
+        return image
 
     def point_queries(self, pts: np.ndarray) -> np.ndarray:
         q = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)

@@ -864,6 +864,165 @@ ORC_EXPORT int orc_point_queries(const orc_params *p, const float *pts, uint64_t
 }
 
 /* ======================================================================
+ * Splatting (deviceCodeSplat.hlsl:29-194)
+ *
+ * ParticleSplatRayGen marches slabs of 15/8 = 1 unit (integer division)
+ * from t = 0; for every slab that reaches past tenter it traces the ray
+ * segment [max(tenter, tslab), min(texit, tslab + unit)], keeps the first
+ * 15 hits the any-hit program sees, sorts them by t and composites them
+ * front to back.  The traversal reports the particles of every leaf whose
+ * box the segment meets (aa_seg_box on the leaf boxes widened by 2^-20
+ * relative, as the BVH stores them) that pass the intersection test; the
+ * hit order is ascending particle index (the reference's is the driver's
+ * traversal order, which only matters when a slab has more than 15 hits).
+ * The sphere radius is rbfRadius (the radius map is not used, ':166') or
+ * the per-particle radius when radii are given.
+ * ====================================================================== */
+
+typedef struct {
+  uint32_t p;
+  float t;
+} splat_hit;
+
+static int cmp_hit_t(const void *a, const void *b) {
+  const splat_hit *x = (const splat_hit *)a, *y = (const splat_hit *)b;
+  if (x->t < y->t) return -1;
+  if (x->t > y->t) return 1;
+  return (x->p > y->p) - (x->p < y->p);
+}
+
+static int cmp_hit_p(const void *a, const void *b) {
+  const splat_hit *x = (const splat_hit *)a, *y = (const splat_hit *)b;
+  return (x->p > y->p) - (x->p < y->p);
+}
+
+static void splat_pixel(const orc_params *p, const float *leaves /* 8 floats per widened leaf */, uint32_t nleaf,
+                        uint32_t ppl, int px, int py, splat_hit *cand, splat_hit *slab, float out[4]) {
+  const int W = p->width, H = p->height;
+  float sx = ((float)px + 0.5f) / (float)W, sy = ((float)py + 0.5f) / (float)H;
+  float lb[3], rt[3], dir[3], inv[3];
+  for (int d = 0; d < 3; ++d) {
+    rt[d] = p->aabb_max[d] + p->rbfRadius; /* +- rbfRadius, not 2x (':66-67') */
+    lb[d] = p->aabb_min[d] - p->rbfRadius;
+    dir[d] = (p->cam_d00[d] + sx * p->cam_du[d]) + sy * p->cam_dv[d];
+  }
+  float len = sqrtf((dir[0] * dir[0] + dir[1] * dir[1]) + dir[2] * dir[2]);
+  for (int d = 0; d < 3; ++d) {
+    dir[d] = dir[d] / len;
+    inv[d] = 1.0f / dir[d];
+  }
+  const float *o = p->cam_pos;
+  float tenter, texit;
+  aabb_intersection(o, dir, 0.0f, 10000.0f, lb, rt, &tenter, &texit);
+  float res[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (tenter < texit) {
+    /* every particle any slab can report: its leaf box meets [tenter, texit]
+     * and its sample lies in the sphere with tenter <= t < texit */
+    uint32_t nc = 0;
+    for (uint32_t l = 0; l < nleaf; ++l) {
+      const float *b = leaves + 8 * (uint64_t)l;
+      if (!aa_seg_box(o[0], o[1], o[2], inv[0], inv[1], inv[2], tenter, texit, b[0], b[1], b[2], b[4], b[5], b[6]))
+        continue;
+      for (uint32_t q = l * ppl; q < (l + 1) * ppl && q < p->numParticles; ++q) {
+        const float *c = p->xyzw + 4 * (uint64_t)q;
+        float r = p->radii ? p->radii[q] : p->rbfRadius, t;
+        if (aa_splat_hit(c[0], c[1], c[2], r, o[0], o[1], o[2], dir[0], dir[1], dir[2], tenter, texit, &t)) {
+          cand[nc].p = q;
+          cand[nc].t = t;
+          ++nc;
+        }
+      }
+    }
+    qsort(cand, nc, sizeof(splat_hit), cmp_hit_t);
+    const float pps = (float)(15 / 8);
+    const float sp = pps * p->unit;
+    const float fade = aa_expf(-p->unit * pps);
+    float tslab = 0.0f;
+    uint32_t lo = 0;
+    while (tslab < texit) {
+      if (res[3] > 0.99f) break;
+      if (!(tslab + sp > tslab)) break; /* unit below half an ulp of t: stop */
+      float tmin = fmaxf(tenter, tslab), tmax = fminf(texit, tslab + sp);
+      if (tmax > tenter) {
+        while (lo < nc && cand[lo].t < tmin) ++lo;
+        uint32_t ns = 0;
+        for (uint32_t k = lo; k < nc && cand[k].t < tmax; ++k) {
+          const float *b = leaves + 8 * (uint64_t)(cand[k].p / ppl);
+          if (aa_seg_box(o[0], o[1], o[2], inv[0], inv[1], inv[2], tmin, tmax, b[0], b[1], b[2], b[4], b[5], b[6]))
+            slab[ns++] = cand[k];
+        }
+        /* the payload keeps the first 15 hits in traversal order, then
+         * SplatPayload::sort orders them by t (stable bubble sort) */
+        qsort(slab, ns, sizeof(splat_hit), cmp_hit_p);
+        if (ns > 15) ns = 15;
+        for (uint32_t a = 1; a < ns; ++a) {
+          splat_hit h = slab[a];
+          uint32_t b = a;
+          while (b > 0 && slab[b - 1].t > h.t) {
+            slab[b] = slab[b - 1];
+            --b;
+          }
+          slab[b] = h;
+        }
+        for (uint32_t k = 0; k < ns; ++k) {
+          const float *c = p->xyzw + 4 * (uint64_t)slab[k].p;
+          float r = p->radii ? p->radii[slab[k].p] : p->rbfRadius;
+          float t = slab[k].t;
+          float x = o[0] + dir[0] * t, y = o[1] + dir[1] * t, z = o[2] + dir[2] * t;
+          float drbf = aa_rbf(aa_dist3(x, y, z, c[0], c[1], c[2]), r);
+          float col[4];
+          if (p->visualizeAttributes) {
+            orc_tf_sample(p->colormap, p->cm_n, c[3], col);
+            col[3] = col[3] * tf1(p->densitymap, p->dm_n, drbf, 0);
+          } else {
+            float cm[4];
+            orc_tf_sample(p->colormap, p->cm_n, drbf, cm);
+            col[0] = cm[0];
+            col[1] = cm[1];
+            col[2] = cm[2];
+            col[3] = tf1(p->densitymap, p->dm_n, drbf, 0);
+          }
+          col[3] = col[3] * fade;
+          float nr[4];
+          aa_over(res, col, nr);
+          memcpy(res, nr, 16);
+        }
+      }
+      tslab = tslab + sp;
+    }
+  }
+  const float black[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+  aa_over(res, black, out);
+}
+
+/* Renders the given pixels (all when pixels == NULL) into image (W*H float4).
+ * leaf_aabbs: the GenRBFBounds output (6 floats per leaf). */
+ORC_EXPORT int orc_splat(const orc_params *p, const float *leaf_aabbs, uint32_t num_leaves, uint32_t ppl,
+                         const int32_t *pixels, uint64_t npix, float *image) {
+  if (ppl == 0) return -1;
+  float *leaves = (float *)malloc(sizeof(float) * 8 * ((uint64_t)num_leaves + 1));
+  splat_hit *cand = (splat_hit *)malloc(sizeof(splat_hit) * ((uint64_t)p->numParticles + 1));
+  splat_hit *slab = (splat_hit *)malloc(sizeof(splat_hit) * ((uint64_t)p->numParticles + 1));
+  if (!leaves || !cand || !slab) {
+    free(leaves);
+    free(cand);
+    free(slab);
+    return -1;
+  }
+  orc_bvh_build_level(leaf_aabbs, 1, num_leaves, 2, leaves);
+  const uint64_t n = pixels ? npix : (uint64_t)p->width * p->height;
+  for (uint64_t i = 0; i < n; ++i) {
+    int px = pixels ? pixels[2 * i] : (int)(i % (uint64_t)p->width);
+    int py = pixels ? pixels[2 * i + 1] : (int)(i / (uint64_t)p->width);
+    splat_pixel(p, leaves, num_leaves, ppl, px, py, cand, slab, image + 4 * ((uint64_t)px + (uint64_t)p->width * py));
+  }
+  free(leaves);
+  free(cand);
+  free(slab);
+  return 0;
+}
+
+/* ======================================================================
  * TAA resolve (deviceCodeCommon.hlsl:27-43,67-115); border texels are
  * clamped to the edge.  Writes taa[] and copies it to taa_prev[]
  * (hostCode.cpp:829-830).
