@@ -1023,6 +1023,284 @@ ORC_EXPORT int orc_splat(const orc_params *p, const float *leaf_aabbs, uint32_t 
 }
 
 /* ======================================================================
+ * FP64 quadrature of what delta tracking converges to
+ *
+ * ParticleRBFRayGen with delta tracking (majorant 1 per unit, accept with
+ * probability D = densitymap(rho) wherever some particle covers the point)
+ * samples a collision at t with density sigma(t) T(t), where sigma = D/unit
+ * (0 where nothing covers) and T(t) = exp(-int_0^t sigma).  A collision
+ * shades with the colour of the point times (amb + (1 - amb) T_L), T_L the
+ * transmittance of the shadow ray to the light; a ray that leaves the box
+ * is black.  The expected pixel is therefore
+ *     E = int_0^texit sigma(t) T(t) albedo(t) (amb + (1 - amb) T_L(t)) dt,
+ * which this section evaluates in double precision: the rays are split at
+ * every sphere entry and exit (the covering set, and so the integrand, is
+ * smooth in between), each piece into panels no longer than r_min / 8, and
+ * every panel integrated with 8-point Gauss-Legendre (the optical depth to
+ * each node by a nested 8-point rule).  The step budget (MAX_DEPTH) is
+ * ignored: its exhaustion probability is reported by the renderer.
+ * ====================================================================== */
+
+static const double gl8_x[8] = {-0.9602898564975363, -0.7966664774136267, -0.5255324099163290,
+                                -0.1834346424956498, 0.1834346424956498,  0.5255324099163290,
+                                0.7966664774136267,  0.9602898564975363};
+static const double gl8_w[8] = {0.1012285362903763, 0.2223810344533745, 0.3137066458778873, 0.3626837833783620,
+                                0.3626837833783620, 0.3137066458778873, 0.2223810344533745, 0.1012285362903763};
+
+static double tf1d(const float *table, uint32_t n, double u, int c) {
+  double x = u * (double)n - 0.5;
+  if (!(x > -1.0)) x = -1.0;
+  if (x > (double)n) x = (double)n;
+  double fl = floor(x);
+  int i0 = (int)fl;
+  double f = x - fl;
+  int a = i0 < 0 ? 0 : (i0 > (int)n - 1 ? (int)n - 1 : i0);
+  int b = i0 + 1 < 0 ? 0 : (i0 + 1 > (int)n - 1 ? (int)n - 1 : i0 + 1);
+  double ta = table[4 * a + c], tb = table[4 * b + c];
+  return ta + f * (tb - ta);
+}
+
+typedef struct {
+  const orc_scene *s;
+  double o[3], d[3];
+  uint32_t *act; /* particles covering some point of the current piece */
+  uint32_t nact;
+} q_ray;
+
+/* density sum, colour sum and count at o + t d over the active particles */
+static void q_payload(const q_ray *r, double t, double *rho, double col[3], uint32_t *count) {
+  const orc_scene *s = r->s;
+  const orc_params *p = s->p;
+  double x[3] = {r->o[0] + t * r->d[0], r->o[1] + t * r->d[1], r->o[2] + t * r->d[2]};
+  *rho = 0.0;
+  col[0] = col[1] = col[2] = 0.0;
+  *count = 0;
+  for (uint32_t k = 0; k < r->nact; ++k) {
+    const baked_particle *b = s->bp + r->act[k];
+    double dx = b->x - x[0], dy = b->y - x[1], dz = b->z - x[2];
+    double dd = sqrt(dx * dx + dy * dy + dz * dz);
+    if (!(dd < b->r)) continue;
+    double q = dd * 3.0 / b->r;
+    double wa = exp(-0.5 * q * q) * b->a3;
+    *count += 1;
+    *rho += wa;
+    for (int j = 0; j < 3; ++j) col[j] += b->c22[j] * wa;
+    if (!p->visualizeAttributes && *rho > 1.0) *rho = 1.0;
+  }
+}
+
+static double q_sigma(const q_ray *r, double t) {
+  const orc_params *p = r->s->p;
+  double rho, col[3];
+  uint32_t count;
+  q_payload(r, t, &rho, col, &count);
+  if (count == 0) return 0.0;
+  double D = tf1d(p->densitymap, p->dm_n, rho, 0);
+  D = D < 0.0 ? 0.0 : (D > 1.0 ? 1.0 : D);
+  return D / (double)p->unit;
+}
+
+typedef struct {
+  double t0, t1;
+  uint32_t p;
+} q_span;
+
+static int cmp_double(const void *a, const void *b) {
+  double x = *(const double *)a, y = *(const double *)b;
+  return (x > y) - (x < y);
+}
+
+/* sphere spans of all particles along the ray over [0, tmax]; returns count */
+static uint32_t q_spans(const orc_scene *s, const double o[3], const double d[3], double tmax, q_span *sp) {
+  const orc_params *p = s->p;
+  uint32_t n = 0;
+  for (uint32_t i = 0; i < p->numParticles; ++i) {
+    const baked_particle *b = s->bp + i;
+    if (!(b->r > 0.0f)) continue;
+    double v[3] = {b->x - o[0], b->y - o[1], b->z - o[2]};
+    double tc = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
+    double h2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] - tc * tc;
+    double w2 = (double)b->r * b->r - h2;
+    if (!(w2 > 0.0)) continue;
+    double w = sqrt(w2);
+    double a = tc - w, c = tc + w;
+    if (c < 0.0 || a > tmax) continue;
+    sp[n].t0 = a < 0.0 ? 0.0 : a;
+    sp[n].t1 = c > tmax ? tmax : c;
+    sp[n].p = i;
+    ++n;
+  }
+  return n;
+}
+
+/* visits the pieces of [0, tmax] between sphere boundaries that have a
+ * non-empty covering set, calling f(piece r, t0, t1, panel length) */
+typedef void (*q_piece_fn)(q_ray *r, double a, double b, double h, void *user);
+
+static void q_pieces(q_ray *r, double tmax, q_span *sp, double *cuts, uint32_t *act, q_piece_fn f, void *user) {
+  uint32_t ns = q_spans(r->s, r->o, r->d, tmax, sp);
+  uint32_t nc = 0;
+  for (uint32_t k = 0; k < ns; ++k) {
+    cuts[nc++] = sp[k].t0;
+    cuts[nc++] = sp[k].t1;
+  }
+  qsort(cuts, nc, sizeof(double), cmp_double);
+  r->act = act;
+  for (uint32_t c = 0; c + 1 < nc; ++c) {
+    double a = cuts[c], b = cuts[c + 1];
+    if (!(b > a)) continue;
+    double m = 0.5 * (a + b), rmin = 1e300;
+    r->nact = 0;
+    for (uint32_t k = 0; k < ns; ++k)
+      if (sp[k].t0 <= m && m <= sp[k].t1) {
+        act[r->nact++] = sp[k].p;
+        double rr = r->s->bp[sp[k].p].r;
+        if (rr < rmin) rmin = rr;
+      }
+    if (r->nact == 0) continue;
+    f(r, a, b, rmin / 8.0, user);
+  }
+}
+
+typedef struct {
+  double tau;
+} q_depth;
+
+static void q_depth_piece(q_ray *r, double a, double b, double h, void *user) {
+  q_depth *u = (q_depth *)user;
+  int m = (int)ceil((b - a) / h);
+  if (m < 1) m = 1;
+  double step = (b - a) / m;
+  for (int k = 0; k < m; ++k) {
+    double lo = a + k * step, half = 0.5 * step, mid = lo + half;
+    for (int j = 0; j < 8; ++j) u->tau += gl8_w[j] * half * q_sigma(r, mid + half * gl8_x[j]);
+  }
+}
+
+typedef struct {
+  q_span *sp;
+  double *cuts;
+  uint32_t *act;
+} q_scratch;
+
+/* optical depth from x along dir to where it leaves the clip box */
+static double q_shadow_depth(const orc_scene *s, const double x[3], q_scratch *w) {
+  const orc_params *p = s->p;
+  float so[3] = {(float)x[0], (float)x[1], (float)x[2]}, lb[3], rt[3], te, tx;
+  for (int d = 0; d < 3; ++d) {
+    rt[d] = p->aabb_max[d] + 2.0f * p->rbfRadius;
+    lb[d] = p->aabb_min[d] - 2.0f * p->rbfRadius;
+  }
+  aabb_intersection(so, s->light, 0.0f, 10000.0f, lb, rt, &te, &tx);
+  if (!(tx > 0.0f)) return 0.0;
+  q_ray r;
+  r.s = s;
+  for (int d = 0; d < 3; ++d) {
+    r.o[d] = x[d];
+    r.d[d] = s->light[d];
+  }
+  q_depth u = {0.0};
+  q_pieces(&r, (double)tx, w->sp, w->cuts, w->act, q_depth_piece, &u);
+  return u.tau;
+}
+
+typedef struct {
+  double T;      /* transmittance at the start of the next piece */
+  double e[3];   /* expected colour */
+  q_scratch *shadow;
+} q_primary;
+
+static void q_primary_piece(q_ray *r, double a, double b, double h, void *user) {
+  q_primary *u = (q_primary *)user;
+  const orc_scene *s = r->s;
+  const orc_params *p = s->p;
+  const double amb = p->light_ambient;
+  int m = (int)ceil((b - a) / h);
+  if (m < 1) m = 1;
+  double step = (b - a) / m;
+  for (int k = 0; k < m; ++k) {
+    double lo = a + k * step, half = 0.5 * step, mid = lo + half;
+    double panel_tau = 0.0;
+    for (int j = 0; j < 8; ++j) {
+      double t = mid + half * gl8_x[j];
+      /* optical depth from the panel start to the node */
+      double hh = 0.5 * (t - lo), mm = lo + hh, tau = 0.0;
+      for (int i = 0; i < 8; ++i) tau += gl8_w[i] * hh * q_sigma(r, mm + hh * gl8_x[i]);
+      double rho, col[3];
+      uint32_t count;
+      q_payload(r, t, &rho, col, &count);
+      if (count == 0) continue;
+      double D = tf1d(p->densitymap, p->dm_n, rho, 0);
+      D = D < 0.0 ? 0.0 : (D > 1.0 ? 1.0 : D);
+      double sig = D / (double)p->unit;
+      if (sig == 0.0) continue;
+      double alb[3];
+      for (int c = 0; c < 3; ++c)
+        alb[c] = p->visualizeAttributes ? pow(col[c] / rho, 1.0 / 2.2) : tf1d(p->colormap, p->cm_n, rho, c);
+      double shade = amb;
+      if (amb != 1.0) {
+        double x[3] = {r->o[0] + t * r->d[0], r->o[1] + t * r->d[1], r->o[2] + t * r->d[2]};
+        shade += (1.0 - amb) * exp(-q_shadow_depth(s, x, u->shadow));
+      }
+      double wgt = gl8_w[j] * half * sig * u->T * exp(-tau);
+      for (int c = 0; c < 3; ++c) u->e[c] += wgt * alb[c] * shade;
+    }
+    for (int j = 0; j < 8; ++j) panel_tau += gl8_w[j] * half * q_sigma(r, mid + half * gl8_x[j]);
+    u->T *= exp(-panel_tau);
+  }
+}
+
+/* Expected RGB of the given pixels (all when pixels == NULL) for delta
+ * tracking (enableBlueNoise = 0); out = 3 doubles per pixel. */
+ORC_EXPORT int orc_expected(const orc_params *p, const int32_t *pixels, uint64_t npix, double *out) {
+  if (p->enableBlueNoise) return -1;
+  orc_scene s;
+  if (scene_init(&s, p) != 0) return -1;
+  uint64_t np = (uint64_t)p->numParticles + 1;
+  q_span *sp = (q_span *)malloc(sizeof(q_span) * np), *sp2 = (q_span *)malloc(sizeof(q_span) * np);
+  double *cuts = (double *)malloc(sizeof(double) * 2 * np), *cuts2 = (double *)malloc(sizeof(double) * 2 * np);
+  uint32_t *act = (uint32_t *)malloc(sizeof(uint32_t) * np), *act2 = (uint32_t *)malloc(sizeof(uint32_t) * np);
+  int rc = (sp && sp2 && cuts && cuts2 && act && act2) ? 0 : -1;
+  const uint64_t n = pixels ? npix : (uint64_t)p->width * p->height;
+  for (uint64_t i = 0; rc == 0 && i < n; ++i) {
+    int px = pixels ? pixels[2 * i] : (int)(i % (uint64_t)p->width);
+    int py = pixels ? pixels[2 * i + 1] : (int)(i / (uint64_t)p->width);
+    float sx = ((float)px + 0.5f) / (float)p->width, sy = ((float)py + 0.5f) / (float)p->height;
+    float rt[3], lb[3], dir[3];
+    for (int d = 0; d < 3; ++d) {
+      rt[d] = p->aabb_max[d] + 2.0f * p->rbfRadius;
+      lb[d] = p->aabb_min[d] - 2.0f * p->rbfRadius;
+      dir[d] = (p->cam_d00[d] + sx * p->cam_du[d]) + sy * p->cam_dv[d];
+    }
+    float len = sqrtf((dir[0] * dir[0] + dir[1] * dir[1]) + dir[2] * dir[2]);
+    for (int d = 0; d < 3; ++d) dir[d] = dir[d] / len;
+    float tenter, texit;
+    aabb_intersection(p->cam_pos, dir, 0.0f, 10000.0f, lb, rt, &tenter, &texit);
+    double *e = out + 3 * i;
+    e[0] = e[1] = e[2] = 0.0;
+    if (!(tenter < texit)) continue;
+    q_ray r;
+    r.s = &s;
+    for (int d = 0; d < 3; ++d) {
+      r.o[d] = p->cam_pos[d];
+      r.d[d] = dir[d];
+    }
+    q_scratch sh = {sp2, cuts2, act2};
+    q_primary u = {1.0, {0.0, 0.0, 0.0}, &sh};
+    q_pieces(&r, (double)texit, sp, cuts, act, q_primary_piece, &u);
+    for (int c = 0; c < 3; ++c) e[c] = u.e[c];
+  }
+  free(sp);
+  free(sp2);
+  free(cuts);
+  free(cuts2);
+  free(act);
+  free(act2);
+  scene_free(&s);
+  return rc;
+}
+
+/* ======================================================================
  * TAA resolve (deviceCodeCommon.hlsl:27-43,67-115); border texels are
  * clamped to the edge.  Writes taa[] and copies it to taa_prev[]
  * (hostCode.cpp:829-830).
