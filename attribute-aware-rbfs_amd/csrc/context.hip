@@ -154,7 +154,7 @@ const char *aarbf_last_error(const aarbf_context *ctx) {
 
 int aarbf_set_render_mode(aarbf_context *ctx, int mode) {
   if (!ctx) return AARBF_ERR_INVALID_ARGUMENT;
-  AARBF_REQUIRE(ctx, mode == 0 || mode == 1, "aarbf_set_render_mode: mode must be 0 or 1");
+  AARBF_REQUIRE(ctx, mode >= 0 && mode <= 2, "aarbf_set_render_mode: mode must be 0, 1 or 2");
   ctx->render_mode = mode;
   return AARBF_OK;
 }

@@ -901,6 +901,246 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
     }
 }
 
+// ---- majorant tracking (render mode 2) -------------------------------------------
+// Delta tracking with local majorants instead of the reference's global
+// majorant of 1 per unit.  Between two entry/exit events of the candidate
+// list the density is at most
+//     mu = M(sum of alpha^3 over the active candidates),
+// M(u) = max of the density map over [0, u] (each covering particle adds
+// rbf * alpha^3 <= alpha^3), and mu = 0 where no candidate is active.  A
+// free flight of optical depth -ln(1 - xi) is spent at rate mu / unit, so
+// empty space costs nothing and faint regions few steps; a tentative
+// collision is accepted with probability D / mu.  The estimator has the
+// same expectation as the reference's (delta tracking is unbiased for any
+// majorant that bounds the density) but consumes random numbers
+// differently, so images agree with the reference in distribution only;
+// they are checked against the FP64 quadrature.  Free flights do not
+// survive a segment end or the step budget (the exponential distribution
+// is memoryless, so a fresh draw there is exact).
+template <bool STATS>
+__global__ __launch_bounds__(kSBlock, 4) void wf_step_m(RenderArgs A, WFState S, const uint32_t *__restrict__ listA,
+                                                   const uint32_t *__restrict__ countA, uint32_t *__restrict__ outA,
+                                                   uint32_t *__restrict__ outG, uint32_t *__restrict__ counts,
+                                                   uint32_t *__restrict__ outD, uint32_t *__restrict__ countD,
+                                                   uint32_t step_cap) {
+  __shared__ uint32_t s_idx[kCand * kSBlock];
+  __shared__ float s_tin[kCand * kSBlock];
+  __shared__ float s_tout[kCand * kSBlock];
+  __shared__ float s_a3[kCand * kSBlock];
+  __shared__ uint8_t s_eo[kCand * kSBlock];
+  __shared__ float s_dm[kDmLds], s_dmax[kDmLds];
+  const bool dm_lds = A.dm_n <= kDmLds;
+  if (dm_lds)
+    for (uint32_t j = threadIdx.x; j < A.dm_n; j += kSBlock) s_dm[j] = A.dmap[j].x;
+  __syncthreads();
+  if (dm_lds && threadIdx.x == 0) { // prefix maxima of the texels
+    float m = 0.0f;
+    for (uint32_t j = 0; j < A.dm_n; ++j) {
+      m = fmaxf(m, s_dm[j]);
+      s_dmax[j] = m;
+    }
+  }
+  __syncthreads();
+  auto density = [&](float u) {
+    if (!dm_lds) return tf_r(A.dmap, A.dm_n, u);
+    uint32_t j0, j1;
+    float f;
+    aa_tf_coords(u, A.dm_n, &j0, &j1, &f);
+    return aa_lerp_tex(s_dm[j0], s_dm[j1], f);
+  };
+  // max of the density map over [0, u]
+  auto dmax = [&](float u) {
+    if (!dm_lds) return 1.0f;
+    uint32_t j0, j1;
+    float f;
+    aa_tf_coords(u, A.dm_n, &j0, &j1, &f);
+    return fmaxf(aa_lerp_tex(s_dm[j0], s_dm[j1], f), s_dmax[j0]);
+  };
+  const uint32_t n_a = *countA;
+  uint32_t gid = blockIdx.x * kSBlock + threadIdx.x;
+  Counters c = {};
+  int outcome = 0; // 0 nothing, 1 keep active, 2 active + gather, 3 dense queue
+  uint32_t slot = 0;
+  if (gid < n_a) {
+    slot = listA[gid];
+    if ((S.u[slot].z >> 16) & F_DENSE) outcome = 3;
+  }
+  if (gid < n_a && outcome == 0) {
+    Lane L;
+    load_lane(S, slot, L);
+    uint32_t *li = s_idx + threadIdx.x;
+    float *ltin = s_tin + threadIdx.x, *ltout = s_tout + threadIdx.x, *la3 = s_a3 + threadIdx.x;
+    uint8_t *leo = s_eo + threadIdx.x;
+    uint32_t n = 0, act = 0, eptr = 0;
+    float t_enter = AA_INF_F, t_exit = AA_INF_F, mu = 0.0f;
+    auto remajor = [&]() {
+      float s3 = 0.0f;
+      t_exit = AA_INF_F;
+      for (uint32_t m = act; m; m &= m - 1u) {
+        uint32_t k = (uint32_t)__ffs(m) - 1u;
+        s3 += la3[k * kSBlock];
+        t_exit = fminf(t_exit, ltout[k * kSBlock]);
+      }
+      mu = act ? fminf(1.0f, dmax(s3) * 1.00001f) : 0.0f;
+    };
+    auto enter = [&](float t) {
+      while (eptr < n) {
+        uint32_t k = leo[eptr * kSBlock];
+        if (!(ltin[k * kSBlock] <= t)) break;
+        if (t < ltout[k * kSBlock]) act |= 1u << k;
+        ++eptr;
+      }
+      t_enter = eptr < n ? ltin[leo[eptr * kSBlock] * kSBlock] : AA_INF_F;
+    };
+    auto leave = [&](float t) {
+      for (uint32_t m = act; m; m &= m - 1u) {
+        uint32_t k = (uint32_t)__ffs(m) - 1u;
+        if (ltout[k * kSBlock] <= t) act &= ~(1u << k);
+      }
+    };
+    auto setup = [&]() {
+      n = (L.flags & F_DENSE) ? 0u : L.seg_n;
+      const bool hd = (L.flags & F_HEAD) != 0;
+      const uint32_t *iidx = hd ? S.hidx : S.cidx;
+      const float *itin = hd ? S.htin : S.ctin, *itout = hd ? S.htout : S.ctout;
+      const uint8_t *ieo = hd ? S.heo : S.ceo;
+      const uint64_t row = hd ? L.pix : slot, stride = hd ? S.npix : S.nslots;
+      for (uint32_t k = 0; k < n; ++k) {
+        uint32_t p = iidx[k * stride + row];
+        li[k * kSBlock] = p;
+        ltin[k * kSBlock] = itin[k * stride + row];
+        ltout[k * kSBlock] = itout[k * stride + row];
+        leo[k * kSBlock] = ieo[k * stride + row];
+        la3[k * kSBlock] = A.pc[p].x;
+      }
+      act = 0;
+      eptr = 0;
+      enter(L.t);
+      leave(L.t);
+      remajor();
+    };
+    setup();
+    float tau = -1.0f; // optical depth left of the current free flight
+    uint32_t steps = 0;
+    // A ray the dense-segment kernel left with a drawn step (majorant 1)
+    // first has that tentative collision tested, with acceptance D / 1.
+    bool pend = (L.flags & F_PENDING) != 0;
+    L.flags &= ~F_PENDING;
+    for (;;) {
+      bool end = false;
+      if (pend) {
+        tau = 0.0f;
+      } else if (tau < 0.0f) {
+        if (!(L.i < A.max_depth)) {
+          end = true;
+        } else if (steps >= step_cap) {
+          outcome = 1;
+          break;
+        } else {
+          ++steps;
+          if (STATS) c.steps++;
+          float xi = aa_lcg_next(&L.trng);
+          tau = -aa_log_step(1.0f - xi);
+        }
+      }
+      if (!end) {
+        const float ev = fminf(fminf(t_enter, t_exit), fminf(L.seg_end, L.tmax));
+        const float dt = pend ? 0.0f : (mu > 0.0f ? tau * (A.unit / mu) : AA_INF_F);
+        const float mu_c = pend ? 1.0f : mu;
+        pend = false;
+        if (L.t + dt < ev || dt == 0.0f) {
+          // tentative collision
+          L.t = L.t + dt;
+          tau = -1.0f;
+          ++L.i;
+          const float x = L.o.x + L.t * L.d.x, y = L.o.y + L.t * L.d.y, z = L.o.z + L.t * L.d.z;
+          Payload pl;
+          pl.count = 0;
+          pl.dens = 0.0f;
+          pl.r = pl.g = pl.b = 0.0f;
+          for (uint32_t m = act; m; m &= m - 1u) {
+            uint32_t k = (uint32_t)__ffs(m) - 1u;
+            uint32_t p = li[k * kSBlock];
+            float4 qq = A.pr[p];
+            float4 kk = A.pc[p];
+            if (STATS) c.particles++;
+            float dd = aa_dist3(qq.x, qq.y, qq.z, x, y, z);
+            if (dd < qq.w) {
+              float wa = aa_rbf(dd, qq.w) * kk.x;
+              if (STATS) c.covering++;
+              pl.count += 1;
+              pl.dens += wa;
+              pl.r += kk.y * wa;
+              pl.g += kk.z * wa;
+              pl.b += kk.w * wa;
+              if (!A.visAttr) pl.dens = fminf(pl.dens, 1.0f);
+            }
+          }
+          if (pl.count > 0) {
+            float D = density(pl.dens);
+            float xi2 = aa_lcg_next(&L.trng);
+            if (xi2 * mu_c < D) {
+              if (L.flags & F_SHADOW) {
+                L.shw = 1.0f;
+              } else {
+                float3 col = colour_of(A, pl);
+                L.alb = make_float4(col.x, col.y, col.z, 1.0f);
+              }
+              end = true;
+            }
+          }
+        } else {
+          // reach the next event
+          if (mu > 0.0f) tau = fmaxf(0.0f, tau - (ev - L.t) * (mu / A.unit));
+          L.t = fmaxf(L.t, ev);
+          if (!(L.t < L.tmax)) {
+            end = true; // left the clip box
+          } else if (!(L.t < L.seg_end)) {
+            outcome = 2; // the next segment needs a gather
+            break;
+          } else {
+            if (L.t >= t_enter) enter(L.t);
+            leave(L.t);
+            remajor();
+          }
+        }
+      }
+      if (!end) continue;
+      // ---- the ray ended
+      if (!(L.flags & F_SHADOW) && L.alb.w > 0.0f && A.amb != 1.0f) {
+        float3 so = make_float3(A.pos.x + L.d.x * L.t, A.pos.y + L.d.y * L.t, A.pos.z + L.d.z * L.t);
+        float ste, stx;
+        aabb_intersection(so, A.light, 0.0f, 10000.0f, A.lb, A.rt, ste, stx);
+        L.o = so;
+        L.d = A.light;
+        L.tmax = stx;
+        L.flags = F_SHADOW;
+        L.shw = 0.0f;
+        L.LHS = 0.0f;
+        L.t = 0.0f;
+        L.seg_end = -AA_INF_F;
+        outcome = 2;
+        break;
+      }
+      float vis = (L.flags & F_SHADOW) ? 1.0f - L.shw : 1.0f;
+      finish_sample<STATS>(A, S, L, vis, c);
+      outcome = next_ray<STATS>(A, S, L, c);
+      if (outcome == 1 && !(L.flags & F_DENSE)) {
+        setup();
+        tau = -1.0f;
+        outcome = 0;
+        continue;
+      }
+      break;
+    }
+    if (outcome) save_lane(S, slot, L);
+  }
+  if (outcome == 1 || outcome == 2) push(outA, &counts[0], slot);
+  if (outcome == 2) push(outG, &counts[1], slot);
+  if (outcome == 3) push(outD, countD, slot);
+  flush_stats<STATS>(A, c);
+}
+
 __global__ void wf_zero(uint32_t *c, int n) {
   if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
 }
@@ -941,7 +1181,12 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
     const uint64_t base_cap = (uint64_t)ctx->tune_step_cap;
     uint32_t cap = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(base_cap, base_cap * nslots / (4ull * nA)));
     const uint32_t blocks = (nA + kSBlock - 1) / kSBlock;
-    if (st) {
+    if (ctx->render_mode == 2 && !A.blue) {
+      if (st)
+        wf_step_m<true><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+      else
+        wf_step_m<false><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+    } else if (st) {
       wf_step<true, false, 1><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
     } else {
       switch (ctx->tune_step_waves) {

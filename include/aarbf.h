@@ -149,9 +149,13 @@ void *aarbf_get_stream(aarbf_context *ctx);
 int aarbf_sync(aarbf_context *ctx);
 const char *aarbf_last_error(const aarbf_context *ctx);
 const char *aarbf_status_string(int status);
-/* Render schedule: 0 (default) persistent waves with per-segment candidate
- * gathering, 1 one lane per pixel with a BVH point query per step.  Both
- * produce identical images; no counterpart in the reference. */
+/* Render schedule: 0 (default) wavefront rounds of per-segment candidate
+ * gathering and tracker steps, 1 one lane per pixel with a BVH point query
+ * per step; both produce the reference's images bit for bit.  2: delta
+ * tracking with local majorants (empty space skipped, faint regions
+ * stepped at their own density bound): the same expected image, different
+ * random-number use, so per-sample results differ; stochastic marching is
+ * rendered as in mode 0.  No counterpart in the reference. */
 int aarbf_set_render_mode(aarbf_context *ctx, int mode);
 /* Schedule parameters of the default renderer (none changes the image):
  * AARBF_TUNE_STEP_WAVES: occupancy target of the step kernel in waves per
