@@ -1141,6 +1141,222 @@ __global__ __launch_bounds__(kSBlock, 4) void wf_step_m(RenderArgs A, WFState S,
   flush_stats<STATS>(A, c);
 }
 
+// ---- dense segments: one wave per ray ---------------------------------------------
+// Rays in crowded clusters (more than kCand candidates where the segment
+// starts) are few, but a point query there finds hundreds of particles, and
+// one lane walking them alone leaves the GPU waiting on a handful of rays.
+// Here a whole wave runs one ray: the tracker arithmetic is uniform across
+// the wave, and each point query is a level-by-level walk whose frontier
+// (nodes holding the point, in ascending order) lives in LDS, 64 child
+// boxes tested per pass.  The particles of the leaves holding the point are
+// tested 64 at a time and the covering ones folded in ascending index
+// order, so the payload equals the serial query's bit for bit.
+constexpr uint32_t kFront = 2048; // frontier entries per level
+
+template <bool STATS>
+__global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S, const uint32_t *__restrict__ listD,
+                                                        const uint32_t *__restrict__ countD,
+                                                        uint32_t *__restrict__ outA, uint32_t *__restrict__ outG,
+                                                        uint32_t *__restrict__ counts, uint32_t step_cap) {
+  __shared__ uint32_t s_front[2][kFront];
+  __shared__ float s_dm[kDmLds];
+  load_levels(A);
+  const bool dm_lds = A.dm_n <= kDmLds;
+  if (dm_lds)
+    for (uint32_t j = threadIdx.x; j < A.dm_n; j += kSBlock) s_dm[j] = A.dmap[j].x;
+  __syncthreads();
+  auto density = [&](float u) {
+    if (!dm_lds) return tf_r(A.dmap, A.dm_n, u);
+    uint32_t j0, j1;
+    float f;
+    aa_tf_coords(u, A.dm_n, &j0, &j1, &f);
+    return aa_lerp_tex(s_dm[j0], s_dm[j1], f);
+  };
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  if (blockIdx.x >= *countD) return;
+  const uint32_t slot = listD[blockIdx.x];
+  const bool marching = A.blue != 0;
+  Counters c = {};
+  const float4 *__restrict__ N = A.bvh.nodes;
+  const uint32_t log2K = A.bvh.log2K, top = A.bvh.top;
+
+  // payload at (x, y, z); false when a frontier overflows
+  auto query = [&](float x, float y, float z, Payload &pl) -> bool {
+    pl.count = 0;
+    pl.dens = 0.0f;
+    pl.r = pl.g = pl.b = 0.0f;
+    const float4 *root = N + g_lvl_ofs[top];
+    if (!inside(root[0], root[1], x, y, z)) return true;
+    uint32_t *cur = s_front[0], *nxt = s_front[1];
+    if (lane == 0) cur[0] = 0;
+    uint32_t nf = 1;
+    __syncthreads();
+    for (uint32_t lvl = top; lvl >= 1; --lvl) {
+      const float4 *cl = N + g_lvl_ofs[lvl - 1];
+      const uint32_t cnt = g_lvl_cnt[lvl - 1], total = nf << log2K;
+      uint32_t nn = 0;
+      for (uint32_t base = 0; base < total; base += 64u) {
+        const uint32_t j = base + lane;
+        bool ok = false;
+        uint32_t child = 0;
+        if (j < total) {
+          child = (cur[j >> log2K] << log2K) + (j & ((1u << log2K) - 1u));
+          ok = child < cnt && inside(cl[2 * child], cl[2 * child + 1], x, y, z);
+        }
+        const uint64_t m = __ballot(ok);
+        const uint32_t pos = nn + (uint32_t)__popcll(m & lt);
+        if (ok && pos < kFront) nxt[pos] = child;
+        nn += (uint32_t)__popcll(m);
+      }
+      if (nn > kFront) return false;
+      __syncthreads();
+      uint32_t *t = cur;
+      cur = nxt;
+      nxt = t;
+      nf = nn;
+      if (nf == 0) return true;
+    }
+    // cur: the leaves whose box holds the point, ascending
+    const uint32_t ppl = A.ppl, total = nf * ppl;
+    for (uint32_t base = 0; base < total; base += 64u) {
+      const uint32_t j = base + lane;
+      bool cov = false;
+      float wa = 0.0f, wr = 0.0f, wg = 0.0f, wb = 0.0f;
+      if (j < total) {
+        const uint32_t p = cur[j / ppl] * ppl + j % ppl;
+        if (p < A.numParticles) {
+          const float4 q = A.pr[p];
+          const float d = aa_dist3(q.x, q.y, q.z, x, y, z);
+          if (d < q.w) {
+            const float4 k = A.pc[p];
+            wa = aa_rbf(d, q.w) * k.x;
+            wr = k.y * wa;
+            wg = k.z * wa;
+            wb = k.w * wa;
+            cov = true;
+          }
+        }
+      }
+      // fold the covering particles in ascending order
+      for (uint64_t m = __ballot(cov); m; m &= m - 1ull) {
+        const int l = __ffsll((unsigned long long)m) - 1;
+        pl.count += 1;
+        pl.dens += __shfl(wa, l, 64);
+        pl.r += __shfl(wr, l, 64);
+        pl.g += __shfl(wg, l, 64);
+        pl.b += __shfl(wb, l, 64);
+        if (!A.visAttr) pl.dens = fminf(pl.dens, 1.0f);
+      }
+      if (STATS && lane == 0) c.particles += min(64u, total - base);
+    }
+    if (STATS && lane == 0) c.covering += pl.count;
+    return true;
+  };
+
+  Lane L;
+  load_lane(S, slot, L);
+  int outcome = 0;
+  uint32_t steps = 0;
+  for (;;) {
+    const bool pending = (L.flags & F_PENDING) != 0;
+    bool end = false;
+    if (!pending) {
+      if (!(L.i < A.max_depth)) {
+        end = true;
+      } else if (steps >= step_cap) {
+        outcome = 1;
+        break;
+      } else {
+        ++steps;
+        if (STATS && lane == 0) c.steps++;
+        if (marching) {
+          L.t = L.t + A.unit;
+        } else {
+          float xi = aa_lcg_next(&L.trng);
+          L.t = L.t - aa_log_step(1.0f - xi) * A.unit;
+        }
+        if (L.t >= L.tmax) end = true;
+        else L.flags |= F_PENDING;
+      }
+    }
+    if (!end) {
+      if (!(L.t < L.seg_end)) {
+        outcome = 2;
+        break;
+      }
+      Payload pl;
+      const float x = L.o.x + L.t * L.d.x, y = L.o.y + L.t * L.d.y, z = L.o.z + L.t * L.d.z;
+      if (STATS && lane == 0) c.queries++;
+      if (!query(x, y, z, pl)) point_query<false>(A, x, y, z, pl, c); // frontier overflow: serial walk
+      L.flags &= ~F_PENDING;
+      const bool shadow = (L.flags & F_SHADOW) != 0;
+      if (marching) {
+        if (pl.count > 0 && pl.dens > 0.0f) {
+          float D = density(pl.dens);
+          L.LHS += D;
+          if (L.LHS > L.RHS && !shadow) {
+            float3 col = colour_of(A, pl);
+            L.alb = make_float4(col.x, col.y, col.z, 1.0f);
+            end = true;
+          } else if (shadow) {
+            L.shw += 1.0f - aa_expf(-D);
+            if (L.shw >= 1.0f) {
+              L.shw = 1.0f;
+              end = true;
+            }
+          }
+        }
+      } else if (pl.count > 0) {
+        float D = density(pl.dens);
+        float xi2 = aa_lcg_next(&L.trng);
+        if (xi2 < D) {
+          if (shadow) {
+            L.shw = 1.0f;
+          } else {
+            float3 col = colour_of(A, pl);
+            L.alb = make_float4(col.x, col.y, col.z, 1.0f);
+          }
+          end = true;
+        }
+      }
+      if (!end) {
+        ++L.i;
+        continue;
+      }
+    }
+    // ---- the ray ended
+    L.flags &= ~F_PENDING;
+    if (!(L.flags & F_SHADOW) && L.alb.w > 0.0f && A.amb != 1.0f) {
+      float3 so = make_float3(A.pos.x + L.d.x * L.t, A.pos.y + L.d.y * L.t, A.pos.z + L.d.z * L.t);
+      float ste, stx;
+      aabb_intersection(so, A.light, 0.0f, 10000.0f, A.lb, A.rt, ste, stx);
+      L.o = so;
+      L.d = A.light;
+      L.tmax = stx;
+      L.flags = F_SHADOW;
+      L.shw = 0.0f;
+      L.LHS = 0.0f;
+      L.t = marching ? -A.unit * L.r2 : 0.0f;
+      L.seg_end = -AA_INF_F;
+      outcome = 2;
+      break;
+    }
+    if (lane == 0) {
+      float vis = (L.flags & F_SHADOW) ? 1.0f - L.shw : 1.0f;
+      finish_sample<STATS>(A, S, L, vis, c);
+      outcome = next_ray<STATS>(A, S, L, c);
+    }
+    break;
+  }
+  if (lane == 0) {
+    if (outcome) save_lane(S, slot, L);
+    if (outcome == 1 || outcome == 2) push(outA, &counts[0], slot);
+    if (outcome == 2) push(outG, &counts[1], slot);
+  }
+  flush_stats<STATS>(A, c);
+}
+
 __global__ void wf_zero(uint32_t *c, int n) {
   if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
 }
@@ -1207,14 +1423,12 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
     AARBF_HIP(ctx, hipMemcpyAsync(&nD, cnt + 4, 4, hipMemcpyDeviceToHost, ctx->stream));
     AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (nD > 0) {
-      const uint32_t dblocks = (nD + kSBlock - 1) / kSBlock;
+      // one wave per dense ray
       if (st)
-        wf_step<true, true, 1><<<dblocks, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, nullptr,
-                                                                     nullptr, cap);
+        wf_dense_wave<true><<<nD, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, cap);
       else
-        wf_step<false, true, 1><<<dblocks, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, nullptr,
-                                                                      nullptr, cap);
-      AARBF_CHECK_LAUNCH(ctx, "wf_step (dense)");
+        wf_dense_wave<false><<<nD, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, cap);
+      AARBF_CHECK_LAUNCH(ctx, "wf_dense_wave");
     }
     if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[3], ctx->stream));
     AARBF_HIP(ctx, hipMemcpyAsync(h, cN, 8, hipMemcpyDeviceToHost, ctx->stream));
