@@ -202,6 +202,9 @@ class Context:
         self.check(self.lib.aarbf_tf_create(self.h, _fptr(t), t.shape[0], C.byref(h)), "aarbf_tf_create")
         return h.value
 
+    def tf_destroy(self, handle: int):
+        self.check(self.lib.aarbf_tf_destroy(self.h, handle), "aarbf_tf_destroy")
+
     def tf_rgba8(self, texels, srgb: bool = True) -> int:
         t = np.ascontiguousarray(np.asarray(texels, dtype=np.uint8).reshape(-1, 4))
         h = C.c_uint64()
@@ -266,6 +269,22 @@ class Context:
     def render(self, rg: RayGenData, pc: PushConstants, width: int, height: int, spp: int = 1):
         self.check(self.lib.aarbf_render_rbf_spp(self.h, C.byref(rg), C.byref(pc), width, height, spp),
                    "aarbf_render_rbf_spp")
+
+    def diff_forward(self, rg: RayGenData, pc: PushConstants, width: int, height: int, spp: int, image, path=None):
+        """Expected stochastic-marching image into `image` (float32 [H, W, 4]);
+        `path` (float32 [H, W, spp, 4]) optionally keeps what the backward needs."""
+        self.check(self.lib.aarbf_diff_forward(self.h, C.byref(rg), C.byref(pc), width, height, spp,
+                                               image.data_ptr(), path.data_ptr() if path is not None else 0),
+                   "aarbf_diff_forward")
+
+    def diff_backward(self, rg: RayGenData, pc: PushConstants, width: int, height: int, spp: int, grad_image,
+                      g_pos, g_radii, g_attr, g_colormap, g_densitymap, g_radiusmap, path=None):
+        """Adds the adjoint of diff_forward into the (zeroed) gradient tensors."""
+        self.check(self.lib.aarbf_diff_backward(self.h, C.byref(rg), C.byref(pc), width, height, spp,
+                                                path.data_ptr() if path is not None else 0,
+                                                grad_image.data_ptr(), g_pos.data_ptr(), g_radii.data_ptr(),
+                                                g_attr.data_ptr(), g_colormap.data_ptr(), g_densitymap.data_ptr(),
+                                                g_radiusmap.data_ptr()), "aarbf_diff_backward")
 
     def render_splat(self, rg: RayGenData, pc: PushConstants, width: int, height: int):
         self.check(self.lib.aarbf_render_splat(self.h, C.byref(rg), C.byref(pc), width, height),

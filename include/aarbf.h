@@ -288,6 +288,29 @@ int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg,
 /* ParticleSplatRayGen (deviceCodeSplat.hlsl:58-154). */
 int aarbf_render_splat(aarbf_context *ctx, const aarbf_raygen_data *rg,
                        const aarbf_push_constants *pc, int width, int height);
+/* ---- differentiable rendering (no counterpart in the reference) -------------
+ * The expectation of stochastic marching (deviceCodeRBF.hlsl:67-139, primary
+ * ray, ambient 1) over its blue-noise threshold: each sample marches
+ * t_k = tenter + unit * (r1 + k), r1 its first LCG draw, and the pixel is the
+ * mean over spp samples of sum_k (F(L_k) - F(L_{k-1})) c_k with L_k the
+ * accumulated densitymap values, F(L) = min(1, (1 - e^-L) / 0.99) and c_k the
+ * attribute-weighted colour.  Needs per-particle radii (pc->radii) and
+ * visualizeAttributes.  image: device float4 [height][width] (rgb, 1).
+ * path: 0, or a device float4 [height][width][spp] buffer that receives one
+ * vector per sample for aarbf_diff_backward (it then replays each sample
+ * once instead of twice). */
+int aarbf_diff_forward(aarbf_context *ctx, const aarbf_raygen_data *rg, const aarbf_push_constants *pc,
+                       int width, int height, uint32_t spp, uint64_t image, uint64_t path);
+/* Adds dLoss/dparameters to the gradient buffers (device float, zeroed by
+ * the caller): g_pos [3N] (x, y, z per particle), g_radii [N], g_attr [N],
+ * g_colormap [4 * colormap texels], g_densitymap / g_radiusmap [texels]
+ * (the red channel they are read from).  grad_image: device float4
+ * [height][width], rgb used; path: the forward's path buffer (same scene
+ * and view) or 0.  Stores nothing per step. */
+int aarbf_diff_backward(aarbf_context *ctx, const aarbf_raygen_data *rg, const aarbf_push_constants *pc,
+                        int width, int height, uint32_t spp, uint64_t path, uint64_t grad_image,
+                        uint64_t g_pos, uint64_t g_radii, uint64_t g_attr, uint64_t g_colormap,
+                        uint64_t g_densitymap, uint64_t g_radiusmap);
 /* CompositeGui's TAA resolve and BGRA8 packing (deviceCodeCommon.hlsl:67-115)
  * plus the taa -> taaPrev history copy (hostCode.cpp:819-830). */
 int aarbf_taa_resolve(aarbf_context *ctx, const aarbf_taa_constants *tc);

@@ -341,3 +341,70 @@ def taa_resolve(image: np.ndarray, taa: np.ndarray, taa_prev: np.ndarray, disabl
     H, W = image.shape[:2]
     im = np.ascontiguousarray(image, np.float32)
     lib().orc_taa_resolve(_f(im), _f(taa), _f(taa_prev), W, H, int(disable))
+
+
+# ---- differentiable renderer (adjoint.c) -----------------------------------------------
+
+class OrcDiff(C.Structure):
+    _fields_ = [
+        ("pos", C.POINTER(C.c_double)), ("attr", C.POINTER(C.c_double)), ("radii", C.POINTER(C.c_double)),
+        ("n", C.c_uint32),
+        ("cm", C.POINTER(C.c_double)), ("dm", C.POINTER(C.c_double)), ("rm", C.POINTER(C.c_double)),
+        ("cm_n", C.c_uint32), ("dm_n", C.c_uint32), ("rm_n", C.c_uint32),
+        ("cam_pos", C.c_float * 3), ("cam_d00", C.c_float * 3), ("cam_du", C.c_float * 3), ("cam_dv", C.c_float * 3),
+        ("width", C.c_int32), ("height", C.c_int32),
+        ("spp", C.c_uint32), ("frameID", C.c_uint32), ("max_depth", C.c_uint32),
+        ("unit", C.c_float), ("rbfRadius", C.c_float),
+        ("aabb", C.c_float * 6),
+    ]
+
+
+class Diff:
+    """FP64 expected-marching image and its adjoint (oracle/adjoint.c).
+    pos [N,3], attr [N], radii [N], TF tables [n,4]; all float64."""
+
+    def __init__(self, pos, attr, radii, colormap, densitymap, radiusmap, camera, width, height, unit, rbf_radius,
+                 aabb, spp=1, frame_id=0, max_depth=0):
+        self.keep = []
+
+        def arr(a):
+            a = np.ascontiguousarray(a, np.float64)
+            self.keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_double))
+
+        p = OrcDiff()
+        p.pos, p.attr, p.radii = arr(pos), arr(attr), arr(radii)
+        p.n = len(attr)
+        cm, dm, rm = (np.asarray(t, np.float64).reshape(-1, 4) for t in (colormap, densitymap, radiusmap))
+        p.cm, p.dm, p.rm = arr(cm), arr(dm), arr(rm)
+        p.cm_n, p.dm_n, p.rm_n = len(cm), len(dm), len(rm)
+        for name, v in (("cam_pos", camera.pos), ("cam_d00", camera.dir_00), ("cam_du", camera.dir_du),
+                        ("cam_dv", camera.dir_dv)):
+            getattr(p, name)[:] = [float(np.float32(x)) for x in v]
+        p.width, p.height, p.spp, p.frameID, p.max_depth = width, height, spp, frame_id, max_depth
+        p.unit, p.rbfRadius = unit, rbf_radius
+        p.aabb[:] = [float(x) for x in aabb[:6]]
+        self.p = p
+        self.sizes = (len(attr), len(cm), len(dm), len(rm))
+        L = lib()
+        L.orc_diff_forward.restype = C.c_int
+        L.orc_diff_forward.argtypes = [C.POINTER(OrcDiff), C.POINTER(C.c_double)]
+        L.orc_diff_adjoint.restype = C.c_int
+        L.orc_diff_adjoint.argtypes = [C.POINTER(OrcDiff)] + [C.POINTER(C.c_double)] * 7
+
+    def forward(self) -> np.ndarray:
+        img = np.zeros((self.p.height, self.p.width, 3), np.float64)
+        if lib().orc_diff_forward(C.byref(self.p), img.ctypes.data_as(C.POINTER(C.c_double))) != 0:
+            raise RuntimeError("orc_diff_forward failed")
+        return img
+
+    def adjoint(self, gimage: np.ndarray) -> dict:
+        n, ncm, ndm, nrm = self.sizes
+        g = np.ascontiguousarray(gimage, np.float64)
+        out = {"pos": np.zeros((n, 3)), "radii": np.zeros(n), "attr": np.zeros(n), "colormap": np.zeros((ncm, 4)),
+               "densitymap": np.zeros(ndm), "radiusmap": np.zeros(nrm)}
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        if lib().orc_diff_adjoint(C.byref(self.p), dp(g), dp(out["pos"]), dp(out["radii"]), dp(out["attr"]),
+                                  dp(out["colormap"]), dp(out["densitymap"]), dp(out["radiusmap"])) != 0:
+            raise RuntimeError("orc_diff_adjoint failed")
+        return out

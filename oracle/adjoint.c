@@ -1,0 +1,372 @@
+/*
+ * adjoint.c -- FP64 restatement of the differentiable renderer.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP forward/adjoint
+ * kernels (csrc/diff.hip), loaded by the test suite.
+ *
+ * The differentiable image is the expectation of the reference's
+ * stochastic marching (deviceCodeRBF.hlsl:67-139, primary ray, ambient 1)
+ * over its blue-noise threshold: a sample marches the points
+ * t_k = tenter + unit*(r1 + k) (r1 the sample's first LCG draw, as in
+ * ParticleRBFRayGen), accumulates L_k = D_1 + ... + D_k over the points
+ * some particle covers (D = densitymap(density)), and stops at point k with
+ * probability F(L_k) - F(L_{k-1}), F(L) = min(1, (1 - e^-L) / 0.99) =
+ * P(-ln(1 - 0.99 xi) < L) for uniform xi.  The sample's colour is
+ *     C = sum_k (F(L_k) - F(L_{k-1})) c_k,
+ * c_k the attribute-weighted colour at point k, and a pixel is the mean of
+ * its samples.  C is a smooth function of particle positions, radii,
+ * attributes and transfer-function texels except where a sphere boundary
+ * crosses a march point (the truncated RBF jumps there; such jumps are not
+ * differentiated).
+ *
+ * The adjoint replays each sample twice (path replay, nothing stored per
+ * step): the first pass sums dC/dL_k = F'(L_k)(c_k - c_{k+1}) over all k,
+ * the second recovers dC/dD_j = sum_{k>=j} dC/dL_k as total minus prefix and
+ * pushes it, with dC/dc_j = F(L_j) - F(L_{j-1}), through the densitymap,
+ * the colour mean and each covering particle's RBF weight, radius,
+ * colormap and radiusmap lookups.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/aarbf_detmath.h"
+
+#define ORC_EXPORT __attribute__((visibility("default")))
+
+typedef struct orc_diff {
+  const double *pos;   /* 3N */
+  const double *attr;  /* N */
+  const double *radii; /* N */
+  uint32_t n;
+  const double *cm, *dm, *rm; /* RGBA texels (dm, rm: channel 0 used) */
+  uint32_t cm_n, dm_n, rm_n;
+  float cam_pos[3], cam_d00[3], cam_du[3], cam_dv[3];
+  int32_t width, height;
+  uint32_t spp, frameID, max_depth;
+  float unit, rbfRadius;
+  float aabb[6];
+} orc_diff;
+
+/* linear texel filter with clamp-to-edge (aa_tf_coords in double): value,
+ * slope d/du, and the two texels with their weights */
+static double tfd(const double *t, uint32_t n, int c, double u, double *slope, uint32_t *j0, uint32_t *j1,
+                  double *f) {
+  double x = u * (double)n - 0.5;
+  int clamped = 0;
+  if (!(x > -1.0)) x = -1.0;
+  if (x > (double)n) x = (double)n;
+  double fl = floor(x);
+  int i0 = (int)fl;
+  *f = x - fl;
+  int a = i0 < 0 ? 0 : (i0 > (int)n - 1 ? (int)n - 1 : i0);
+  int b = i0 + 1 < 0 ? 0 : (i0 + 1 > (int)n - 1 ? (int)n - 1 : i0 + 1);
+  if (a == b) clamped = 1;
+  *j0 = (uint32_t)a;
+  *j1 = (uint32_t)b;
+  double ta = t[4 * a + c], tb = t[4 * b + c];
+  *slope = clamped ? 0.0 : (double)n * (tb - ta);
+  return ta + *f * (tb - ta);
+}
+
+static double Fcdf(double L, double *dF) {
+  double v = (1.0 - exp(-L)) / 0.99;
+  if (v >= 1.0) {
+    *dF = 0.0;
+    return 1.0;
+  }
+  *dF = exp(-L) / 0.99;
+  return v;
+}
+
+typedef struct {
+  double alpha, a3, c[3], c22[3], r, rm;
+} dpart;
+
+static void part_values(const orc_diff *p, uint32_t i, dpart *q) {
+  double s, f;
+  uint32_t j0, j1;
+  for (int c = 0; c < 3; ++c) q->c[c] = tfd(p->cm, p->cm_n, c, p->attr[i], &s, &j0, &j1, &f);
+  q->alpha = tfd(p->cm, p->cm_n, 3, p->attr[i], &s, &j0, &j1, &f);
+  q->a3 = q->alpha * q->alpha * q->alpha;
+  for (int c = 0; c < 3; ++c) q->c22[c] = q->c[c] > 0.0 ? pow(q->c[c], 2.2) : 0.0;
+  q->rm = tfd(p->rm, p->rm_n, 0, p->attr[i], &s, &j0, &j1, &f);
+  q->r = q->alpha == 0.0 ? 0.0 : p->radii[i] * q->rm;
+}
+
+/* one march point: density, colour sums and count at x */
+typedef struct {
+  uint32_t count;
+  double rho, S[3];
+} dpay;
+
+static void query(const orc_diff *p, const dpart *pp, const double x[3], dpay *pl) {
+  pl->count = 0;
+  pl->rho = pl->S[0] = pl->S[1] = pl->S[2] = 0.0;
+  for (uint32_t i = 0; i < p->n; ++i) {
+    if (!(pp[i].r > 0.0)) continue;
+    double dx = p->pos[3 * i] - x[0], dy = p->pos[3 * i + 1] - x[1], dz = p->pos[3 * i + 2] - x[2];
+    double d = sqrt(dx * dx + dy * dy + dz * dz);
+    if (!(d < pp[i].r)) continue;
+    double q = 3.0 * d / pp[i].r;
+    double w = exp(-0.5 * q * q) * pp[i].a3;
+    pl->count += 1;
+    pl->rho += w;
+    for (int c = 0; c < 3; ++c) pl->S[c] += pp[i].c22[c] * w;
+  }
+}
+
+static void colour(const dpay *pl, double c[3]) {
+  for (int k = 0; k < 3; ++k) c[k] = pow(pl->S[k] / pl->rho, 1.0 / 2.2);
+}
+
+typedef struct {
+  float o[3], d[3];
+  float t, tmax;
+  int live;
+} dray;
+
+static void sample_ray(const orc_diff *p, int px, int py, uint32_t s, dray *r) {
+  uint32_t fid = (p->frameID + s) & 0xffffu;
+  uint32_t rng = aa_rng_seed(fid, (uint32_t)px, (uint32_t)py, (uint32_t)p->width);
+  float r1 = aa_lcg_next(&rng);
+  float sx = ((float)px + 0.5f) / (float)p->width, sy = ((float)py + 0.5f) / (float)p->height;
+  float lb[3], rt[3], d[3];
+  for (int k = 0; k < 3; ++k) {
+    rt[k] = p->aabb[3 + k] + 2.0f * p->rbfRadius;
+    lb[k] = p->aabb[k] - 2.0f * p->rbfRadius;
+    d[k] = (p->cam_d00[k] + sx * p->cam_du[k]) + sy * p->cam_dv[k];
+  }
+  float len = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+  for (int k = 0; k < 3; ++k) d[k] = d[k] / len;
+  float inv[3], t1[3], t2[3];
+  for (int k = 0; k < 3; ++k) {
+    inv[k] = 1.0f / d[k];
+    t2[k] = (rt[k] - p->cam_pos[k]) * inv[k];
+    t1[k] = (lb[k] - p->cam_pos[k]) * inv[k];
+  }
+  float te = fmaxf(fmaxf(fminf(t1[0], t2[0]), fminf(t1[1], t2[1])), fminf(t1[2], t2[2]));
+  float tx = fminf(fminf(fmaxf(t1[0], t2[0]), fmaxf(t1[1], t2[1])), fmaxf(t1[2], t2[2]));
+  te = fmaxf(te, 0.0f);
+  tx = fminf(tx, 10000.0f);
+  for (int k = 0; k < 3; ++k) {
+    r->o[k] = p->cam_pos[k];
+    r->d[k] = d[k];
+  }
+  r->live = te < tx;
+  r->t = te + p->unit * r1;
+  r->tmax = tx;
+}
+
+/* visits the march points that some particle covers: fn(k-th point, x, payload) */
+typedef void (*point_fn)(const orc_diff *p, const dpart *pp, const double x[3], const dpay *pl, void *u);
+
+static void march(const orc_diff *p, const dpart *pp, int px, int py, uint32_t s, point_fn fn, void *u) {
+  dray r;
+  sample_ray(p, px, py, s, &r);
+  if (!r.live) return;
+  float t = r.t;
+  uint32_t maxd = p->max_depth ? p->max_depth : 10000u;
+  for (uint32_t i = 0; i < maxd; ++i) {
+    t = t + p->unit;
+    if (t >= r.tmax) return;
+    float xf[3] = {r.o[0] + t * r.d[0], r.o[1] + t * r.d[1], r.o[2] + t * r.d[2]};
+    double x[3] = {xf[0], xf[1], xf[2]};
+    dpay pl;
+    query(p, pp, x, &pl);
+    if (pl.count > 0 && pl.rho > 0.0) fn(p, pp, x, &pl, u);
+  }
+}
+
+typedef struct {
+  double L, C[3];
+} fwd_state;
+
+static void fwd_point(const orc_diff *p, const dpart *pp, const double x[3], const dpay *pl, void *u) {
+  (void)pp;
+  (void)x;
+  fwd_state *st = (fwd_state *)u;
+  double s, f, dF;
+  uint32_t j0, j1;
+  double D = tfd(p->dm, p->dm_n, 0, pl->rho, &s, &j0, &j1, &f);
+  double F0 = Fcdf(st->L, &dF);
+  st->L += D;
+  double F1 = Fcdf(st->L, &dF);
+  double c[3];
+  colour(pl, c);
+  for (int k = 0; k < 3; ++k) st->C[k] += (F1 - F0) * c[k];
+}
+
+static dpart *parts(const orc_diff *p) {
+  dpart *pp = (dpart *)malloc(sizeof(dpart) * (p->n ? p->n : 1));
+  if (pp)
+    for (uint32_t i = 0; i < p->n; ++i) part_values(p, i, pp + i);
+  return pp;
+}
+
+/* image: W*H*3 doubles (mean over spp samples) */
+ORC_EXPORT int orc_diff_forward(const orc_diff *p, double *image) {
+  dpart *pp = parts(p);
+  if (!pp) return -1;
+  for (int py = 0; py < p->height; ++py)
+    for (int px = 0; px < p->width; ++px) {
+      double acc[3] = {0, 0, 0};
+      for (uint32_t s = 0; s < p->spp; ++s) {
+        fwd_state st = {0.0, {0, 0, 0}};
+        march(p, pp, px, py, s, fwd_point, &st);
+        for (int k = 0; k < 3; ++k) acc[k] += st.C[k];
+      }
+      for (int k = 0; k < 3; ++k) image[3 * ((size_t)py * p->width + px) + k] = acc[k] / p->spp;
+    }
+  free(pp);
+  return 0;
+}
+
+/* ---- adjoint ---- */
+typedef struct {
+  const double *g; /* dLoss/dC of this sample (rgb, already / spp) */
+  /* pass 1 */
+  double L, prev_c[3], total;
+  int have_prev;
+  /* pass 2 */
+  double prefix, Lp, cprev[3], aLprev;
+  double *g_pos, *g_rad, *g_attr, *g_cm, *g_dm, *g_rm;
+} adj_state;
+
+static void pass1(const orc_diff *p, const dpart *pp, const double x[3], const dpay *pl, void *u) {
+  (void)pp;
+  (void)x;
+  adj_state *st = (adj_state *)u;
+  double s, f, dF;
+  uint32_t j0, j1;
+  double D = tfd(p->dm, p->dm_n, 0, pl->rho, &s, &j0, &j1, &f);
+  double c[3];
+  colour(pl, c);
+  if (st->have_prev) { /* dC/dL_{k-1} = F'(L_{k-1}) (c_{k-1} - c_k) */
+    Fcdf(st->L, &dF);
+    double gd = 0;
+    for (int k = 0; k < 3; ++k) gd += st->g[k] * (st->prev_c[k] - c[k]);
+    st->total += dF * gd;
+  }
+  st->L += D;
+  memcpy(st->prev_c, c, sizeof(c));
+  st->have_prev = 1;
+}
+
+static void pass2(const orc_diff *p, const dpart *pp, const double x[3], const dpay *pl, void *u) {
+  adj_state *st = (adj_state *)u;
+  double sD, fD, dF;
+  uint32_t j0, j1;
+  double D = tfd(p->dm, p->dm_n, 0, pl->rho, &sD, &j0, &j1, &fD);
+  double c[3];
+  colour(pl, c);
+  /* the term dC/dL_{j-1} belongs to the prefix once c_j is known */
+  if (st->have_prev) {
+    Fcdf(st->Lp, &dF);
+    double gd = 0;
+    for (int k = 0; k < 3; ++k) gd += st->g[k] * (st->cprev[k] - c[k]);
+    st->prefix += dF * gd;
+  }
+  double aD = st->total - st->prefix; /* dC/dD_j */
+  double F0 = Fcdf(st->Lp, &dF);
+  st->Lp += D;
+  double F1 = Fcdf(st->Lp, &dF);
+  double pj = F1 - F0;
+  /* densitymap texels and slope */
+  st->g_dm[j0] += aD * (1.0 - fD);
+  st->g_dm[j1] += aD * fD;
+  double a_rho = aD * sD, a_S[3];
+  /* colour mean c = (S / rho)^(1/2.2) */
+  for (int k = 0; k < 3; ++k) {
+    double gc = st->g[k] * pj;
+    double m = pl->S[k] / pl->rho;
+    double dcdm = m > 0.0 ? (1.0 / 2.2) * pow(m, 1.0 / 2.2 - 1.0) : 0.0;
+    a_S[k] = gc * dcdm / pl->rho;
+    a_rho -= gc * dcdm * m / pl->rho;
+  }
+  /* covering particles */
+  for (uint32_t i = 0; i < p->n; ++i) {
+    const dpart *q = pp + i;
+    if (!(q->r > 0.0)) continue;
+    double dx = p->pos[3 * i] - x[0], dy = p->pos[3 * i + 1] - x[1], dz = p->pos[3 * i + 2] - x[2];
+    double d = sqrt(dx * dx + dy * dy + dz * dz);
+    if (!(d < q->r)) continue;
+    double qq = 3.0 * d / q->r;
+    double rbf = exp(-0.5 * qq * qq), w = rbf * q->a3;
+    double a_w = a_rho;
+    for (int k = 0; k < 3; ++k) a_w += a_S[k] * q->c22[k];
+    double a_rbf = a_w * q->a3, a_a3 = a_w * rbf;
+    double a_c22[3];
+    for (int k = 0; k < 3; ++k) a_c22[k] = a_S[k] * w;
+    /* rbf = exp(-4.5 d^2 / r^2) */
+    double drbf_dd = -9.0 * d / (q->r * q->r) * rbf, drbf_dr = 9.0 * d * d / (q->r * q->r * q->r) * rbf;
+    if (d > 0.0) {
+      double k = a_rbf * drbf_dd / d;
+      st->g_pos[3 * i] += k * dx;
+      st->g_pos[3 * i + 1] += k * dy;
+      st->g_pos[3 * i + 2] += k * dz;
+    }
+    double a_r = a_rbf * drbf_dr;
+    /* r = radius * radiusmap(attr) */
+    double s, f;
+    uint32_t k0, k1;
+    tfd(p->rm, p->rm_n, 0, p->attr[i], &s, &k0, &k1, &f);
+    st->g_rad[i] += a_r * q->rm;
+    double a_rm = a_r * p->radii[i];
+    st->g_rm[k0] += a_rm * (1.0 - f);
+    st->g_rm[k1] += a_rm * f;
+    st->g_attr[i] += a_rm * s;
+    /* a3 = alpha^3, c22 = c^2.2, from colormap(attr) */
+    double a_alpha = a_a3 * 3.0 * q->alpha * q->alpha;
+    tfd(p->cm, p->cm_n, 3, p->attr[i], &s, &k0, &k1, &f);
+    st->g_cm[4 * k0 + 3] += a_alpha * (1.0 - f);
+    st->g_cm[4 * k1 + 3] += a_alpha * f;
+    st->g_attr[i] += a_alpha * s;
+    for (int c2 = 0; c2 < 3; ++c2) {
+      double a_c = q->c[c2] > 0.0 ? a_c22[c2] * 2.2 * pow(q->c[c2], 1.2) : 0.0;
+      tfd(p->cm, p->cm_n, c2, p->attr[i], &s, &k0, &k1, &f);
+      st->g_cm[4 * k0 + c2] += a_c * (1.0 - f);
+      st->g_cm[4 * k1 + c2] += a_c * f;
+      st->g_attr[i] += a_c * s;
+    }
+  }
+  memcpy(st->cprev, c, sizeof(c));
+  st->have_prev = 1;
+}
+
+/* gimage: dLoss/dimage, W*H*3.  Gradients are accumulated (callers zero
+ * them): g_pos 3N, g_rad N, g_attr N, g_cm 4*cm_n, g_dm dm_n, g_rm rm_n. */
+ORC_EXPORT int orc_diff_adjoint(const orc_diff *p, const double *gimage, double *g_pos, double *g_rad, double *g_attr,
+                                double *g_cm, double *g_dm, double *g_rm) {
+  dpart *pp = parts(p);
+  if (!pp) return -1;
+  for (int py = 0; py < p->height; ++py)
+    for (int px = 0; px < p->width; ++px) {
+      double g[3];
+      for (int k = 0; k < 3; ++k) g[k] = gimage[3 * ((size_t)py * p->width + px) + k] / p->spp;
+      for (uint32_t s = 0; s < p->spp; ++s) {
+        adj_state st;
+        memset(&st, 0, sizeof(st));
+        st.g = g;
+        march(p, pp, px, py, s, pass1, &st);
+        if (st.have_prev) { /* the last point: c_{K+1} = 0 */
+          double dF;
+          Fcdf(st.L, &dF);
+          double gd = 0;
+          for (int k = 0; k < 3; ++k) gd += g[k] * st.prev_c[k];
+          st.total += dF * gd;
+        }
+        st.have_prev = 0;
+        st.g_pos = g_pos;
+        st.g_rad = g_rad;
+        st.g_attr = g_attr;
+        st.g_cm = g_cm;
+        st.g_dm = g_dm;
+        st.g_rm = g_rm;
+        march(p, pp, px, py, s, pass2, &st);
+      }
+    }
+  free(pp);
+  return 0;
+}
