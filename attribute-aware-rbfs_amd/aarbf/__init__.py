@@ -133,6 +133,21 @@ class Context:
         """Schedule parameters of the default renderer (images do not change)."""
         self.check(self.lib.aarbf_set_tuning(self.h, int(param), int(value)), "aarbf_set_tuning")
 
+    PAYLOAD_FP32, PAYLOAD_FP16 = 0, 1
+
+    def set_payload(self, fmt: int):
+        """Particle payload format of the renders (PAYLOAD_FP32 / PAYLOAD_FP16)."""
+        self.check(self.lib.aarbf_set_payload(self.h, int(fmt)), "aarbf_set_payload")
+
+    def payload_read(self, n: int):
+        """Decoded payload of the last render: float32 [n, 8] (x, y, z, r, a^3, rgb^2.2)."""
+        torch = _torch()
+        out = torch.empty((n, 8), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        self.check(self.lib.aarbf_payload_read(self.h, out.data_ptr(), n), "aarbf_payload_read")
+        self.sync()
+        return out.cpu().numpy()
+
     def set_stats(self, enable: bool):
         self.check(self.lib.aarbf_set_stats(self.h, 1 if enable else 0), "aarbf_set_stats")
 

@@ -91,6 +91,8 @@ SIGNATURES = {
     "aarbf_render_stats_read": (C.c_int, [P, C.POINTER(RenderStats), C.c_int]),
     "aarbf_render_histogram": (C.c_int, [P, C.POINTER(u64), C.c_int]),
     "aarbf_set_timing": (C.c_int, [P, C.c_int]),
+    "aarbf_set_payload": (C.c_int, [P, C.c_int]),
+    "aarbf_payload_read": (C.c_int, [P, u64, C.c_uint32]),
     "aarbf_diff_forward": (C.c_int, [P, C.POINTER(RayGenData), C.POINTER(PushConstants), C.c_int, C.c_int,
                                      C.c_uint32, u64, u64]),
     "aarbf_diff_backward": (C.c_int, [P, C.POINTER(RayGenData), C.POINTER(PushConstants), C.c_int, C.c_int,

@@ -88,6 +88,9 @@ struct aarbf_context {
   // baked particle payload
   float4 *baked = nullptr;
   size_t baked_count = 0;
+  int payload = 0;          // 0 FP32 (x, y, z, r) + (alpha^3, rgb^2.2); 1 FP16 leaf-local
+  int baked_fmt = 0;        // format and size of the last bake
+  uint32_t baked_n = 0;
   std::set<uint64_t> tfs, stbns, bvhs;
 };
 

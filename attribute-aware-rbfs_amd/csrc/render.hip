@@ -36,6 +36,82 @@ __global__ __launch_bounds__(256) void bake_kernel(const float4 *__restrict__ pa
 }
 
 
+// FP16 payload: per chunk of 32 particles (Hilbert order, so spatially
+// compact) the centres' bounding box gives an origin and a per-axis step
+// (computed in double, stored as float) for 16-bit fixed-point positions.
+__global__ __launch_bounds__(256) void quant_chunks(const float4 *__restrict__ particles, uint32_t n,
+                                                    float4 *__restrict__ qc) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (n + 31u) >> kQChunkLog2) return;
+  float lo[3] = {AA_INF_F, AA_INF_F, AA_INF_F}, hi[3] = {-AA_INF_F, -AA_INF_F, -AA_INF_F};
+  for (uint32_t i = c << kQChunkLog2; i < min(n, (c + 1u) << kQChunkLog2); ++i) {
+    const float4 q = particles[i];
+    const float v[3] = {q.x, q.y, q.z};
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fminf(lo[d], v[d]);
+      hi[d] = fmaxf(hi[d], v[d]);
+    }
+  }
+  float s[3];
+  for (int d = 0; d < 3; ++d) s[d] = (float)(((double)hi[d] - (double)lo[d]) / 65535.0);
+  qc[2 * c] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+  qc[2 * c + 1] = make_float4(s[0], s[1], s[2], 0.0f);
+}
+
+// Encodes particle i.  Positions round to the nearest grid point of the
+// chunk; the radius is reduced by the rounding shift (and 2^-10 more, then
+// rounded towards zero to binary16), so every decoded sphere lies inside the
+// original one and so inside its leaf box: the BVH built from the FP32
+// bounds stays valid.  alpha (cubed when decoded) and rgb^2.2 round to
+// nearest.  No half is subnormal: a radius or rgb^2.2 below 2^-14 becomes
+// 0, and a positive alpha is at least 2^-14 (a zero alpha would leave points
+// with count > 0 and density 0, whose colour divides 0 by 0).
+__global__ __launch_bounds__(256) void bake_q(const float4 *__restrict__ particles, const float *__restrict__ radii,
+                                              uint32_t n, float rbfRadius, const float4 *__restrict__ rmap,
+                                              uint32_t rm_n, const float4 *__restrict__ cmap, uint32_t cm_n,
+                                              const float4 *__restrict__ qc, uint4 *__restrict__ qp) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 q = particles[i];
+  const float rp = tf_r(rmap, rm_n, q.w);
+  const float radius = (radii ? radii[i] : rbfRadius) * rp;
+  const float4 c = tf4(cmap, cm_n, q.w);
+  const float r = c.w == 0.0f ? 0.0f : radius;
+  const float4 o = qc[2 * (i >> kQChunkLog2)], s = qc[2 * (i >> kQChunkLog2) + 1];
+  const float v[3] = {q.x, q.y, q.z}, ov[3] = {o.x, o.y, o.z}, sv[3] = {s.x, s.y, s.z};
+  uint32_t qa[3];
+  double shift2 = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    double t = sv[d] > 0.0f ? rint(((double)v[d] - (double)ov[d]) / (double)sv[d]) : 0.0;
+    qa[d] = (uint32_t)fmin(fmax(t, 0.0), 65535.0);
+    const float dec = ov[d] + (float)qa[d] * sv[d];
+    shift2 += ((double)dec - v[d]) * ((double)dec - v[d]);
+  }
+  const double rr = ((double)r - sqrt(shift2)) * (1.0 - 1.0 / 1024.0);
+  auto normal = [](uint32_t h) { return h < 0x400u ? 0u : h; };
+  const uint32_t rh = r > 0.0f && rr > 0.0 ? normal(aa_f2h_rz((float)rr)) : 0u;
+  uint32_t ah = aa_f2h_rn(c.w);
+  if (ah < 0x400u && c.w > 0.0f) ah = 0x400u;
+  qp[i] = make_uint4(qa[0] | (qa[1] << 16), qa[2] | (rh << 16), ah | (normal(aa_f2h_rn(aa_powf(c.x, 2.2f))) << 16),
+                     normal(aa_f2h_rn(aa_powf(c.y, 2.2f))) | (normal(aa_f2h_rn(aa_powf(c.z, 2.2f))) << 16));
+}
+
+// The decoded payload (x, y, z, r, alpha^3, rgb^2.2 per particle) of either format.
+__global__ __launch_bounds__(256) void payload_decode(RenderArgs A, float *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.numParticles) return;
+  const float4 a = ld_pr(A, i), b = ld_pc(A, i);
+  float *o = out + 8 * (uint64_t)i;
+  o[0] = a.x;
+  o[1] = a.y;
+  o[2] = a.z;
+  o[3] = a.w;
+  o[4] = b.x;
+  o[5] = b.y;
+  o[6] = b.z;
+  o[7] = b.w;
+}
+
 // counter shared by the primary and the shadow ray.
 template <bool STATS>
 __device__ void track(const RenderArgs &A, bool marching, bool shadow, uint32_t &i, float &t, uint32_t &rng,
@@ -227,7 +303,10 @@ int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg, const 
   }
   hipSetDevice(ctx->device);
   uint32_t n = pc->numParticles;
+  // FP32: 32 B per particle; FP16: 16 B per particle, then 32 B per chunk
   size_t need = 2 * (size_t)(n ? n : 1);
+  const size_t need16 = (size_t)(n ? n : 1) + 2 * (size_t)((n + 31u) >> kQChunkLog2);
+  if (need16 > need) need = need16;
   if (ctx->baked_count < need) {
     if (ctx->baked) {
       hipStreamSynchronize(ctx->stream);
@@ -239,10 +318,22 @@ int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg, const 
     ctx->baked_count = need;
   }
   float4 *pr = ctx->baked, *pcol = ctx->baked + (n ? n : 1);
-  if (n)
+  uint4 *qp = (uint4 *)ctx->baked;
+  float4 *qc = ctx->baked + (n ? n : 1);
+  const bool q16 = ctx->payload == 1;
+  if (n && !q16)
     bake_kernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>((const float4 *)pc->particles, (const float *)pc->radii,
                                                           n, pc->rbfRadius, rm->table, rm->n, cm->table, cm->n, pr,
                                                           pcol);
+  if (n && q16) {
+    const uint32_t nc = (n + 31u) >> kQChunkLog2;
+    quant_chunks<<<(nc + 255) / 256, 256, 0, ctx->stream>>>((const float4 *)pc->particles, n, qc);
+    bake_q<<<(n + 255) / 256, 256, 0, ctx->stream>>>((const float4 *)pc->particles, (const float *)pc->radii, n,
+                                                     pc->rbfRadius, rm->table, rm->n, cm->table, cm->n, qc, qp);
+  }
+  AARBF_CHECK_LAUNCH(ctx, "bake");
+  ctx->baked_fmt = q16 ? 1 : 0;
+  ctx->baked_n = n;
   RenderArgs A{};
   A.frameID = pc->frameID;
   A.accumID = pc->accumID;
@@ -265,6 +356,9 @@ int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg, const 
   A.lb = make_float3(rg->globalAABBMin.x - m2, rg->globalAABBMin.y - m2, rg->globalAABBMin.z - m2);
   A.pr = pr;
   A.pc = pcol;
+  A.qp = qp;
+  A.qc = qc;
+  A.q16 = q16 ? 1u : 0u;
   A.numParticles = n;
   A.ppl = pc->particlesPerLeaf;
   A.cmap = cm->table;
@@ -288,6 +382,29 @@ int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg, const 
     return AARBF_OK;
   }
   return render_wavefront(ctx, A);
+}
+
+int aarbf_set_payload(aarbf_context *ctx, int format) {
+  if (!ctx) return AARBF_ERR_INVALID_ARGUMENT;
+  AARBF_REQUIRE(ctx, format == 0 || format == 1, "aarbf_set_payload: format must be 0 (FP32) or 1 (FP16)");
+  ctx->payload = format;
+  return AARBF_OK;
+}
+
+int aarbf_payload_read(aarbf_context *ctx, uint64_t out, uint32_t n) {
+  if (!ctx) return AARBF_ERR_INVALID_ARGUMENT;
+  AARBF_REQUIRE(ctx, out && ctx->baked && n == ctx->baked_n, "aarbf_payload_read: no payload of that size");
+  RenderArgs A{};
+  const uint32_t m = n ? n : 1;
+  A.pr = ctx->baked;
+  A.pc = ctx->baked + m;
+  A.qp = (const uint4 *)ctx->baked;
+  A.qc = ctx->baked + m;
+  A.q16 = ctx->baked_fmt ? 1u : 0u;
+  A.numParticles = n;
+  if (n) payload_decode<<<(n + 255) / 256, 256, 0, ctx->stream>>>(A, (float *)out);
+  AARBF_CHECK_LAUNCH(ctx, "payload_decode");
+  return AARBF_OK;
 }
 
 int aarbf_render_rbf(aarbf_context *ctx, const aarbf_raygen_data *rg, const aarbf_push_constants *pc, int W,

@@ -21,6 +21,12 @@ struct RenderArgs {
   float3 light, pos, d00, du, dv, lb, rt;
   const float4 *pr; // baked x, y, z, radius (0 = never covers)
   const float4 *pc; // baked alpha^3, rgb^2.2
+  // FP16 payload (q16 != 0): per particle 16 bytes -- x, y, z as 16-bit
+  // fixed point in the frame of its chunk of 32 particles, radius, alpha^3
+  // and rgb^2.2 as binary16; per chunk qc[2c] = origin, qc[2c+1] = step
+  const uint4 *qp;
+  const float4 *qc;
+  uint32_t q16;
   uint32_t numParticles, ppl;
   const float4 *cmap;
   const float4 *dmap;
@@ -34,6 +40,26 @@ struct RenderArgs {
 
 // Default schedule (wavefront.hip): rounds of gather and step kernels.
 int render_wavefront(aarbf_context *ctx, const RenderArgs &A);
+
+constexpr uint32_t kQChunkLog2 = 5; // particles per FP16 quantisation chunk: 32
+
+// binary16 -> binary32 by the hardware conversion (exact; the FP16 payload
+// holds no subnormal halves, so the denormal mode does not matter).
+__device__ inline float h16(uint32_t h) { return (float)__builtin_bit_cast(_Float16, (uint16_t)h); }
+
+// Baked particle p: position and radius, and (alpha^3, rgb^2.2).
+__device__ inline float4 ld_pr(const RenderArgs &A, uint32_t p) {
+  if (!A.q16) return A.pr[p];
+  const uint4 w = A.qp[p];
+  const float4 o = A.qc[2 * (p >> kQChunkLog2)], s = A.qc[2 * (p >> kQChunkLog2) + 1];
+  return make_float4(o.x + (float)(w.x & 0xffffu) * s.x, o.y + (float)(w.x >> 16) * s.y,
+                     o.z + (float)(w.y & 0xffffu) * s.z, h16(w.y >> 16));
+}
+__device__ inline float4 ld_pc(const RenderArgs &A, uint32_t p) {
+  if (!A.q16) return A.pc[p];
+  const uint4 w = A.qp[p];
+  return make_float4(aa_cube(h16(w.z & 0xffffu)), h16(w.z >> 16), h16(w.w & 0xffffu), h16(w.w >> 16));
+}
 
 namespace {
 
@@ -115,11 +141,11 @@ __device__ inline void point_query(const RenderArgs &A, float x, float y, float 
         }
         uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
         for (uint32_t p = p0; p < p1; ++p) {
-          float4 q = A.pr[p];
+          float4 q = ld_pr(A, p);
           if (STATS) c.particles++;
           float d = aa_dist3(q.x, q.y, q.z, x, y, z);
           if (d < q.w) {
-            float4 k = A.pc[p];
+            float4 k = ld_pc(A, p);
             float wa = aa_rbf(d, q.w) * k.x;
             if (STATS) c.covering++;
             pl.count += 1;
@@ -358,7 +384,7 @@ __device__ inline void point_query_q(const RenderArgs &A, uint16_t *stk, float x
   auto particle = [&](uint32_t p, const float4 &q) {
     float d = aa_dist3(q.x, q.y, q.z, x, y, z);
     if (d < q.w) {
-      float4 k = A.pc[p];
+      float4 k = ld_pc(A, p);
       float wa = aa_rbf(d, q.w) * k.x;
       if (STATS) c.covering++;
       pl.count += 1;
@@ -374,7 +400,7 @@ __device__ inline void point_query_q(const RenderArgs &A, uint16_t *stk, float x
       float4 q[8];
 #pragma unroll
       for (uint32_t k = 0; k < 8; ++k)
-        if ((mask >> k) & 1u) q[k] = A.pr[lb + k];
+        if ((mask >> k) & 1u) q[k] = ld_pr(A, lb + k);
 #pragma unroll
       for (uint32_t k = 0; k < 8; ++k)
         if ((mask >> k) & 1u) {
@@ -389,7 +415,7 @@ __device__ inline void point_query_q(const RenderArgs &A, uint16_t *stk, float x
       uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
       for (uint32_t p = p0; p < p1; ++p) {
         if (STATS) c.particles++;
-        particle(p, A.pr[p]);
+        particle(p, ld_pr(A, p));
       }
     }
     return false;
@@ -410,7 +436,7 @@ __device__ inline void point_query_b(const RenderArgs &A, uint16_t *stk, float x
   auto particle = [&](uint32_t p, const float4 &q) {
     float d = aa_dist3(q.x, q.y, q.z, x, y, z);
     if (d < q.w) {
-      float4 k = A.pc[p];
+      float4 k = ld_pc(A, p);
       float wa = aa_rbf(d, q.w) * k.x;
       if (STATS) c.covering++;
       pl.count += 1;
@@ -426,12 +452,12 @@ __device__ inline void point_query_b(const RenderArgs &A, uint16_t *stk, float x
       float4 q[8];
 #pragma unroll
       for (uint32_t k = 0; k < 8; ++k)
-        if (k < m) q[k] = A.pr[lb + k];
+        if (k < m) q[k] = ld_pr(A, lb + k);
       if (STATS) c.particles += m;
 #pragma unroll
       for (uint32_t k = 0; k < 8; ++k)
         if (k < m) particle(lb + k, q[k]);
-      for (uint32_t k = 8; k < m; ++k) particle(lb + k, A.pr[lb + k]);
+      for (uint32_t k = 8; k < m; ++k) particle(lb + k, ld_pr(A, lb + k));
       return false;
     }
     const float4 *L0 = A.bvh.nodes + g_lvl_ofs[0];
@@ -442,7 +468,7 @@ __device__ inline void point_query_b(const RenderArgs &A, uint16_t *stk, float x
       uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
       for (uint32_t p = p0; p < p1; ++p) {
         if (STATS) c.particles++;
-        particle(p, A.pr[p]);
+        particle(p, ld_pr(A, p));
       }
     }
     return false;

@@ -515,7 +515,7 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
         for (uint32_t r = 0; r < 8; ++r) {
           uint32_t nib = (ord >> (4u * r)) & 15u;
           pid[r] = nib ? lb + nib - 1u : 0xffffffffu;
-          if (nib) P[r] = A.pr[pid[r]];
+          if (nib) P[r] = ld_pr(A, pid[r]);
         }
 #pragma unroll
         for (uint32_t r = 0; r < 8; ++r)
@@ -530,7 +530,7 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
         uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
         for (uint32_t p = p0; p < p1; ++p) {
           if (STATS) c.particles++;
-          take(p, A.pr[p]);
+          take(p, ld_pr(A, p));
         }
       }
     };
@@ -804,8 +804,8 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
             m &= m - 1u;
             uint32_t p = li[k * kSBlock];
             // both halves of the payload in one round trip
-            float4 qq = A.pr[p];
-            float4 kk = A.pc[p];
+            float4 qq = ld_pr(A, p);
+            float4 kk = ld_pc(A, p);
             if (STATS) c.particles++;
             float dd = aa_dist3(qq.x, qq.y, qq.z, x, y, z);
             if (dd < qq.w) {
@@ -1011,7 +1011,7 @@ __global__ __launch_bounds__(kSBlock, 4) void wf_step_m(RenderArgs A, WFState S,
         ltin[k * kSBlock] = itin[k * stride + row];
         ltout[k * kSBlock] = itout[k * stride + row];
         leo[k * kSBlock] = ieo[k * stride + row];
-        la3[k * kSBlock] = A.pc[p].x;
+        la3[k * kSBlock] = ld_pc(A, p).x;
       }
       act = 0;
       eptr = 0;
@@ -1061,8 +1061,8 @@ __global__ __launch_bounds__(kSBlock, 4) void wf_step_m(RenderArgs A, WFState S,
           for (uint32_t m = act; m; m &= m - 1u) {
             uint32_t k = (uint32_t)__ffs(m) - 1u;
             uint32_t p = li[k * kSBlock];
-            float4 qq = A.pr[p];
-            float4 kk = A.pc[p];
+            float4 qq = ld_pr(A, p);
+            float4 kk = ld_pc(A, p);
             if (STATS) c.particles++;
             float dd = aa_dist3(qq.x, qq.y, qq.z, x, y, z);
             if (dd < qq.w) {
@@ -1226,10 +1226,10 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
       if (j < total) {
         const uint32_t p = cur[j / ppl] * ppl + j % ppl;
         if (p < A.numParticles) {
-          const float4 q = A.pr[p];
+          const float4 q = ld_pr(A, p);
           const float d = aa_dist3(q.x, q.y, q.z, x, y, z);
           if (d < q.w) {
-            const float4 k = A.pc[p];
+            const float4 k = ld_pc(A, p);
             wa = aa_rbf(d, q.w) * k.x;
             wr = k.y * wa;
             wg = k.z * wa;

@@ -44,6 +44,7 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mode", type=int, default=0, help="0 wavefront gather/step kernels, 1 point queries")
+    ap.add_argument("--payload", type=int, default=0, help="0 FP32 particle payload, 1 FP16 leaf-local quantised")
     ap.add_argument("--step-waves", type=int, default=0)
     ap.add_argument("--step-cap", type=int, default=4096)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
@@ -81,6 +82,7 @@ def main() -> int:
     torch.cuda.set_stream(stream)
     ctx.set_stream(stream)
     ctx.set_render_mode(a.mode)
+    ctx.set_payload(a.payload)
     ctx.set_tuning(ctx.TUNE_STEP_WAVES, a.step_waves)
     ctx.set_tuning(ctx.TUNE_STEP_CAP, a.step_cap)
     torch.cuda.synchronize()
@@ -206,7 +208,8 @@ def main() -> int:
             "workload": f"mixture-{wl.xyzw.shape[0]} delta tracking {W}x{H} {S} spp/frame, TAA, orbit of "
                         f"{wl.frames} frames",
             "particles": int(wl.xyzw.shape[0]), "width": W, "height": H, "spp": S, "particles_per_leaf": 1,
-            "bvh_branching": 8, "unit": wl.unit, "render_mode": a.mode, "max_depth": aarbf.MAX_DEPTH,
+            "bvh_branching": 8, "unit": wl.unit, "render_mode": a.mode,
+            "payload": ("fp32", "fp16 leaf-local quantised")[a.payload], "max_depth": aarbf.MAX_DEPTH,
             "exhausted_fraction": st["exhausted"] / samples, "preprocess_ms": round(prep_s * 1e3, 1),
             "parallelism": "single GPU",
         },

@@ -233,6 +233,32 @@ AA_FN int aa_splat_hit(float cx, float cy, float cz, float r, float ox, float oy
   return ((fx * fx + fy * fy) + fz * fz) < r * r && tmin <= t && t < tmax;
 }
 
+/* IEEE binary16 <-> binary32 for the FP16 particle payload.  aa_h2f is
+ * exact; aa_f2h_rn rounds to nearest even, aa_f2h_rz towards zero; both take
+ * finite non-negative inputs below 65504 (colours, alphas, radii). */
+AA_FN float aa_h2f(uint32_t h) {
+  uint32_t e = (h >> 10) & 31u, m = h & 1023u;
+  if (e == 0u) return (float)m * 5.9604644775390625e-08f; /* m * 2^-24, exact */
+  return aa_u2f(((e + 112u) << 23) | (m << 13));
+}
+AA_FN uint32_t aa_f2h_rn(float x) {
+  uint32_t u = aa_f2u(x);
+  int e = (int)((u >> 23) & 255u) - 127;
+  if (e < -14) { /* subnormal half: round x * 2^24 to an integer */
+    float m = AA_RINTF(x * 16777216.0f);
+    return (uint32_t)m; /* 1024 is the smallest normal */
+  }
+  uint32_t mant = u & 0x7fffffu, h = ((uint32_t)(e + 15) << 10) + (mant >> 13), rem = mant & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h += 1u;
+  return h;
+}
+AA_FN uint32_t aa_f2h_rz(float x) {
+  uint32_t u = aa_f2u(x);
+  int e = (int)((u >> 23) & 255u) - 127;
+  if (e < -14) return (uint32_t)AA_FLOORF(x * 16777216.0f);
+  return ((uint32_t)(e + 15) << 10) + ((u & 0x7fffffu) >> 13);
+}
+
 /* One LCG step (rng.h:52-57) and its [0,1] float (rng.h:59-61): the uint32
  * is converted with round-to-nearest-even, so states >= 0xFFFFFF80 give
  * exactly 1.0f. */

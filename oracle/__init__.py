@@ -35,6 +35,7 @@ class OrcParams(C.Structure):
         ("cm_n", C.c_uint32), ("rm_n", C.c_uint32), ("dm_n", C.c_uint32),
         ("stbn", C.POINTER(C.c_uint8)), ("stbn_w", C.c_int32), ("stbn_h", C.c_int32),
         ("max_depth", C.c_uint32),
+        ("baked", f32p),
     ]
 
 
@@ -217,7 +218,7 @@ class Render:
 
     def __init__(self, xyzw, radii, aabb, colormap, radiusmap, densitymap, camera, width, height, unit,
                  rbf_radius, frame_id=0, accum_id=1, blue_noise=False, stbn=None, show_noise=False,
-                 visualize_attributes=True, light=(0.0, 0.0, 0.5), max_depth=0):
+                 visualize_attributes=True, light=(0.0, 0.0, 0.5), max_depth=0, baked=None):
         self.keep = []
 
         def arr(a, dt):
@@ -251,6 +252,8 @@ class Render:
             p.stbn = s.ctypes.data_as(C.POINTER(C.c_uint8))
             p.stbn_h, p.stbn_w = s.shape[0], s.shape[1]
         p.max_depth = max_depth
+        if baked is not None:  # decoded payload [N, 8] used instead of baking
+            p.baked = _f(arr(np.asarray(baked, np.float32).reshape(-1, 8), np.float32))
         self.p = p
 
     def run(self, pixels=None, spp: int = 1, threads: int = 8, image=None, accum=None):

@@ -337,6 +337,7 @@ typedef struct orc_params {
   const uint8_t *stbn;    /* RGBA8 atlas or NULL */
   int32_t stbn_w, stbn_h;
   uint32_t max_depth;     /* 0 = ORC_MAX_DEPTH */
+  const float *baked;     /* NULL, or 8 floats per particle (x, y, z, r, alpha^3, rgb^2.2) used as is */
 } orc_params;
 
 typedef struct {
@@ -370,6 +371,18 @@ ORC_EXPORT void orc_light_direction(float azimuth, float elevation, float out[3]
 /* Per-particle values of the intersection/any-hit programs that depend
  * only on the particle (deviceCodeRBF.hlsl:350-351,366-376). */
 static void bake(const orc_params *p, baked_particle *bp) {
+  if (p->baked) { /* a payload given in decoded form */
+    for (uint32_t i = 0; i < p->numParticles; ++i) {
+      const float *b = p->baked + 8 * (uint64_t)i;
+      bp[i].x = b[0];
+      bp[i].y = b[1];
+      bp[i].z = b[2];
+      bp[i].r = b[3];
+      bp[i].a3 = b[4];
+      for (int k = 0; k < 3; ++k) bp[i].c22[k] = b[5 + k];
+    }
+    return;
+  }
   for (uint32_t i = 0; i < p->numParticles; ++i) {
     const float *q = p->xyzw + 4 * (uint64_t)i;
     float rp = tf1(p->radiusmap, p->rm_n, q[3], 0);

@@ -1,0 +1,124 @@
+"""FP16 leaf-local payload (aarbf_set_payload(ctx, 1)): the decoded payload
+keeps every sphere inside the original one (so the FP32-built BVH stays
+valid) with the stated precision, the renderer's images equal the oracle's
+bit for bit on the decoded scene, and converged images stay within the
+stated RMSE of the FP64 quadrature of the original scene."""
+import numpy as np
+import pytest
+
+import aarbf
+from aarbf import scenes
+from helpers import default_camera, random_scene
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _render(ctx, sc, cam, W, H, unit, spp=1, frame=0, light=(0.3, 0.2, 0.4), mode=0):
+    import torch
+    img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    acc = torch.zeros_like(img)
+    ctx.set_render_mode(mode)
+    try:
+        ctx.render(sc.raygen_data(img, acc, W, H), sc.push_constants(cam, unit, frame_id=frame, light=light), W, H,
+                   spp)
+        ctx.sync()
+    finally:
+        ctx.set_render_mode(0)
+    return img.cpu().numpy(), acc.cpu().numpy()
+
+
+def _setup(ctx, n=3000, seed=6):
+    xyzw, radii = random_scene(n, seed=seed, extent=1.0, r=(0.03, 0.09))
+    tfs = (scenes.ramp_table(256), scenes.constant_table(n=4), scenes.ramp_table(64))
+    return aarbf.Scene.prepare(ctx, xyzw, *tfs, 0.09, radii=radii), tfs
+
+
+def test_fp16_payload_bounds(gpu_ctx):
+    sc, tfs = _setup(gpu_ctx)
+    W = H = 16
+    cam = default_camera(W, H, look_from=(0.4, 0.3, 2.2))
+    try:
+        _render(gpu_ctx, sc, cam, W, H, 0.004)
+        ref = gpu_ctx.payload_read(sc.n)
+        gpu_ctx.set_payload(gpu_ctx.PAYLOAD_FP16)
+        _render(gpu_ctx, sc, cam, W, H, 0.004)
+        q = gpu_ctx.payload_read(sc.n)
+    finally:
+        gpu_ctx.set_payload(gpu_ctx.PAYLOAD_FP32)
+    shift = np.linalg.norm(q[:, :3].astype(np.float64) - ref[:, :3], axis=1)
+    # chunk steps: extent of 32 consecutive centres (the last chunk ragged) / 65535 per axis
+    bound = np.empty(sc.n)
+    for c0 in range(0, sc.n, 32):
+        c = ref[c0:c0 + 32, :3].astype(np.float64)
+        bound[c0:c0 + 32] = 0.5 * np.linalg.norm((c.max(0) - c.min(0)) / 65535.0) * 1.01 + 1e-7
+    assert sc.n % 32 != 0 and np.all(shift <= bound)
+    live = ref[:, 3] > 0
+    assert np.all(q[live, 3] + shift[live] <= ref[live, 3])          # decoded sphere inside the original
+    assert np.all(q[live, 3] >= (ref[live, 3] - shift[live]) * (1 - 2 ** -10) * (1 - 2 ** -10) - 1e-7)
+    # alpha is stored (alpha^3 = its cube, so 3x the relative error), rgb^2.2 directly
+    # (no subnormal halves: values below 2^-14 flush to 0, alpha is at least 2^-14)
+    np.testing.assert_allclose(q[:, 4], ref[:, 4], rtol=3.01 * 2 ** -11, atol=2.0 ** -42)
+    np.testing.assert_allclose(q[:, 5:], ref[:, 5:], rtol=2 ** -11, atol=2 ** -14)
+    assert np.all((q[:, 4] > 0) == (ref[:, 4] > 0))  # no weight rounds to zero
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fp16_render_matches_oracle_on_decoded_scene(gpu_ctx, mode):
+    sc, tfs = _setup(gpu_ctx)
+    W, H = 48, 40
+    cam = default_camera(W, H, look_from=(0.4, 0.3, 2.2))
+    try:
+        gpu_ctx.set_payload(gpu_ctx.PAYLOAD_FP16)
+        img, acc = _render(gpu_ctx, sc, cam, W, H, 0.004, spp=2, frame=5, mode=mode)
+        q = gpu_ctx.payload_read(sc.n)
+    finally:
+        gpu_ctx.set_payload(gpu_ctx.PAYLOAD_FP32)
+    op = sc.particles.cpu().numpy()
+    r = oracle.Render(op, sc.radii.cpu().numpy(), sc.aabb, *tfs, cam, W, H, 0.004, 0.09, frame_id=5, accum_id=1,
+                      light=(0.3, 0.2, 0.4), baked=q)
+    oimg, oacc, _ = r.run(spp=2, threads=8)
+    assert (oacc[..., :3] > 0).mean() > 0.05
+    assert np.array_equal(acc, oacc) and np.array_equal(img, oimg)
+
+
+def test_fp16_converges_to_quadrature_of_the_fp32_scene(gpu_ctx):
+    W = H = 20
+    xyzw, radii = random_scene(400, seed=31, extent=0.8, r=(0.05, 0.12))
+    tfs = (scenes.ramp_table(256), scenes.constant_table(n=4), scenes.ramp_table(64))
+    sc = aarbf.Scene.prepare(gpu_ctx, xyzw, *tfs, 0.12, radii=radii)
+    cam = default_camera(W, H, look_from=(0.3, 0.2, 1.6))
+    try:
+        gpu_ctx.set_payload(gpu_ctx.PAYLOAD_FP16)
+        _, a = _render(gpu_ctx, sc, cam, W, H, 0.01, spp=2048, frame=0, light=(0.3, 0.2, 1.0))
+        _, b = _render(gpu_ctx, sc, cam, W, H, 0.01, spp=2048, frame=40000, light=(0.3, 0.2, 1.0))
+    finally:
+        gpu_ctx.set_payload(gpu_ctx.PAYLOAD_FP32)
+    e = oracle.Render(sc.particles.cpu().numpy(), sc.radii.cpu().numpy(), sc.aabb, *tfs, cam, W, H, 0.01, 0.12,
+                      light=(0.3, 0.2, 1.0)).expected()
+    a, b = a[..., :3].astype(np.float64), b[..., :3].astype(np.float64)
+    rmse = float(np.sqrt(np.mean((a - e) ** 2)))
+    noise = float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(2.0))
+    print(f"FP16 payload: RMSE vs FP64 quadrature of the FP32 scene {rmse:.2e}, Monte Carlo noise {noise:.2e}")
+    assert np.isfinite(a).all() and np.isfinite(b).all()
+    assert rmse < 1.5 * noise + 1e-3
+
+
+@pytest.mark.parametrize("n", [1, 33])
+def test_fp16_small_and_ragged_scenes(gpu_ctx, n):
+    xyzw, radii = random_scene(n, seed=40 + n, extent=0.3, r=(0.1, 0.2))
+    tfs = (scenes.ramp_table(256), scenes.constant_table(n=4), scenes.ramp_table(64))
+    sc = aarbf.Scene.prepare(gpu_ctx, xyzw, *tfs, 0.2, radii=radii)
+    W, H = 24, 20
+    cam = default_camera(W, H, look_from=(0.2, 0.1, 1.2))
+    try:
+        gpu_ctx.set_payload(gpu_ctx.PAYLOAD_FP16)
+        img, acc = _render(gpu_ctx, sc, cam, W, H, 0.005, spp=4, frame=3)
+        q = gpu_ctx.payload_read(sc.n)
+    finally:
+        gpu_ctx.set_payload(gpu_ctx.PAYLOAD_FP32)
+    r = oracle.Render(sc.particles.cpu().numpy(), sc.radii.cpu().numpy(), sc.aabb, *tfs, cam, W, H, 0.005, 0.2,
+                      frame_id=3, accum_id=1, light=(0.3, 0.2, 0.4), baked=q)
+    oimg, oacc, _ = r.run(spp=4, threads=8)
+    assert np.isfinite(acc).all()
+    assert np.array_equal(acc, oacc) and np.array_equal(img, oimg)
