@@ -285,6 +285,23 @@ class Context:
         self.check(self.lib.aarbf_render_rbf_spp(self.h, C.byref(rg), C.byref(pc), width, height, spp),
                    "aarbf_render_rbf_spp")
 
+    def render_batch(self, requests):
+        """Renders views of one scene in one schedule (aarbf_render_batch);
+        `requests`: sequence of (rg, pc, width, height, spp)."""
+        arr = (_lib.Request * len(requests))()
+        for i, (rg, pc, w, h, spp) in enumerate(requests):
+            arr[i].rg = rg
+            arr[i].pc = pc
+            arr[i].width, arr[i].height, arr[i].spp = int(w), int(h), int(spp)
+        self.check(self.lib.aarbf_render_batch(self.h, arr, len(requests)), "aarbf_render_batch")
+
+    def batch_latency(self, n: int) -> np.ndarray:
+        """Per view of the last batch: ms from its start to the view's last sample."""
+        out = np.empty(n, dtype=np.float64)
+        self.check(self.lib.aarbf_batch_latency(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), n),
+                   "aarbf_batch_latency")
+        return out
+
     def diff_forward(self, rg: RayGenData, pc: PushConstants, width: int, height: int, spp: int, image, path=None):
         """Expected stochastic-marching image into `image` (float32 [H, W, 4]);
         `path` (float32 [H, W, spp, 4]) optionally keeps what the backward needs."""

@@ -68,6 +68,11 @@ class RayGenData(C.Structure):
     ]
 
 
+class Request(C.Structure):
+    """One view of aarbf_render_batch."""
+    _fields_ = [("rg", RayGenData), ("pc", PushConstants), ("width", C.c_int), ("height", C.c_int), ("spp", u32)]
+
+
 class RenderStats(C.Structure):
     _fields_ = [(n, u64) for n in ("samples", "steps", "queries", "nodes", "leaves", "particles",
                                    "covering", "exhausted")]
@@ -126,6 +131,8 @@ SIGNATURES = {
                                        C.c_int, u32]),
     "aarbf_render_splat": (C.c_int, [P, C.POINTER(RayGenData), C.POINTER(PushConstants), C.c_int,
                                      C.c_int]),
+    "aarbf_render_batch": (C.c_int, [P, C.POINTER(Request), u32]),
+    "aarbf_batch_latency": (C.c_int, [P, C.POINTER(C.c_double), u32]),
     "aarbf_taa_resolve": (C.c_int, [P, C.POINTER(TAAConstants)]),
     "aarbf_abi_layout": (u32, [C.POINTER(u32), u32]),
 }
@@ -141,6 +148,7 @@ def _expected_layout():
         C.sizeof(TAAConstants), TAAConstants.imageTexture.offset,
         C.sizeof(RayGenData), RayGenData.fbSize.offset, RayGenData.globalAABBMax.offset,
         C.sizeof(RenderStats),
+        C.sizeof(Request), Request.pc.offset, Request.spp.offset,
     ]
 
 

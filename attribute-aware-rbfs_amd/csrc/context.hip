@@ -125,6 +125,7 @@ int aarbf_context_destroy(aarbf_context *ctx) {
   }
   if (ctx->scratch) hipFree(ctx->scratch);
   if (ctx->baked) hipFree(ctx->baked);
+  if (ctx->lat_buf) hipFree(ctx->lat_buf);
   if (ctx->d_stats) hipFree(ctx->d_stats);
   if (ctx->d_counter) hipFree(ctx->d_counter);
   for (hipEvent_t &e : ctx->tev)
@@ -204,6 +205,24 @@ int aarbf_set_timing(aarbf_context *ctx, int enable) {
     for (hipEvent_t &e : ctx->tev)
       if (!e) AARBF_HIP(ctx, hipEventCreate(&e));
   ctx->timing = enable != 0;
+  return AARBF_OK;
+}
+
+int aarbf_batch_latency(aarbf_context *ctx, double *ms, uint32_t n) {
+  if (!ctx || !ms) return AARBF_ERR_INVALID_ARGUMENT;
+  AARBF_REQUIRE(ctx, ctx->lat_n > 0 && n == ctx->lat_n, "aarbf_batch_latency: n differs from the last batch");
+  std::vector<unsigned long long> h(2 * (size_t)n + 1);
+  AARBF_HIP(ctx, hipMemcpyAsync(h.data(), ctx->lat_buf, sizeof(unsigned long long) * h.size(),
+                                hipMemcpyDeviceToHost, ctx->stream));
+  AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  int khz = 0;
+  AARBF_HIP(ctx, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+  AARBF_REQUIRE(ctx, khz > 0, "aarbf_batch_latency: no wall clock rate");
+  const unsigned long long t0 = h[2 * (size_t)n];
+  for (uint32_t i = 0; i < n; ++i) {
+    const unsigned long long t = h[2 * (size_t)i + 1];
+    ms[i] = t >= t0 && t ? (double)(t - t0) / (double)khz : -1.0;
+  }
   return AARBF_OK;
 }
 
@@ -332,6 +351,9 @@ uint32_t aarbf_abi_layout(uint32_t *out, uint32_t n) {
       (uint32_t)offsetof(aarbf_raygen_data, fbSize),
       (uint32_t)offsetof(aarbf_raygen_data, globalAABBMax),
       (uint32_t)sizeof(aarbf_render_stats),
+      (uint32_t)sizeof(aarbf_request),
+      (uint32_t)offsetof(aarbf_request, pc),
+      (uint32_t)offsetof(aarbf_request, spp),
   };
   uint32_t m = (uint32_t)(sizeof(v) / sizeof(v[0]));
   for (uint32_t i = 0; i < n && i < m; ++i) out[i] = v[i];

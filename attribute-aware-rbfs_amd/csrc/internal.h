@@ -91,6 +91,10 @@ struct aarbf_context {
   int payload = 0;          // 0 FP32 (x, y, z, r) + (alpha^3, rgb^2.2); 1 FP16 leaf-local
   int baked_fmt = 0;        // format and size of the last bake
   uint32_t baked_n = 0;
+  // per-view latency record of the last aarbf_render_batch: 2 words per view
+  // (samples finished, wall clock at the last) and the clock at the start
+  unsigned long long *lat_buf = nullptr;
+  uint32_t lat_cap = 0, lat_n = 0;
   std::set<uint64_t> tfs, stbns, bvhs;
 };
 

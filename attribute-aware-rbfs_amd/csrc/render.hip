@@ -15,6 +15,9 @@
 // definitions bit for bit.
 #include "render_common.h"
 
+#include <cstring>
+#include <vector>
+
 namespace aarbf {
 namespace {
 
@@ -280,11 +283,37 @@ static float3 light_direction(float azimuth, float elevation) {
   return make_float3(sinf(a) * cosf(b), sinf(a) * sinf(b), cosf(a));
 }
 
-extern "C" {
+// Per-view fields of a render call (camera, counters, light, buffers).
+static ReqArgs view_args(const aarbf_raygen_data *rg, const aarbf_push_constants *pc, int W, int H, uint32_t spp,
+                         const STBN *stbn) {
+  ReqArgs R{};
+  R.pos = make_float3(pc->camera.pos.x, pc->camera.pos.y, pc->camera.pos.z);
+  R.d00 = make_float3(pc->camera.dir_00.x, pc->camera.dir_00.y, pc->camera.dir_00.z);
+  R.du = make_float3(pc->camera.dir_du.x, pc->camera.dir_du.y, pc->camera.dir_du.z);
+  R.dv = make_float3(pc->camera.dir_dv.x, pc->camera.dir_dv.y, pc->camera.dir_dv.z);
+  R.light = light_direction(pc->light.azimuth, pc->light.elevation);
+  R.amb = pc->light.ambient;
+  R.W = W;
+  R.H = H;
+  R.tiles_x = (uint32_t)(W + 7) / 8;
+  R.frameID = pc->frameID;
+  R.accumID = pc->accumID;
+  R.spp = spp;
+  R.blue = pc->enableBlueNoise ? 1u : 0u;
+  R.showNoise = pc->showNoise ? 1u : 0u;
+  R.stbn = stbn ? stbn->rgba : nullptr;
+  R.stbn_w = stbn ? stbn->width : 0;
+  R.stbn_h = stbn ? stbn->height : 0;
+  R.image = (float4 *)rg->imageBuffer;
+  R.accum = (float4 *)rg->accumBuffer;
+  R.samples = (uint64_t)W * (uint64_t)H * spp;
+  return R;
+}
 
-int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg, const aarbf_push_constants *pc, int W,
-                         int H, uint32_t spp) {
-  if (!ctx) return AARBF_ERR_INVALID_ARGUMENT;
+// Validates a render call, bakes the particle payload and fills the
+// kernel arguments.  *stbn_out: the call's blue-noise atlas (or null).
+static int render_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const aarbf_push_constants *pc, int W,
+                        int H, uint32_t spp, RenderArgs &A, BVH *&bvh_out, STBN *&stbn_out) {
   AARBF_REQUIRE(ctx, rg && pc, "aarbf_render_rbf: null struct");
   AARBF_REQUIRE(ctx, W > 0 && H > 0 && spp >= 1, "aarbf_render_rbf: empty launch");
   AARBF_REQUIRE(ctx, rg->imageBuffer && rg->accumBuffer, "aarbf_render_rbf: null image/accum buffer");
@@ -301,6 +330,8 @@ int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg, const 
     stbn = get_stbn(ctx, rg->stbnTexture);
     AARBF_REQUIRE(ctx, stbn, "aarbf_render_rbf: blue noise enabled without an STBN atlas");
   }
+  bvh_out = bvh;
+  stbn_out = stbn;
   hipSetDevice(ctx->device);
   uint32_t n = pc->numParticles;
   // FP32: 32 B per particle; FP16: 16 B per particle, then 32 B per chunk
@@ -334,7 +365,7 @@ int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg, const 
   AARBF_CHECK_LAUNCH(ctx, "bake");
   ctx->baked_fmt = q16 ? 1 : 0;
   ctx->baked_n = n;
-  RenderArgs A{};
+  A = RenderArgs{};
   A.frameID = pc->frameID;
   A.accumID = pc->accumID;
   A.spp = spp;
@@ -372,7 +403,23 @@ int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg, const 
   A.accum = (float4 *)rg->accumBuffer;
   A.stats = ctx->d_stats;
   A.bvh = make_view(*bvh);
-  if (ctx->render_mode == 1 || bvh->num_levels > kStackLevels || !bvh->groups) {
+  return AARBF_OK;
+}
+
+// The wavefront schedule needs the quantised tree within its LDS stack.
+static bool wavefront_ok(const BVH *bvh) { return bvh->num_levels <= kStackLevels && bvh->groups; }
+
+extern "C" {
+
+int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg, const aarbf_push_constants *pc, int W,
+                         int H, uint32_t spp) {
+  if (!ctx) return AARBF_ERR_INVALID_ARGUMENT;
+  RenderArgs A;
+  BVH *bvh = nullptr;
+  STBN *stbn = nullptr;
+  int rc = render_setup(ctx, rg, pc, W, H, spp, A, bvh, stbn);
+  if (rc != AARBF_OK) return rc;
+  if (ctx->render_mode == 1 || !wavefront_ok(bvh)) {
     dim3 grid((W + 15) / 16, (H + 15) / 16);
     if (ctx->stats)
       render_kernel<true><<<grid, 256, 0, ctx->stream>>>(A);
@@ -381,7 +428,52 @@ int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg, const 
     AARBF_CHECK_LAUNCH(ctx, "render_kernel");
     return AARBF_OK;
   }
-  return render_wavefront(ctx, A);
+  const ReqArgs R = view_args(rg, pc, W, H, spp, stbn);
+  return render_wavefront(ctx, A, &R, 1, false);
+}
+
+int aarbf_render_batch(aarbf_context *ctx, const aarbf_request *reqs, uint32_t n) {
+  if (!ctx) return AARBF_ERR_INVALID_ARGUMENT;
+  AARBF_REQUIRE(ctx, reqs && n >= 1, "aarbf_render_batch: no requests");
+  const aarbf_push_constants &p0 = reqs[0].pc;
+  const aarbf_raygen_data &g0 = reqs[0].rg;
+  uint32_t max_spp = 0;
+  std::vector<ReqArgs> R(n);
+  std::vector<STBN *> atlas(n, nullptr);
+  for (uint32_t i = 0; i < n; ++i) {
+    const aarbf_request &q = reqs[i];
+    AARBF_REQUIRE(ctx, q.width > 0 && q.height > 0 && q.spp >= 1, "aarbf_render_batch: empty request");
+    AARBF_REQUIRE(ctx, q.rg.imageBuffer && q.rg.accumBuffer, "aarbf_render_batch: null image/accum buffer");
+    const aarbf_push_constants &p = q.pc;
+    // the scene is shared: every field that is not per view must match request 0
+    AARBF_REQUIRE(ctx,
+                  p.world == p0.world && p.particles == p0.particles && p.numParticles == p0.numParticles &&
+                      p.particlesPerLeaf == p0.particlesPerLeaf && p.colormap == p0.colormap &&
+                      p.radiusmap == p0.radiusmap && p.densitymap == p0.densitymap && p.radii == p0.radii &&
+                      p.rbfRadius == p0.rbfRadius && p.unit == p0.unit &&
+                      p.visualizeAttributes == p0.visualizeAttributes &&
+                      !memcmp(&q.rg.globalAABBMin, &g0.globalAABBMin, sizeof(g0.globalAABBMin)) &&
+                      !memcmp(&q.rg.globalAABBMax, &g0.globalAABBMax, sizeof(g0.globalAABBMax)),
+                  "aarbf_render_batch: requests must share the scene (world, particles, radii, transfer "
+                  "functions, rbfRadius, unit, visualizeAttributes, global AABB)");
+    if (p.enableBlueNoise) {
+      atlas[i] = get_stbn(ctx, q.rg.stbnTexture);
+      AARBF_REQUIRE(ctx, atlas[i], "aarbf_render_batch: blue noise enabled without an STBN atlas");
+    }
+    max_spp = std::max(max_spp, q.spp);
+  }
+  RenderArgs A;
+  BVH *bvh = nullptr;
+  STBN *stbn = nullptr;
+  int rc = render_setup(ctx, &g0, &p0, reqs[0].width, reqs[0].height, max_spp, A, bvh, stbn);
+  if (rc != AARBF_OK) return rc;
+  AARBF_REQUIRE(ctx, wavefront_ok(bvh), "aarbf_render_batch: the BVH is too deep for the wavefront schedule");
+  A.blue = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    R[i] = view_args(&reqs[i].rg, &reqs[i].pc, reqs[i].width, reqs[i].height, reqs[i].spp, atlas[i]);
+    A.blue |= R[i].blue;
+  }
+  return render_wavefront(ctx, A, R.data(), n, true);
 }
 
 int aarbf_set_payload(aarbf_context *ctx, int format) {

@@ -11,6 +11,23 @@
 
 namespace aarbf {
 
+// One view of a render call (the wavefront kernels read these from a
+// device table, so a batch of views of uneven size shares one schedule).
+// Its pixels are the tiles [tile0, tile0 + tiles_x * tiles_y) of the
+// combined 8x8-tile pixel space.
+struct ReqArgs {
+  float3 pos, d00, du, dv, light;
+  float amb;
+  int W, H;
+  uint32_t tiles_x, tile0;
+  uint32_t frameID, accumID, spp;
+  uint32_t blue, showNoise;
+  const uint8_t *stbn;
+  int stbn_w, stbn_h;
+  float4 *image, *accum;
+  uint64_t samples; // W * H * spp (completion count of the latency record)
+};
+
 // Kernel arguments of the RBF render kernels (one launch).
 struct RenderArgs {
   uint32_t frameID, accumID, spp;
@@ -36,10 +53,19 @@ struct RenderArgs {
   float4 *image, *accum;
   uint64_t *stats;
   BVHView bvh;
+  // wavefront schedule: the views of a batch, the view of every tile, and
+  // per view (samples finished, wall clock at the last) with [2 * nreq] =
+  // the wall clock at the start, or null
+  ReqArgs v0; // the view of a single-view call
+  const ReqArgs *req;
+  const uint32_t *tile_req;
+  uint32_t nreq;
+  unsigned long long *lat;
 };
 
 // Default schedule (wavefront.hip): rounds of gather and step kernels.
-int render_wavefront(aarbf_context *ctx, const RenderArgs &A);
+int render_wavefront(aarbf_context *ctx, const RenderArgs &A, const ReqArgs *views, uint32_t nviews,
+                     bool batch);
 
 constexpr uint32_t kQChunkLog2 = 5; // particles per FP16 quantisation chunk: 32
 

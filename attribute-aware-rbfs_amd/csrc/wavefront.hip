@@ -57,6 +57,8 @@ constexpr uint32_t F_SHADOW = 1u, F_PENDING = 2u, F_DENSE = 4u;
 // of all its samples, which follow the same line); F_HEADGATHER: the next
 // gather builds that head list from the ray's first possible position.
 constexpr uint32_t F_HEAD = 8u, F_HEADGATHER = 16u;
+// F_MARCH: the ray's view uses stochastic marching (else delta tracking)
+constexpr uint32_t F_MARCH = 32u;
 
 // Ray-slot state, structure of arrays (one entry per slot).
 struct WFState {
@@ -79,8 +81,7 @@ struct WFState {
   uint32_t *next_s; // [npix] next unclaimed sample of each pixel
   uint32_t *visit;  // next pixel for slots that help
   uint32_t nslots; // == npix: slot k owns pixel k
-  uint32_t npix;  // pixels in tile order (8x8 tiles, padded)
-  uint32_t tiles_x;
+  uint32_t npix;  // pixels in tile order (8x8 tiles of each view, padded)
   uint32_t s0, s1; // samples of this pass
 };
 
@@ -129,36 +130,58 @@ __device__ inline void save_lane(const WFState &S, uint32_t slot, const Lane &L)
   S.seg[slot] = make_float4(L.seg_t0, L.seg_end, L.rx, 0.0f);
 }
 
-__device__ inline void pix_xy(const WFState &S, uint32_t pix, int &px, int &py) {
-  uint32_t tile = pix >> 6, w = pix & 63u;
-  px = (int)((tile % S.tiles_x) * 8u + (w & 7u));
-  py = (int)((tile / S.tiles_x) * 8u + (w >> 3));
+// The view a pixel of the combined pixel space belongs to: with B (a
+// batch) looked up in the device table, else the call's one view, read from
+// the kernel arguments.
+template <bool B>
+__device__ inline const ReqArgs &view(const RenderArgs &A, uint32_t pix) {
+  if (B) return A.req[A.tile_req[pix >> 6]];
+  return A.v0;
+}
+
+// Stochastic marching (else delta tracking) for the ray's view.
+template <bool B>
+__device__ inline bool is_march(const RenderArgs &A, const Lane &L) {
+  return B ? (L.flags & F_MARCH) != 0u : A.v0.blue != 0u;
+}
+
+// Pixel coordinates within its view.
+__device__ inline void pix_xy(const ReqArgs &R, uint32_t pix, int &px, int &py) {
+  uint32_t tile = (pix >> 6) - R.tile0, w = pix & 63u;
+  px = (int)((tile % R.tiles_x) * 8u + (w & 7u));
+  py = (int)((tile / R.tiles_x) * 8u + (w >> 3));
+}
+
+// One past the last sample of this pass that pixel pix has to render.
+template <bool B>
+__device__ inline uint32_t sample_end(const RenderArgs &A, const WFState &S, uint32_t pix) {
+  return B ? min(view<B>(A, pix).spp, S.s1) : S.s1;
 }
 
 // Set-up of sample L.s (deviceCodeRBF.hlsl:207-280).  Returns true when the
 // primary ray enters the clip box and must be tracked.
-__device__ inline bool start_sample(const RenderArgs &A, int px, int py, Lane &L) {
-  const bool marching = A.blue != 0;
-  uint32_t fid = (A.frameID + L.s) & 0xffffu;
-  uint32_t rng = aa_rng_seed(fid, (uint32_t)px, (uint32_t)py, (uint32_t)A.W);
-  float sx = ((float)px + 0.5f) / (float)A.W;
-  float sy = ((float)py + 0.5f) / (float)A.H;
+__device__ inline bool start_sample(const RenderArgs &A, const ReqArgs &R, int px, int py, Lane &L) {
+  const bool marching = R.blue != 0;
+  uint32_t fid = (R.frameID + L.s) & 0xffffu;
+  uint32_t rng = aa_rng_seed(fid, (uint32_t)px, (uint32_t)py, (uint32_t)R.W);
+  float sx = ((float)px + 0.5f) / (float)R.W;
+  float sy = ((float)py + 0.5f) / (float)R.H;
   float3 dir;
-  dir.x = (A.d00.x + sx * A.du.x) + sy * A.dv.x;
-  dir.y = (A.d00.y + sx * A.du.y) + sy * A.dv.y;
-  dir.z = (A.d00.z + sx * A.du.z) + sy * A.dv.z;
+  dir.x = (R.d00.x + sx * R.du.x) + sy * R.dv.x;
+  dir.y = (R.d00.y + sx * R.du.y) + sy * R.dv.y;
+  dir.z = (R.d00.z + sx * R.du.z) + sy * R.dv.z;
   float len = sqrtf((dir.x * dir.x + dir.y * dir.y) + dir.z * dir.z);
   dir.x = dir.x / len;
   dir.y = dir.y / len;
   dir.z = dir.z / len;
   float tenter, texit;
-  aabb_intersection(A.pos, dir, 0.0f, 10000.0f, A.lb, A.rt, tenter, texit);
+  aabb_intersection(R.pos, dir, 0.0f, 10000.0f, A.lb, A.rt, tenter, texit);
   float r0;
   if (marching) {
     int gx = (int)(fid % 8u), gy = (int)((fid / 8u) % 16u);
     int cx = gx * 256 + px % 256, cy = gy * 256 + py % 256;
     uint8_t v = 0;
-    if (cx < A.stbn_w && cy < A.stbn_h) v = A.stbn[4 * ((uint64_t)cy * A.stbn_w + cx)];
+    if (cx < R.stbn_w && cy < R.stbn_h) v = R.stbn[4 * ((uint64_t)cy * R.stbn_w + cx)];
     r0 = (float)v / 255.0f;
   } else {
     r0 = aa_lcg_next(&rng);
@@ -170,9 +193,9 @@ __device__ inline bool start_sample(const RenderArgs &A, int px, int py, Lane &L
   L.i = 0;
   L.alb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   L.shw = 0.0f;
-  L.flags = 0;
+  L.flags = marching ? F_MARCH : 0u;
   if (!(tenter < texit)) return false;
-  L.o = A.pos;
+  L.o = R.pos;
   L.d = dir;
   L.tmax = texit;
   L.t = marching ? tenter + A.unit * r1 : 0.0f;
@@ -185,14 +208,32 @@ __device__ inline bool start_sample(const RenderArgs &A, int px, int py, Lane &L
 // Shading and compositing over black (':316-324'); the colour goes to the
 // sample's entry of the colour buffer, the running mean (':325-328') is
 // formed by wf_fold.
-template <bool STATS>
+template <bool STATS, bool B>
 __device__ inline void finish_sample(const RenderArgs &A, const WFState &S, Lane &L, float vis, Counters &c) {
+  const float amb = view<B>(A, L.pix).amb;
   float4 color = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (L.alb.w == 1.0f) {
-    color.x = (L.alb.x * vis) * (1.0f - A.amb) + L.alb.x * A.amb;
-    color.y = (L.alb.y * vis) * (1.0f - A.amb) + L.alb.y * A.amb;
-    color.z = (L.alb.z * vis) * (1.0f - A.amb) + L.alb.z * A.amb;
+    color.x = (L.alb.x * vis) * (1.0f - amb) + L.alb.x * amb;
+    color.y = (L.alb.y * vis) * (1.0f - amb) + L.alb.y * amb;
+    color.z = (L.alb.z * vis) * (1.0f - amb) + L.alb.z * amb;
     color.w = 1.0f;
+  }
+  if (B && A.lat) {
+    // finished samples per view, one atomic per view present among the
+    // active lanes; the sample that completes its view stamps the clock
+    const uint32_t r = A.tile_req[L.pix >> 6];
+    bool todo = true;
+    const int lane = threadIdx.x & 63;
+    for (uint64_t m = __ballot(todo); m; m = __ballot(todo)) {
+      const int leader = __ffsll((unsigned long long)m) - 1;
+      const uint32_t rl = __shfl(r, leader, 64);
+      const uint64_t mine = __ballot(todo && r == rl);
+      if (lane == leader) {
+        const unsigned long long k = (unsigned long long)__popcll(mine);
+        if (atomicAdd(&A.lat[2 * rl], k) + k == A.req[rl].samples) A.lat[2 * rl + 1] = wall_clock64();
+      }
+      if (r == rl) todo = false;
+    }
   }
   if (STATS) {
     c.samples++;
@@ -218,13 +259,14 @@ constexpr uint32_t kNoPix = 0xffffffffu;
 // claimed exactly once (atomic per-pixel counters).  Returns false when
 // the slot finds no work (all pixels visited, or after kProbes pixels
 // without samples left: the owners finish them).
-__device__ inline bool take_item(const WFState &S, Lane &L) {
+template <bool B>
+__device__ inline bool take_item(const RenderArgs &A, const WFState &S, Lane &L) {
   constexpr int kProbes = 32;
   uint32_t p = L.pix;
   for (int probe = 0; probe < kProbes; ++probe) {
     if (p != kNoPix) {
       uint32_t s = atomicAdd(&S.next_s[p], 1u);
-      if (s < S.s1) {
+      if (s < sample_end<B>(A, S, p)) {
         L.pix = p;
         L.s = s;
         return true;
@@ -240,7 +282,7 @@ __device__ inline bool take_item(const WFState &S, Lane &L) {
     uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     uint32_t q = base + (uint32_t)__popcll(mask & lt);
     if (q >= S.npix) break;
-    p = __hip_atomic_load(&S.next_s[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < S.s1 ? q : kNoPix;
+    p = __hip_atomic_load(&S.next_s[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < sample_end<B>(A, S, q) ? q : kNoPix;
   }
   L.pix = kNoPix;
   return false;
@@ -251,13 +293,14 @@ __device__ inline bool take_item(const WFState &S, Lane &L) {
 // built), 0 when the slot has no work left.  The head list is built by the
 // pixel's sample 0; a later sample that starts before it is ready gathers
 // its own list.
-template <bool STATS>
+template <bool STATS, bool B>
 __device__ inline int next_ray(const RenderArgs &A, const WFState &S, Lane &L, Counters &c) {
   for (;;) {
-    if (!take_item(S, L)) return 0;
+    if (!take_item<B>(A, S, L)) return 0;
+    const ReqArgs &R = view<B>(A, L.pix);
     int px, py;
-    pix_xy(S, L.pix, px, py);
-    if (start_sample(A, px, py, L)) {
+    pix_xy(R, L.pix, px, py);
+    if (start_sample(A, R, px, py, L)) {
       float4 h = S.head[L.pix];
       uint32_t hf = __float_as_uint(h.w);
       if (hf & 1u) {
@@ -271,7 +314,7 @@ __device__ inline int next_ray(const RenderArgs &A, const WFState &S, Lane &L, C
       L.seg_end = -AA_INF_F;
       return 2;
     }
-    finish_sample<STATS>(A, S, L, 1.0f, c);
+    finish_sample<STATS, B>(A, S, L, 1.0f, c);
   }
 }
 
@@ -304,18 +347,20 @@ __global__ __launch_bounds__(256) void wf_head_reset(WFState S) {
   if (p < S.npix) S.head[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// Per pass: every pixel on the image has samples s0..s1-1 to claim, the
-// padding pixels of partial tiles none.
+// Per pass: every pixel on its view's image has samples s0..min(spp, s1)-1
+// to claim, the padding pixels of partial tiles none.
+template <bool B>
 __global__ __launch_bounds__(256) void wf_pass_reset(RenderArgs A, WFState S) {
   uint32_t p = blockIdx.x * 256u + threadIdx.x;
   if (p >= S.npix) return;
+  const ReqArgs &R = view<B>(A, p);
   int px, py;
-  pix_xy(S, p, px, py);
-  S.next_s[p] = (px < A.W && py < A.H) ? S.s0 : S.s1;
+  pix_xy(R, p, px, py);
+  S.next_s[p] = (px < R.W && py < R.H && S.s0 < R.spp) ? S.s0 : S.s1;
 }
 
 // Slot k starts on its own pixel k.
-template <bool STATS>
+template <bool STATS, bool B>
 __global__ __launch_bounds__(256) void wf_init(RenderArgs A, WFState S, uint32_t *listA, uint32_t *listG,
                                                uint32_t *counts) {
   uint32_t slot = blockIdx.x * 256u + threadIdx.x;
@@ -326,7 +371,7 @@ __global__ __launch_bounds__(256) void wf_init(RenderArgs A, WFState S, uint32_t
     L.pix = slot;
     L.seg_n = 0;
     L.seg_t0 = 0.0f;
-    int r = next_ray<STATS>(A, S, L, c);
+    int r = next_ray<STATS, B>(A, S, L, c);
     if (r) {
       save_lane(S, slot, L);
       push_a = true;
@@ -340,19 +385,22 @@ __global__ __launch_bounds__(256) void wf_init(RenderArgs A, WFState S, uint32_t
 
 // Running mean of every pixel over the pass's samples, in sample order
 // (deviceCodeRBF.hlsl:325-335).
+template <bool B>
 __global__ __launch_bounds__(256) void wf_fold(RenderArgs A, WFState S) {
   uint32_t p = blockIdx.x * 256u + threadIdx.x;
   if (p >= S.npix) return;
+  const ReqArgs &R = view<B>(A, p);
   int px, py;
-  pix_xy(S, p, px, py);
-  if (px >= A.W || py >= A.H) return;
-  uint64_t ofs = (uint64_t)px + (uint64_t)A.W * py;
-  float4 acc = A.accum[ofs];
+  pix_xy(R, p, px, py);
+  if (px >= R.W || py >= R.H || S.s0 >= R.spp) return;
+  uint64_t ofs = (uint64_t)px + (uint64_t)R.W * py;
+  float4 acc = R.accum[ofs];
   float rx = 0.0f;
-  for (uint32_t s = S.s0; s < S.s1; ++s) {
+  const uint32_t s1 = min(R.spp, S.s1);
+  for (uint32_t s = S.s0; s < s1; ++s) {
     float4 col = S.col[(uint64_t)(s - S.s0) * S.npix + p];
     rx = col.w;
-    uint32_t aid = (A.accumID + s) & 0xffffu;
+    uint32_t aid = (R.accumID + s) & 0xffffu;
     float w1 = 1.0f / (float)aid;
     float w0 = (float)((int)aid - 1) / (float)aid;
     float4 fin;
@@ -362,8 +410,8 @@ __global__ __launch_bounds__(256) void wf_fold(RenderArgs A, WFState S) {
     fin.w = w1 * 1.0f + w0 * acc.w;
     acc = fin;
   }
-  A.accum[ofs] = acc;
-  A.image[ofs] = A.showNoise ? make_float4(rx, rx, rx, 1.0f) : acc;
+  R.accum[ofs] = acc;
+  R.image[ofs] = R.showNoise ? make_float4(rx, rx, rx, 1.0f) : acc;
 }
 
 // ---- gather --------------------------------------------------------------------
@@ -628,7 +676,7 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
 // segment are handed to the dense queue untouched.  DENSE = true: the dense
 // queue, stepped with point queries over the quantised tree.  Keeping the
 // two apart keeps the BVH walk's registers out of the common kernel.
-template <bool STATS, bool DENSE, int WPE>
+template <bool STATS, bool DENSE, int WPE, bool B>
 __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S, const uint32_t *__restrict__ listA,
                                                    const uint32_t *__restrict__ countA, uint32_t *__restrict__ outA,
                                                    uint32_t *__restrict__ outG, uint32_t *__restrict__ counts,
@@ -655,7 +703,6 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
     aa_tf_coords(u, A.dm_n, &j0, &j1, &f);
     return aa_lerp_tex(s_dm[j0], s_dm[j1], f);
   };
-  const bool marching = A.blue != 0;
   const uint32_t n_a = *countA;
   uint32_t gid = blockIdx.x * kSBlock + threadIdx.x;
   Counters c = {};
@@ -748,7 +795,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
       // taken.  Marching steps a fixed distance and draws nothing.
       uint32_t ns = L.trng;
       float tn;
-      if (marching) {
+      if (is_march<B>(A, L)) {
         tn = L.t + A.unit;
       } else {
         float xi = aa_lcg_next(&ns);
@@ -822,7 +869,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
         }
         L.flags &= ~F_PENDING;
         const bool shadow = (L.flags & F_SHADOW) != 0;
-        if (marching) {
+        if (is_march<B>(A, L)) {
           if (pl.count > 0 && pl.dens > 0.0f) {
             float D = density(pl.dens);
             L.LHS += D;
@@ -858,25 +905,26 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
       }
       // ---- the ray ended
       L.flags &= ~F_PENDING;
-      if (!(L.flags & F_SHADOW) && L.alb.w > 0.0f && A.amb != 1.0f) {
+      const ReqArgs &R = view<B>(A, L.pix);
+      if (!(L.flags & F_SHADOW) && L.alb.w > 0.0f && R.amb != 1.0f) {
         // NEE shadow ray (':288-311'); the step counter carries over
-        float3 so = make_float3(A.pos.x + L.d.x * L.t, A.pos.y + L.d.y * L.t, A.pos.z + L.d.z * L.t);
+        float3 so = make_float3(R.pos.x + L.d.x * L.t, R.pos.y + L.d.y * L.t, R.pos.z + L.d.z * L.t);
         float ste, stx;
-        aabb_intersection(so, A.light, 0.0f, 10000.0f, A.lb, A.rt, ste, stx);
+        aabb_intersection(so, R.light, 0.0f, 10000.0f, A.lb, A.rt, ste, stx);
         L.o = so;
-        L.d = A.light;
+        L.d = R.light;
         L.tmax = stx;
-        L.flags = F_SHADOW;
+        L.flags = F_SHADOW | (L.flags & F_MARCH);
         L.shw = 0.0f;
         L.LHS = 0.0f;
-        L.t = marching ? -A.unit * L.r2 : 0.0f;
+        L.t = is_march<B>(A, L) ? -A.unit * L.r2 : 0.0f;
         L.seg_end = -AA_INF_F;
         outcome = 2;
         break;
       }
       float vis = (L.flags & F_SHADOW) ? 1.0f - L.shw : 1.0f;
-      finish_sample<STATS>(A, S, L, vis, c);
-      outcome = next_ray<STATS>(A, S, L, c);
+      finish_sample<STATS, B>(A, S, L, vis, c);
+      outcome = next_ray<STATS, B>(A, S, L, c);
       if (!DENSE && outcome == 1 && !(L.flags & F_DENSE)) {
         // the pixel's next sample can step at once: go on with it here
         setup();
@@ -917,7 +965,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
 // they are checked against the FP64 quadrature.  Free flights do not
 // survive a segment end or the step budget (the exponential distribution
 // is memoryless, so a fresh draw there is exact).
-template <bool STATS>
+template <bool STATS, bool B>
 __global__ __launch_bounds__(kSBlock, 4) void wf_step_m(RenderArgs A, WFState S, const uint32_t *__restrict__ listA,
                                                    const uint32_t *__restrict__ countA, uint32_t *__restrict__ outA,
                                                    uint32_t *__restrict__ outG, uint32_t *__restrict__ counts,
@@ -1107,12 +1155,13 @@ __global__ __launch_bounds__(kSBlock, 4) void wf_step_m(RenderArgs A, WFState S,
       }
       if (!end) continue;
       // ---- the ray ended
-      if (!(L.flags & F_SHADOW) && L.alb.w > 0.0f && A.amb != 1.0f) {
-        float3 so = make_float3(A.pos.x + L.d.x * L.t, A.pos.y + L.d.y * L.t, A.pos.z + L.d.z * L.t);
+      const ReqArgs &R = view<B>(A, L.pix);
+      if (!(L.flags & F_SHADOW) && L.alb.w > 0.0f && R.amb != 1.0f) {
+        float3 so = make_float3(R.pos.x + L.d.x * L.t, R.pos.y + L.d.y * L.t, R.pos.z + L.d.z * L.t);
         float ste, stx;
-        aabb_intersection(so, A.light, 0.0f, 10000.0f, A.lb, A.rt, ste, stx);
+        aabb_intersection(so, R.light, 0.0f, 10000.0f, A.lb, A.rt, ste, stx);
         L.o = so;
-        L.d = A.light;
+        L.d = R.light;
         L.tmax = stx;
         L.flags = F_SHADOW;
         L.shw = 0.0f;
@@ -1123,8 +1172,8 @@ __global__ __launch_bounds__(kSBlock, 4) void wf_step_m(RenderArgs A, WFState S,
         break;
       }
       float vis = (L.flags & F_SHADOW) ? 1.0f - L.shw : 1.0f;
-      finish_sample<STATS>(A, S, L, vis, c);
-      outcome = next_ray<STATS>(A, S, L, c);
+      finish_sample<STATS, B>(A, S, L, vis, c);
+      outcome = next_ray<STATS, B>(A, S, L, c);
       if (outcome == 1 && !(L.flags & F_DENSE)) {
         setup();
         tau = -1.0f;
@@ -1153,7 +1202,7 @@ __global__ __launch_bounds__(kSBlock, 4) void wf_step_m(RenderArgs A, WFState S,
 // order, so the payload equals the serial query's bit for bit.
 constexpr uint32_t kFront = 2048; // frontier entries per level
 
-template <bool STATS>
+template <bool STATS, bool B>
 __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S, const uint32_t *__restrict__ listD,
                                                         const uint32_t *__restrict__ countD,
                                                         uint32_t *__restrict__ outA, uint32_t *__restrict__ outG,
@@ -1176,7 +1225,6 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   if (blockIdx.x >= *countD) return;
   const uint32_t slot = listD[blockIdx.x];
-  const bool marching = A.blue != 0;
   Counters c = {};
   const float4 *__restrict__ N = A.bvh.nodes;
   const uint32_t log2K = A.bvh.log2K, top = A.bvh.top;
@@ -1270,7 +1318,7 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
       } else {
         ++steps;
         if (STATS && lane == 0) c.steps++;
-        if (marching) {
+        if (is_march<B>(A, L)) {
           L.t = L.t + A.unit;
         } else {
           float xi = aa_lcg_next(&L.trng);
@@ -1291,7 +1339,7 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
       if (!query(x, y, z, pl)) point_query<false>(A, x, y, z, pl, c); // frontier overflow: serial walk
       L.flags &= ~F_PENDING;
       const bool shadow = (L.flags & F_SHADOW) != 0;
-      if (marching) {
+      if (is_march<B>(A, L)) {
         if (pl.count > 0 && pl.dens > 0.0f) {
           float D = density(pl.dens);
           L.LHS += D;
@@ -1327,25 +1375,26 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
     }
     // ---- the ray ended
     L.flags &= ~F_PENDING;
-    if (!(L.flags & F_SHADOW) && L.alb.w > 0.0f && A.amb != 1.0f) {
-      float3 so = make_float3(A.pos.x + L.d.x * L.t, A.pos.y + L.d.y * L.t, A.pos.z + L.d.z * L.t);
+    const ReqArgs &R = view<B>(A, L.pix);
+    if (!(L.flags & F_SHADOW) && L.alb.w > 0.0f && R.amb != 1.0f) {
+      float3 so = make_float3(R.pos.x + L.d.x * L.t, R.pos.y + L.d.y * L.t, R.pos.z + L.d.z * L.t);
       float ste, stx;
-      aabb_intersection(so, A.light, 0.0f, 10000.0f, A.lb, A.rt, ste, stx);
+      aabb_intersection(so, R.light, 0.0f, 10000.0f, A.lb, A.rt, ste, stx);
       L.o = so;
-      L.d = A.light;
+      L.d = R.light;
       L.tmax = stx;
-      L.flags = F_SHADOW;
+      L.flags = F_SHADOW | (L.flags & F_MARCH);
       L.shw = 0.0f;
       L.LHS = 0.0f;
-      L.t = marching ? -A.unit * L.r2 : 0.0f;
+      L.t = is_march<B>(A, L) ? -A.unit * L.r2 : 0.0f;
       L.seg_end = -AA_INF_F;
       outcome = 2;
       break;
     }
     if (lane == 0) {
       float vis = (L.flags & F_SHADOW) ? 1.0f - L.shw : 1.0f;
-      finish_sample<STATS>(A, S, L, vis, c);
-      outcome = next_ray<STATS>(A, S, L, c);
+      finish_sample<STATS, B>(A, S, L, vis, c);
+      outcome = next_ray<STATS, B>(A, S, L, c);
     }
     break;
   }
@@ -1366,9 +1415,43 @@ __global__ void wf_reset_counts(uint32_t *c) {
   if (threadIdx.x < 16) c[threadIdx.x] = 0u;
 }
 
+// ---- the view table -------------------------------------------------------------
+constexpr uint32_t kViewPack = 8;
+struct ViewPack {
+  ReqArgs v[kViewPack];
+  uint32_t first, n;
+};
+
+// Copies up to kViewPack views (passed by value) into the device table.
+__global__ void wf_put_views(ReqArgs *dst, ViewPack P) {
+  if (threadIdx.x < P.n) dst[P.first + threadIdx.x] = P.v[threadIdx.x];
+}
+
+// The view of every tile: the last view whose first tile is <= the tile.
+__global__ __launch_bounds__(256) void wf_tile_views(const ReqArgs *__restrict__ V, uint32_t nv,
+                                                     uint32_t *__restrict__ tile_req, uint32_t ntiles) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= ntiles) return;
+  uint32_t lo = 0, hi = nv - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) / 2;
+    if (V[mid].tile0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  tile_req[t] = lo;
+}
+
+// Latency record: per view (samples finished, clock at the last) and the
+// clock at the start.
+__global__ void wf_lat_start(unsigned long long *lat, uint32_t nv) {
+  for (uint32_t i = threadIdx.x; i < 2 * nv; i += blockDim.x) lat[i] = 0ull;
+  if (threadIdx.x == 0) lat[2 * nv] = (unsigned long long)wall_clock64();
+}
+
 } // namespace
 
 // The gather / step rounds of one pass, until no ray is left.
+template <bool B>
 static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S, uint32_t *const lists[4],
                       uint32_t *listD, uint32_t *cnt) {
   const bool st = ctx->stats;
@@ -1399,21 +1482,21 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
     const uint32_t blocks = (nA + kSBlock - 1) / kSBlock;
     if (ctx->render_mode == 2 && !A.blue) {
       if (st)
-        wf_step_m<true><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+        wf_step_m<true, B><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
       else
-        wf_step_m<false><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+        wf_step_m<false, B><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
     } else if (st) {
-      wf_step<true, false, 1><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+      wf_step<true, false, 1, B><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
     } else {
       switch (ctx->tune_step_waves) {
       case 5:
-        wf_step<false, false, 5><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+        wf_step<false, false, 5, B><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
         break;
       case 6:
-        wf_step<false, false, 6><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+        wf_step<false, false, 6, B><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
         break;
       default: // 4 waves per SIMD
-        wf_step<false, false, 4><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+        wf_step<false, false, 4, B><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
       }
     }
     AARBF_CHECK_LAUNCH(ctx, "wf_step");
@@ -1425,9 +1508,9 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
     if (nD > 0) {
       // one wave per dense ray
       if (st)
-        wf_dense_wave<true><<<nD, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, cap);
+        wf_dense_wave<true, B><<<nD, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, cap);
       else
-        wf_dense_wave<false><<<nD, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, cap);
+        wf_dense_wave<false, B><<<nD, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, cap);
       AARBF_CHECK_LAUNCH(ctx, "wf_dense_wave");
     }
     if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[3], ctx->stream));
@@ -1453,16 +1536,26 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
 }
 
 
-// Host driver of the wavefront rounds.  Grows the slot state in the
-// context's scratch pool; reads the two queue lengths back after each round.
-// Samples are rendered in passes of at most kPassSamples (the colour buffer
-// holds one pass), each pass folded into accum before the next.
-int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
-  const uint32_t tiles_x = (uint32_t)(A.W + 7) / 8, tiles_y = (uint32_t)(A.H + 7) / 8;
-  const uint64_t npix = (uint64_t)tiles_x * tiles_y * 64u;
+// Host driver of the wavefront rounds over one or more views.  The views'
+// 8x8 tiles are laid out one after another in a combined pixel space with a
+// slot per pixel, so views of any size share every round.  Grows the slot
+// state in the context's scratch pool; reads the two queue lengths back
+// after each round.  Samples are rendered in passes of at most
+// kPassSamples (the colour buffer holds one pass), each pass folded into
+// the views' accum buffers before the next.
+int render_wavefront(aarbf_context *ctx, const RenderArgs &A0, const ReqArgs *views, uint32_t nv, bool batch) {
+  std::vector<ReqArgs> V(views, views + nv);
+  uint64_t ntiles = 0;
+  uint32_t max_spp = 0;
+  for (ReqArgs &R : V) {
+    R.tile0 = (uint32_t)std::min<uint64_t>(ntiles, 0xffffffffu);
+    ntiles += (uint64_t)R.tiles_x * (uint64_t)((R.H + 7) / 8);
+    max_spp = std::max(max_spp, R.spp);
+  }
+  const uint64_t npix = ntiles * 64u;
   const uint64_t nslots = npix;
-  AARBF_REQUIRE(ctx, npix < (1ull << 28), "aarbf_render_rbf: image too large");
-  const uint64_t chunk = std::min<uint64_t>(A.spp, kPassSamples);
+  AARBF_REQUIRE(ctx, npix < (1ull << 28), "aarbf_render_rbf: image too large (2^28 pixels per call)");
+  const uint64_t chunk = std::min<uint64_t>(max_spp, kPassSamples);
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t o = off;
@@ -1475,7 +1568,8 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
          o_ceo = take(kCand * nslots), o_hidx = take(4 * kCand * npix), o_htin = take(4 * kCand * npix),
          o_htout = take(4 * kCand * npix), o_heo = take(kCand * npix), o_head = take(16 * npix),
          o_col = take(16 * npix * chunk), o_next = take(4 * npix), o_l0 = take(4 * nslots), o_l1 = take(4 * nslots), o_l2 = take(4 * nslots),
-         o_l3 = take(4 * nslots), o_l4 = take(4 * nslots), o_cnt = take(64);
+         o_l3 = take(4 * nslots), o_l4 = take(4 * nslots), o_cnt = take(64), o_view = take(sizeof(ReqArgs) * nv),
+         o_tview = take(batch ? 4 * ntiles : 4);
   char *base = (char *)scratch(ctx, off);
   if (!base) return set_error(ctx, AARBF_ERR_OUT_OF_MEMORY, "aarbf_render_rbf: wavefront state");
   WFState S;
@@ -1499,7 +1593,6 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
   S.next_s = (uint32_t *)(base + o_next);
   S.nslots = (uint32_t)nslots;
   S.npix = (uint32_t)npix;
-  S.tiles_x = tiles_x;
   uint32_t *lists[4] = {(uint32_t *)(base + o_l0), (uint32_t *)(base + o_l1), (uint32_t *)(base + o_l2),
                         (uint32_t *)(base + o_l3)};
   uint32_t *listD = (uint32_t *)(base + o_l4);
@@ -1507,18 +1600,61 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
   uint32_t *cnt = (uint32_t *)(base + o_cnt);
   S.visit = cnt + 8;
   const bool st = ctx->stats;
+  RenderArgs A = A0;
+  A.v0 = V[0];
+  A.req = nullptr;
+  A.tile_req = nullptr;
+  A.nreq = nv;
+  A.lat = nullptr;
+  AARBF_REQUIRE(ctx, batch || nv == 1, "render_wavefront: several views need the batch schedule");
+  if (batch) {
+    // the view table and the view of every tile, then the latency record
+    A.req = (const ReqArgs *)(base + o_view);
+    A.tile_req = (const uint32_t *)(base + o_tview);
+    for (uint32_t i = 0; i < nv; i += kViewPack) {
+      ViewPack P{};
+      P.first = i;
+      P.n = std::min(kViewPack, nv - i);
+      for (uint32_t k = 0; k < P.n; ++k) P.v[k] = V[i + k];
+      wf_put_views<<<1, 64, 0, ctx->stream>>>((ReqArgs *)A.req, P);
+    }
+    wf_tile_views<<<(uint32_t)((ntiles + 255) / 256), 256, 0, ctx->stream>>>(A.req, nv, (uint32_t *)A.tile_req,
+                                                                              (uint32_t)ntiles);
+    if (ctx->lat_cap < 2 * nv + 1) {
+      if (ctx->lat_buf) {
+        hipStreamSynchronize(ctx->stream);
+        hipFree(ctx->lat_buf);
+        ctx->lat_buf = nullptr;
+        ctx->lat_cap = 0;
+      }
+      AARBF_HIP(ctx, hipMalloc(&ctx->lat_buf, sizeof(unsigned long long) * (2 * nv + 1)));
+      ctx->lat_cap = 2 * nv + 1;
+    }
+    A.lat = ctx->lat_buf;
+    wf_lat_start<<<1, 256, 0, ctx->stream>>>(A.lat, nv);
+  }
+  ctx->lat_n = batch ? nv : 0;
+  AARBF_CHECK_LAUNCH(ctx, "view table");
   const uint32_t pix_blocks = (uint32_t)((npix + 255) / 256), init_blocks = (uint32_t)((nslots + 255) / 256);
   wf_head_reset<<<pix_blocks, 256, 0, ctx->stream>>>(S);
-  for (uint32_t s0 = 0; s0 < A.spp; s0 += (uint32_t)chunk) {
+  for (uint32_t s0 = 0; s0 < max_spp; s0 += (uint32_t)chunk) {
     S.s0 = s0;
-    S.s1 = (uint32_t)std::min<uint64_t>(A.spp, s0 + chunk);
+    S.s1 = (uint32_t)std::min<uint64_t>(max_spp, s0 + chunk);
     if (ctx->timing) AARBF_HIP(ctx, hipEventRecord(ctx->tev[0], ctx->stream));
     wf_reset_counts<<<1, 64, 0, ctx->stream>>>(cnt);
-    wf_pass_reset<<<pix_blocks, 256, 0, ctx->stream>>>(A, S);
-    if (st)
-      wf_init<true><<<init_blocks, 256, 0, ctx->stream>>>(A, S, lists[0], lists[1], cnt);
-    else
-      wf_init<false><<<init_blocks, 256, 0, ctx->stream>>>(A, S, lists[0], lists[1], cnt);
+    if (batch) {
+      wf_pass_reset<true><<<pix_blocks, 256, 0, ctx->stream>>>(A, S);
+      if (st)
+        wf_init<true, true><<<init_blocks, 256, 0, ctx->stream>>>(A, S, lists[0], lists[1], cnt);
+      else
+        wf_init<false, true><<<init_blocks, 256, 0, ctx->stream>>>(A, S, lists[0], lists[1], cnt);
+    } else {
+      wf_pass_reset<false><<<pix_blocks, 256, 0, ctx->stream>>>(A, S);
+      if (st)
+        wf_init<true, false><<<init_blocks, 256, 0, ctx->stream>>>(A, S, lists[0], lists[1], cnt);
+      else
+        wf_init<false, false><<<init_blocks, 256, 0, ctx->stream>>>(A, S, lists[0], lists[1], cnt);
+    }
     AARBF_CHECK_LAUNCH(ctx, "wf_init");
     if (ctx->timing) {
       AARBF_HIP(ctx, hipEventRecord(ctx->tev[1], ctx->stream));
@@ -1528,10 +1664,13 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A) {
       ctx->tms[3] += e;
       ctx->tlaunch[3] += 1;
     }
-    int rc = run_rounds(ctx, A, S, lists, listD, cnt);
+    int rc = batch ? run_rounds<true>(ctx, A, S, lists, listD, cnt) : run_rounds<false>(ctx, A, S, lists, listD, cnt);
     if (rc != AARBF_OK) return rc;
     if (ctx->timing) AARBF_HIP(ctx, hipEventRecord(ctx->tev[0], ctx->stream));
-    wf_fold<<<pix_blocks, 256, 0, ctx->stream>>>(A, S);
+    if (batch)
+      wf_fold<true><<<pix_blocks, 256, 0, ctx->stream>>>(A, S);
+    else
+      wf_fold<false><<<pix_blocks, 256, 0, ctx->stream>>>(A, S);
     AARBF_CHECK_LAUNCH(ctx, "wf_fold");
     if (ctx->timing) {
       AARBF_HIP(ctx, hipEventRecord(ctx->tev[1], ctx->stream));

@@ -296,6 +296,29 @@ int aarbf_set_payload(aarbf_context *ctx, int format);
 /* The decoded payload of the last render (x, y, z, r, alpha^3, r^2.2, g^2.2,
  * b^2.2 per particle) into device float out[8 * n]. */
 int aarbf_payload_read(aarbf_context *ctx, uint64_t out, uint32_t n);
+/* ---- batched rendering (no counterpart in the reference) -------------------
+ * One view of a batch: what one aarbf_render_rbf_spp call would take. */
+typedef struct aarbf_request {
+  aarbf_raygen_data rg;    /* image / accum buffers, STBN atlas, global AABB */
+  aarbf_push_constants pc; /* camera, frameID, accumID, enableBlueNoise, light, ... */
+  int width, height;
+  uint32_t spp;
+} aarbf_request;
+/* Renders n views of one scene in one wavefront schedule: the views' pixels
+ * share every gather / step round, so small views do not leave the GPU idle.
+ * Each view's image equals that of aarbf_render_rbf_spp(ctx, &rg, &pc, width,
+ * height, spp) bit for bit.  Views may differ in size, spp, camera, frame /
+ * accum counters, light, showNoise and enableBlueNoise (stochastic marching
+ * or delta tracking, mixed in one batch); world, particles, numParticles,
+ * particlesPerLeaf, radii, transfer functions, rbfRadius, unit,
+ * visualizeAttributes and the global AABB must match request 0.  Render
+ * mode 1 does not apply (the wavefront schedule is used); mode 2 applies
+ * when no view uses marching.  Records each view's latency. */
+int aarbf_render_batch(aarbf_context *ctx, const aarbf_request *reqs, uint32_t n);
+/* Per view of the last aarbf_render_batch (n = its view count): ms from the
+ * start of the batch's GPU work to the end of the view's last sample (device
+ * wall clock), or -1 if it did not finish.  Synchronises the stream. */
+int aarbf_batch_latency(aarbf_context *ctx, double *ms, uint32_t n);
 /* ParticleSplatRayGen (deviceCodeSplat.hlsl:58-154). */
 int aarbf_render_splat(aarbf_context *ctx, const aarbf_raygen_data *rg,
                        const aarbf_push_constants *pc, int width, int height);
