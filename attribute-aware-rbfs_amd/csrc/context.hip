@@ -126,6 +126,7 @@ int aarbf_context_destroy(aarbf_context *ctx) {
   if (ctx->scratch) hipFree(ctx->scratch);
   if (ctx->baked) hipFree(ctx->baked);
   if (ctx->lat_buf) hipFree(ctx->lat_buf);
+  for (auto &g : ctx->graphs) hipGraphExecDestroy(g.second);
   if (ctx->d_stats) hipFree(ctx->d_stats);
   if (ctx->d_counter) hipFree(ctx->d_counter);
   for (hipEvent_t &e : ctx->tev)
@@ -170,6 +171,11 @@ int aarbf_set_tuning(aarbf_context *ctx, int param, int value) {
   case AARBF_TUNE_STEP_CAP:
     AARBF_REQUIRE(ctx, value >= 16 && value <= 65536, "aarbf_set_tuning: step cap must be in [16, 65536]");
     ctx->tune_step_cap = value;
+    return AARBF_OK;
+  case AARBF_TUNE_GRAPH_ROUNDS:
+    AARBF_REQUIRE(ctx, value >= 0 && value <= 64 && value % 2 == 0,
+                  "aarbf_set_tuning: graph rounds must be 0 or an even number up to 64");
+    ctx->tune_graph_rounds = value;
     return AARBF_OK;
   default:
     return set_error(ctx, AARBF_ERR_INVALID_ARGUMENT, "aarbf_set_tuning: unknown parameter");

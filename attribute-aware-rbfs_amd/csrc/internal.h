@@ -7,6 +7,8 @@
 #include <mutex>
 #include <set>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "aarbf.h"
 
@@ -77,6 +79,8 @@ struct aarbf_context {
   int cus = 0;                   // compute units of the device
   int tune_step_waves = 0;       // occupancy target of the step kernel (0 = default, 4 waves per SIMD)
   int tune_step_cap = 4096;      // tracker steps per ray per wavefront round
+  int tune_graph_rounds = 0;     // wavefront rounds per captured hipGraph (0 = host-driven rounds)
+  std::vector<std::pair<std::string, hipGraphExec_t>> graphs; // captured rounds, keyed by their arguments
   // kernel timing (HIP events around each kernel class of the render rounds)
   bool timing = false;
   hipEvent_t tev[4] = {nullptr, nullptr, nullptr, nullptr};

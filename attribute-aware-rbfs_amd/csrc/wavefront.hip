@@ -39,6 +39,9 @@
 #include "render_common.h"
 
 #include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
 
 namespace aarbf {
 namespace {
@@ -422,10 +425,13 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
   __shared__ float s_tin[kCand * kGBlock];
   __shared__ float s_tout[kCand * kGBlock];
   __shared__ uint32_t s_stack[kStackLevels * kGBlock];
-  load_levels(A);
   const uint32_t n_g = *countG;
-  uint32_t gid = blockIdx.x * kGBlock + threadIdx.x;
+  if (blockIdx.x * kGBlock >= n_g) return; // the whole block: no barrier is skipped
+  load_levels(A);
   Counters c = {};
+  // grid-stride, so a fixed grid (a captured graph) covers any count
+  for (uint32_t base = blockIdx.x * kGBlock; base < n_g; base += gridDim.x * kGBlock) {
+  const uint32_t gid = base + threadIdx.x;
   if (gid < n_g) {
     const uint32_t slot = listG[gid];
     uint32_t *gi = s_idx + threadIdx.x;
@@ -668,7 +674,16 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
     float4 g = S.seg[slot];
     S.seg[slot] = make_float4(t0, seg_end, g.z, g.w);
   }
+  }
   flush_stats<STATS>(A, c);
+}
+
+// Tracker steps per ray in a round: fewer live rays -> more steps each, so
+// the tail of a pass costs few rounds.  Only the split of a ray's steps
+// over rounds depends on it, not the image.
+__device__ inline uint32_t round_cap(uint32_t base_cap, uint32_t nslots, uint32_t n_active) {
+  const uint64_t b = base_cap;
+  return (uint32_t)min(65536ull, max(b, b * nslots / (4ull * max(n_active, 1u))));
 }
 
 // ---- step ------------------------------------------------------------------------
@@ -690,6 +705,9 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
   __shared__ uint8_t s_eo[NC * kSBlock];
   __shared__ uint16_t s_stack[NS * kSBlock];
   __shared__ float s_dm[kDmLds];
+  const uint32_t n_a = *countA;
+  if (blockIdx.x * kSBlock >= n_a) return; // the whole block: no barrier is skipped
+  step_cap = round_cap(step_cap, S.nslots, n_a);
   if (DENSE) load_levels(A);
   // density map red channel, read at every accepted-candidate step
   const bool dm_lds = A.dm_n <= kDmLds;
@@ -703,10 +721,11 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
     aa_tf_coords(u, A.dm_n, &j0, &j1, &f);
     return aa_lerp_tex(s_dm[j0], s_dm[j1], f);
   };
-  const uint32_t n_a = *countA;
-  uint32_t gid = blockIdx.x * kSBlock + threadIdx.x;
   Counters c = {};
   uint64_t simt[3] = {0, 0, 0};
+  // grid-stride, so a fixed grid (a captured graph) covers any count
+  for (uint32_t base = blockIdx.x * kSBlock; base < n_a; base += gridDim.x * kSBlock) {
+  const uint32_t gid = base + threadIdx.x;
   int outcome = 0; // 0 nothing, 1 keep active, 2 active + gather, 3 dense queue
   uint32_t slot = 0;
   if (gid < n_a) {
@@ -786,7 +805,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
           simt[0] += 1;
           simt[1] += (uint32_t)__popcll(live);
         }
-        uint64_t touch = __ballot(act != 0 || (L.flags & F_PENDING) != 0 && DENSE);
+        uint64_t touch = __ballot(act != 0 || ((L.flags & F_PENDING) != 0 && DENSE));
         if (touch && (uint32_t)__ffsll((unsigned long long)live) - 1u == (threadIdx.x & 63u)) simt[2] += 1;
       }
       const bool pending = (L.flags & F_PENDING) != 0;
@@ -939,6 +958,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
   if (outcome == 1 || outcome == 2) push(outA, &counts[0], slot);
   if (outcome == 2) push(outG, &counts[1], slot);
   if (outcome == 3) push(outD, countD, slot);
+  }
   flush_stats<STATS>(A, c);
   if (STATS)
     for (int k = 0; k < 3; ++k) {
@@ -977,6 +997,9 @@ __global__ __launch_bounds__(kSBlock, 4) void wf_step_m(RenderArgs A, WFState S,
   __shared__ float s_a3[kCand * kSBlock];
   __shared__ uint8_t s_eo[kCand * kSBlock];
   __shared__ float s_dm[kDmLds], s_dmax[kDmLds];
+  const uint32_t n_a = *countA;
+  if (blockIdx.x * kSBlock >= n_a) return; // the whole block: no barrier is skipped
+  step_cap = round_cap(step_cap, S.nslots, n_a);
   const bool dm_lds = A.dm_n <= kDmLds;
   if (dm_lds)
     for (uint32_t j = threadIdx.x; j < A.dm_n; j += kSBlock) s_dm[j] = A.dmap[j].x;
@@ -1004,9 +1027,9 @@ __global__ __launch_bounds__(kSBlock, 4) void wf_step_m(RenderArgs A, WFState S,
     aa_tf_coords(u, A.dm_n, &j0, &j1, &f);
     return fmaxf(aa_lerp_tex(s_dm[j0], s_dm[j1], f), s_dmax[j0]);
   };
-  const uint32_t n_a = *countA;
-  uint32_t gid = blockIdx.x * kSBlock + threadIdx.x;
   Counters c = {};
+  for (uint32_t base = blockIdx.x * kSBlock; base < n_a; base += gridDim.x * kSBlock) {
+  const uint32_t gid = base + threadIdx.x;
   int outcome = 0; // 0 nothing, 1 keep active, 2 active + gather, 3 dense queue
   uint32_t slot = 0;
   if (gid < n_a) {
@@ -1187,6 +1210,7 @@ __global__ __launch_bounds__(kSBlock, 4) void wf_step_m(RenderArgs A, WFState S,
   if (outcome == 1 || outcome == 2) push(outA, &counts[0], slot);
   if (outcome == 2) push(outG, &counts[1], slot);
   if (outcome == 3) push(outD, countD, slot);
+  }
   flush_stats<STATS>(A, c);
 }
 
@@ -1206,9 +1230,13 @@ template <bool STATS, bool B>
 __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S, const uint32_t *__restrict__ listD,
                                                         const uint32_t *__restrict__ countD,
                                                         uint32_t *__restrict__ outA, uint32_t *__restrict__ outG,
-                                                        uint32_t *__restrict__ counts, uint32_t step_cap) {
+                                                        uint32_t *__restrict__ counts, const uint32_t *countA,
+                                                        uint32_t step_cap) {
   __shared__ uint32_t s_front[2][kFront];
   __shared__ float s_dm[kDmLds];
+  const uint32_t n_d = *countD;
+  if (blockIdx.x >= n_d) return; // the whole block: no barrier is skipped
+  step_cap = round_cap(step_cap, S.nslots, *countA);
   load_levels(A);
   const bool dm_lds = A.dm_n <= kDmLds;
   if (dm_lds)
@@ -1223,8 +1251,6 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
   };
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  if (blockIdx.x >= *countD) return;
-  const uint32_t slot = listD[blockIdx.x];
   Counters c = {};
   const float4 *__restrict__ N = A.bvh.nodes;
   const uint32_t log2K = A.bvh.log2K, top = A.bvh.top;
@@ -1302,6 +1328,9 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
     return true;
   };
 
+  // one ray per wave at a time; a fixed grid (a captured graph) loops
+  for (uint32_t item = blockIdx.x; item < n_d; item += gridDim.x) {
+  const uint32_t slot = listD[item];
   Lane L;
   load_lane(S, slot, L);
   int outcome = 0;
@@ -1403,6 +1432,7 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
     if (outcome == 1 || outcome == 2) push(outA, &counts[0], slot);
     if (outcome == 2) push(outG, &counts[1], slot);
   }
+  }
   flush_stats<STATS>(A, c);
 }
 
@@ -1450,73 +1480,160 @@ __global__ void wf_lat_start(unsigned long long *lat, uint32_t nv) {
 
 } // namespace
 
-// The gather / step rounds of one pass, until no ray is left.
+// Launches one gather / step round on list pair `cur`.  Grids: gather and
+// step from the host's counts (gb, sb), the dense kernel from its count read
+// back after the step (db == 0), or all three fixed (graph capture; the
+// kernels read their counts on the device and loop over them).
+template <bool B>
+static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &S, uint32_t *const lists[4],
+                        uint32_t *listD, uint32_t *cnt, int cur, uint32_t gb, uint32_t sb, uint32_t db,
+                        uint32_t *nD_out) {
+  const bool st = ctx->stats;
+  uint32_t *lA = lists[cur], *lG = lists[cur + 1], *nA2 = lists[2 - cur], *nG2 = lists[3 - cur];
+  uint32_t *cA = cnt + (cur ? 2 : 0), *cN = cnt + (cur ? 0 : 2);
+  const bool tm = ctx->timing && db == 0;
+  if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[0], ctx->stream));
+  if (gb > 0) {
+    if (st)
+      wf_gather<true><<<gb, kGBlock, 0, ctx->stream>>>(A, S, lG, cA + 1);
+    else
+      wf_gather<false><<<gb, kGBlock, 0, ctx->stream>>>(A, S, lG, cA + 1);
+  }
+  wf_zero<<<1, 64, 0, ctx->stream>>>(cN, 2);
+  wf_zero<<<1, 64, 0, ctx->stream>>>(cnt + 4, 1);
+  if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[1], ctx->stream));
+  const uint32_t cap = (uint32_t)ctx->tune_step_cap;
+  if (ctx->render_mode == 2 && !A.blue) {
+    if (st)
+      wf_step_m<true, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+    else
+      wf_step_m<false, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+  } else if (st) {
+    wf_step<true, false, 1, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+  } else {
+    switch (ctx->tune_step_waves) {
+    case 5:
+      wf_step<false, false, 5, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+      break;
+    case 6:
+      wf_step<false, false, 6, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+      break;
+    default: // 4 waves per SIMD
+      wf_step<false, false, 4, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+    }
+  }
+  AARBF_CHECK_LAUNCH(ctx, "wf_step");
+  if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[2], ctx->stream));
+  uint32_t nD = db;
+  if (db == 0) {
+    // the dense queue (filled by the step kernel) sizes its own grid
+    AARBF_HIP(ctx, hipMemcpyAsync(&nD, cnt + 4, 4, hipMemcpyDeviceToHost, ctx->stream));
+    AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  if (nD > 0) {
+    // one wave per dense ray
+    if (st)
+      wf_dense_wave<true, B><<<nD, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, cA, cap);
+    else
+      wf_dense_wave<false, B><<<nD, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, cA, cap);
+    AARBF_CHECK_LAUNCH(ctx, "wf_dense_wave");
+  }
+  if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[3], ctx->stream));
+  if (nD_out) *nD_out = nD;
+  return AARBF_OK;
+}
+
+// Fixed grids of the captured rounds: enough blocks for every CU at the
+// kernels' occupancy, each looping over its share of the queue.
+static void graph_grids(const aarbf_context *ctx, uint64_t nslots, uint32_t &gb, uint32_t &sb, uint32_t &db) {
+  const uint64_t cus = (uint64_t)std::max(ctx->cus, 1);
+  gb = (uint32_t)std::min<uint64_t>((nslots + kGBlock - 1) / kGBlock, 12 * cus);
+  sb = (uint32_t)std::min<uint64_t>((nslots + kSBlock - 1) / kSBlock, 32 * cus);
+  db = (uint32_t)std::min<uint64_t>(nslots, 8 * cus);
+}
+
+// The gather / step rounds of one pass, until no ray is left.  Host-driven:
+// the two queue lengths are read back after each round and size the next
+// round's grids.  With graph rounds R > 0: R rounds with fixed grids are
+// captured once into a hipGraph (cached in the context, keyed by every
+// argument the kernels see) and replayed, with one read-back per replay.
 template <bool B>
 static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S, uint32_t *const lists[4],
                       uint32_t *listD, uint32_t *cnt) {
-  const bool st = ctx->stats;
-  const uint64_t nslots = S.nslots;
   uint32_t h[2];
   AARBF_HIP(ctx, hipMemcpyAsync(h, cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
   AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  int cur = 0; // lists[cur] = A, lists[cur+1] = G; the other pair receives
   uint32_t nA = h[0], nG = h[1];
-  while (nA > 0) {
-    uint32_t *lA = lists[cur], *lG = lists[cur + 1], *nA2 = lists[2 - cur], *nG2 = lists[3 - cur];
-    uint32_t *cA = cnt + (cur ? 2 : 0), *cN = cnt + (cur ? 0 : 2);
-    const bool tm = ctx->timing;
-    if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[0], ctx->stream));
-    if (nG > 0) {
-      uint32_t blocks = (nG + kGBlock - 1) / kGBlock;
-      if (st)
-        wf_gather<true><<<blocks, kGBlock, 0, ctx->stream>>>(A, S, lG, cA + 1);
-      else
-        wf_gather<false><<<blocks, kGBlock, 0, ctx->stream>>>(A, S, lG, cA + 1);
-    }
-    wf_zero<<<1, 64, 0, ctx->stream>>>(cN, 2);
-    wf_zero<<<1, 64, 0, ctx->stream>>>(cnt + 4, 1);
-    if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[1], ctx->stream));
-    // fewer live rays -> more steps per round, so the tail costs few rounds
-    const uint64_t base_cap = (uint64_t)ctx->tune_step_cap;
-    uint32_t cap = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(base_cap, base_cap * nslots / (4ull * nA)));
-    const uint32_t blocks = (nA + kSBlock - 1) / kSBlock;
-    if (ctx->render_mode == 2 && !A.blue) {
-      if (st)
-        wf_step_m<true, B><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
-      else
-        wf_step_m<false, B><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
-    } else if (st) {
-      wf_step<true, false, 1, B><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
-    } else {
-      switch (ctx->tune_step_waves) {
-      case 5:
-        wf_step<false, false, 5, B><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
-        break;
-      case 6:
-        wf_step<false, false, 6, B><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
-        break;
-      default: // 4 waves per SIMD
-        wf_step<false, false, 4, B><<<blocks, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+  const int R = ctx->tune_graph_rounds;
+  if (R > 0 && !ctx->timing) {
+    uint32_t gb, sb, db;
+    graph_grids(ctx, S.nslots, gb, sb, db);
+    struct Key {
+      RenderArgs A;
+      WFState S;
+      uint32_t *lists[4], *listD, *cnt;
+      uint32_t gb, sb, db;
+      int R, batch, stats, mode, waves, cap;
+    } key;
+    memset(&key, 0, sizeof(key));
+    key.A = A;
+    key.S = S;
+    for (int k = 0; k < 4; ++k) key.lists[k] = lists[k];
+    key.listD = listD;
+    key.cnt = cnt;
+    key.gb = gb;
+    key.sb = sb;
+    key.db = db;
+    key.R = R;
+    key.batch = B;
+    key.stats = ctx->stats;
+    key.mode = ctx->render_mode;
+    key.waves = ctx->tune_step_waves;
+    key.cap = ctx->tune_step_cap;
+    const std::string kb((const char *)&key, sizeof(key));
+    hipGraphExec_t exec = nullptr;
+    for (auto &g : ctx->graphs)
+      if (g.first == kb) exec = g.second;
+    if (!exec) {
+      AARBF_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      int rc = AARBF_OK;
+      for (int r = 0; r < R && rc == AARBF_OK; ++r)
+        rc = launch_round<B>(ctx, A, S, lists, listD, cnt, (r & 1) ? 2 : 0, gb, sb, db, nullptr);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+      if (rc != AARBF_OK) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
       }
+      AARBF_HIP(ctx, e);
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      AARBF_HIP(ctx, ei);
+      if (ctx->graphs.size() >= 8) {
+        hipGraphExecDestroy(ctx->graphs.front().second);
+        ctx->graphs.erase(ctx->graphs.begin());
+      }
+      ctx->graphs.emplace_back(kb, exec);
     }
-    AARBF_CHECK_LAUNCH(ctx, "wf_step");
-    if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[2], ctx->stream));
-    // the dense queue (filled by the step kernel) sizes its own grid
+    // R is even: after a replay the next lists are pair 0 again
+    while (nA > 0) {
+      AARBF_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+      AARBF_HIP(ctx, hipMemcpyAsync(h, cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
+      AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      nA = h[0];
+    }
+    return AARBF_OK;
+  }
+  int cur = 0; // lists[cur] = A, lists[cur+1] = G; the other pair receives
+  while (nA > 0) {
     uint32_t nD = 0;
-    AARBF_HIP(ctx, hipMemcpyAsync(&nD, cnt + 4, 4, hipMemcpyDeviceToHost, ctx->stream));
-    AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (nD > 0) {
-      // one wave per dense ray
-      if (st)
-        wf_dense_wave<true, B><<<nD, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, cap);
-      else
-        wf_dense_wave<false, B><<<nD, kSBlock, 0, ctx->stream>>>(A, S, listD, cnt + 4, nA2, nG2, cN, cap);
-      AARBF_CHECK_LAUNCH(ctx, "wf_dense_wave");
-    }
-    if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[3], ctx->stream));
+    const uint32_t gb = (nG + kGBlock - 1) / kGBlock, sb = (nA + kSBlock - 1) / kSBlock;
+    int rc = launch_round<B>(ctx, A, S, lists, listD, cnt, cur, gb, sb, 0, &nD);
+    if (rc != AARBF_OK) return rc;
+    uint32_t *cN = cnt + (cur ? 0 : 2);
     AARBF_HIP(ctx, hipMemcpyAsync(h, cN, 8, hipMemcpyDeviceToHost, ctx->stream));
     AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (tm) {
+    if (ctx->timing) {
       float e[3];
       for (int k = 0; k < 3; ++k) AARBF_HIP(ctx, hipEventElapsedTime(&e[k], ctx->tev[k], ctx->tev[k + 1]));
       if (nG > 0) {
@@ -1544,6 +1661,9 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
 // kPassSamples (the colour buffer holds one pass), each pass folded into
 // the views' accum buffers before the next.
 int render_wavefront(aarbf_context *ctx, const RenderArgs &A0, const ReqArgs *views, uint32_t nv, bool batch) {
+  // captured rounds read the view from the table, so one graph serves
+  // every frame of a view
+  if (ctx->tune_graph_rounds > 0) batch = true;
   std::vector<ReqArgs> V(views, views + nv);
   uint64_t ntiles = 0;
   uint32_t max_spp = 0;
@@ -1601,7 +1721,33 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A0, const ReqArgs *vi
   S.visit = cnt + 8;
   const bool st = ctx->stats;
   RenderArgs A = A0;
-  A.v0 = V[0];
+  if (batch) {
+    // per-view fields live in the table; cleared here so that the kernel
+    // arguments (and a captured graph's key) do not change from frame to frame
+    const uint32_t blue = A.blue;
+    A = RenderArgs{};
+    A.blue = blue;
+    A.visAttr = A0.visAttr;
+    A.max_depth = A0.max_depth;
+    A.unit = A0.unit;
+    A.lb = A0.lb;
+    A.rt = A0.rt;
+    A.pr = A0.pr;
+    A.pc = A0.pc;
+    A.qp = A0.qp;
+    A.qc = A0.qc;
+    A.q16 = A0.q16;
+    A.numParticles = A0.numParticles;
+    A.ppl = A0.ppl;
+    A.cmap = A0.cmap;
+    A.dmap = A0.dmap;
+    A.cm_n = A0.cm_n;
+    A.dm_n = A0.dm_n;
+    A.stats = A0.stats;
+    A.bvh = A0.bvh;
+  } else {
+    A.v0 = V[0];
+  }
   A.req = nullptr;
   A.tile_req = nullptr;
   A.nreq = nv;

@@ -47,6 +47,7 @@ def main() -> int:
     ap.add_argument("--payload", type=int, default=0, help="0 FP32 particle payload, 1 FP16 leaf-local quantised")
     ap.add_argument("--step-waves", type=int, default=0)
     ap.add_argument("--step-cap", type=int, default=4096)
+    ap.add_argument("--graph-rounds", type=int, default=0, help="rounds per captured hipGraph (0 = host-driven)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "render_traffic.json"),
                     help="HBM bytes per render launch measured with rocprofv3 --pmc")
@@ -85,6 +86,7 @@ def main() -> int:
     ctx.set_payload(a.payload)
     ctx.set_tuning(ctx.TUNE_STEP_WAVES, a.step_waves)
     ctx.set_tuning(ctx.TUNE_STEP_CAP, a.step_cap)
+    ctx.set_tuning(ctx.TUNE_GRAPH_ROUNDS, a.graph_rounds)
     torch.cuda.synchronize()
     tp = time.time()
     sc = aarbf.Scene.prepare(ctx, wl.xyzw, wl.colormap, wl.radiusmap, wl.densitymap, wl.rbf_radius,

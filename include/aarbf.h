@@ -160,10 +160,16 @@ int aarbf_set_render_mode(aarbf_context *ctx, int mode);
 /* Schedule parameters of the default renderer (none changes the image):
  * AARBF_TUNE_STEP_WAVES: occupancy target of the step kernel in waves per
  * SIMD (0 = default 4, or 4..6); AARBF_TUNE_STEP_CAP: tracker steps
- * per ray per wavefront round (16..65536, default 4096).  No counterpart in
- * the reference. */
+ * per ray per wavefront round (16..65536, default 4096);
+ * AARBF_TUNE_GRAPH_ROUNDS: 0 (default) = the host reads the queue lengths
+ * back after every round and sizes the next round's grids; an even R in
+ * 2..64 = R rounds with fixed grids are captured once into a hipGraph
+ * (cached per set of arguments; kernels read their counts on the device)
+ * and replayed with one read-back per replay.  No counterpart in the
+ * reference. */
 #define AARBF_TUNE_STEP_WAVES 1
 #define AARBF_TUNE_STEP_CAP 2
+#define AARBF_TUNE_GRAPH_ROUNDS 3
 int aarbf_set_tuning(aarbf_context *ctx, int param, int value);
 /* Enable (1) or disable (0) render statistics; read them back (and reset
  * them) with aarbf_render_stats_read. */

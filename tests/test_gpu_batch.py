@@ -130,3 +130,38 @@ def test_batch_rejects_mixed_scenes(gpu_ctx):
             (sc2.raygen_data(img, acc, W, H), sc2.push_constants(cam, 0.004), W, H, 1)]
     with pytest.raises(aarbf.AarbfError, match="share the scene"):
         gpu_ctx.render_batch(reqs)
+
+
+@pytest.mark.parametrize("rounds", [2, 8])
+def test_graph_rounds_equal_host_rounds(gpu_ctx, rounds):
+    """Rounds captured into a hipGraph and replayed (the graph is cached and
+    replayed again on the next call) give the host-driven images; eager
+    renders run between the graph calls."""
+    sc, _ = _scene(gpu_ctx, n=2500, seed=16)
+    stbn = gpu_ctx.stbn(scenes.synthetic_stbn(seed=6))
+    ref = _views(gpu_ctx, sc, VIEWS, stbn)
+    _run_batch(gpu_ctx, sc, ref, stbn)
+    single = _views(gpu_ctx, sc, VIEWS[:2], stbn)
+    for (pc, img, acc, W, H, spp, m) in single:
+        gpu_ctx.render(sc.raygen_data(img, acc, W, H, stbn=stbn if m else 0), pc, W, H, spp)
+    gpu_ctx.sync()
+    try:
+        for rep in range(3):
+            gpu_ctx.set_tuning(gpu_ctx.TUNE_GRAPH_ROUNDS, rounds)
+            got = _views(gpu_ctx, sc, VIEWS, stbn)
+            _run_batch(gpu_ctx, sc, got, stbn)
+            for k, (va, vb) in enumerate(zip(ref, got)):
+                assert np.array_equal(va[2].cpu().numpy(), vb[2].cpu().numpy()), f"replay {rep} view {k}"
+                assert np.array_equal(va[1].cpu().numpy(), vb[1].cpu().numpy()), f"replay {rep} view {k}"
+            one = _views(gpu_ctx, sc, VIEWS[:2], stbn)
+            for (pc, img, acc, W, H, spp, m) in one:
+                gpu_ctx.render(sc.raygen_data(img, acc, W, H, stbn=stbn if m else 0), pc, W, H, spp)
+            gpu_ctx.sync()
+            for va, vb in zip(single, one):
+                assert np.array_equal(va[2].cpu().numpy(), vb[2].cpu().numpy())
+            # eager (host-driven) work between the graph calls
+            gpu_ctx.set_tuning(gpu_ctx.TUNE_GRAPH_ROUNDS, 0)
+            eager = _views(gpu_ctx, sc, VIEWS[2:4], stbn)
+            _run_batch(gpu_ctx, sc, eager, stbn)
+    finally:
+        gpu_ctx.set_tuning(gpu_ctx.TUNE_GRAPH_ROUNDS, 0)
