@@ -289,12 +289,26 @@ class Context:
     def render_batch(self, requests):
         """Renders views of one scene in one schedule (aarbf_render_batch);
         `requests`: sequence of (rg, pc, width, height, spp)."""
+        self.check(self.lib.aarbf_render_batch(self.h, self._requests(requests), len(requests)),
+                   "aarbf_render_batch")
+
+    @staticmethod
+    def _requests(requests):
         arr = (_lib.Request * len(requests))()
         for i, (rg, pc, w, h, spp) in enumerate(requests):
             arr[i].rg = rg
             arr[i].pc = pc
             arr[i].width, arr[i].height, arr[i].spp = int(w), int(h), int(spp)
-        self.check(self.lib.aarbf_render_batch(self.h, arr, len(requests)), "aarbf_render_batch")
+        return arr
+
+    def autotune(self, requests, reps: int = 1) -> dict:
+        """Schedule autotuner (aarbf_autotune) over the views `requests`;
+        keeps the fastest setting in the context and returns it."""
+        res = _lib.TuneResult()
+        self.check(self.lib.aarbf_autotune(self.h, self._requests(requests), len(requests), reps, C.byref(res)),
+                   "aarbf_autotune")
+        return {"step_cap": res.step_cap, "step_waves": res.step_waves, "graph_rounds": res.graph_rounds,
+                "candidates": res.candidates, "baseline_ms": res.baseline_ms, "best_ms": res.best_ms}
 
     def batch_latency(self, n: int) -> np.ndarray:
         """Per view of the last batch: ms from its start to the view's last sample."""

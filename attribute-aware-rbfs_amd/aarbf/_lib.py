@@ -73,6 +73,11 @@ class Request(C.Structure):
     _fields_ = [("rg", RayGenData), ("pc", PushConstants), ("width", C.c_int), ("height", C.c_int), ("spp", u32)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("step_cap", C.c_int32), ("step_waves", C.c_int32), ("graph_rounds", C.c_int32),
+                ("candidates", u32), ("baseline_ms", C.c_double), ("best_ms", C.c_double)]
+
+
 class RenderStats(C.Structure):
     _fields_ = [(n, u64) for n in ("samples", "steps", "queries", "nodes", "leaves", "particles",
                                    "covering", "exhausted")]
@@ -133,6 +138,7 @@ SIGNATURES = {
                                      C.c_int]),
     "aarbf_render_batch": (C.c_int, [P, C.POINTER(Request), u32]),
     "aarbf_batch_latency": (C.c_int, [P, C.POINTER(C.c_double), u32]),
+    "aarbf_autotune": (C.c_int, [P, C.POINTER(Request), u32, u32, C.POINTER(TuneResult)]),
     "aarbf_taa_resolve": (C.c_int, [P, C.POINTER(TAAConstants)]),
     "aarbf_abi_layout": (u32, [C.POINTER(u32), u32]),
 }

@@ -219,3 +219,36 @@ def mixture_8m(n: int = 8388608, seed: int = 3, width: int = 1920, height: int =
     return Workload(f"mixture-{n}", xyzw, radii, float(radii.max()), 0.0004, width, height, spp, False, lf, la, lu,
                     fovy_scale(45.0), reference_tf("colormap"), reference_tf("radiusmap"),
                     reference_tf("densitymap"), frames=frames, taa=True)
+
+
+def mixture_16m(n: int = 16777216, seed: int = 5):
+    """Batching workload scene: 16,777,216 particles from a 64-component
+    Gaussian mixture (as mixture_8m otherwise)."""
+    xyzw, radii = gaussian_mixture(n, 64, seed)
+    return xyzw, radii
+
+
+def batch_views(n: int = 256, seed: int = 11, side=(128, 1024), aspect=(0.5, 2.0), spp=(1, 8),
+                radius: float = 1.6, centre=(0.5, 0.5, 0.5)):
+    """Camera requests of uneven resolution: side log-uniform in `side`,
+    aspect uniform in `aspect` (W = side sqrt(aspect), H = side / sqrt(aspect)),
+    spp uniform in {spp[0]..spp[1]}, stochastic marching or delta tracking
+    with probability 1/2 each, cameras on a seeded sphere of `radius` about
+    `centre` looking at it.  Returns a list of dicts."""
+    u = [uniform(seed, n, s) for s in range(6)]
+    lo, hi = np.log(side[0]), np.log(side[1])
+    out = []
+    for i in range(n):
+        s = float(np.exp(lo + (hi - lo) * u[0][i]))
+        a = aspect[0] + (aspect[1] - aspect[0]) * float(u[1][i])
+        W = max(1, int(round(s * math.sqrt(a))))
+        H = max(1, int(round(s / math.sqrt(a))))
+        k = spp[0] + int(u[2][i] * (spp[1] - spp[0] + 1))
+        z = 2.0 * float(u[3][i]) - 1.0
+        phi = 2.0 * math.pi * float(u[4][i])
+        rr = math.sqrt(max(0.0, 1.0 - z * z))
+        pos = tuple(np.asarray(centre) + radius * np.array([rr * math.cos(phi), z, rr * math.sin(phi)]))
+        up = (0.0, 1.0, 0.0) if abs(z) < 0.95 else (1.0, 0.0, 0.0)
+        out.append({"width": W, "height": H, "spp": min(k, spp[1]), "marching": bool(u[5][i] < 0.5),
+                    "look_from": pos, "look_at": tuple(centre), "up": up, "frame": i})
+    return out

@@ -49,6 +49,13 @@ def main() -> int:
     ap.add_argument("--step-cap", type=int, default=4096)
     ap.add_argument("--graph-rounds", type=int, default=0, help="rounds per captured hipGraph (0 = host-driven)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--diff-views", type=int, default=4,
+                    help="views of the differentiable-rendering leg timed (0 = skip the leg)")
+    ap.add_argument("--diff-particles", type=int, default=1048576)
+    ap.add_argument("--batch-views", type=int, default=0,
+                    help="camera requests of the batching leg (0 = skip the leg; the batching workload has 256)")
+    ap.add_argument("--batch-particles", type=int, default=16777216)
+    ap.add_argument("--batch-payload", type=int, default=1)
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "render_traffic.json"),
                     help="HBM bytes per render launch measured with rocprofv3 --pmc")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -192,6 +199,11 @@ def main() -> int:
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(sc, wl, cams[0], a.cpu_threads)
+    ctx.set_timing(False)
+    ctx.set_render_mode(0)
+    ctx.set_payload(0)
+    diff = diff_leg(ctx, stream, a) if a.diff_views > 0 else None
+    batch = batch_leg(ctx, stream, a) if a.batch_views > 0 else None
 
     out = {
         "metric": "Msamples/s at 8M RBFs 1080p delta tracking; fwd+bwd ms/view; 1 MI355X",
@@ -225,11 +237,160 @@ def main() -> int:
             "kernels": kernels,
         },
         "cpu_baseline": cpu,
+        "diff": diff,
+        "batch": batch,
         "wall_s": round(wall, 3),
     }
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
+
+
+def diff_leg(ctx, stream, a):
+    """Differentiable rendering: forward and forward + adjoint ms per view on
+    the differentiation workload (1,048,576 particles from the 32-component
+    mixture with attributes normalised to [0, 1], 512x512, 4 spp expected
+    stochastic marching, 256-entry colormap and densitymap, L2 image loss
+    against a perturbed copy of the scene), library calls timed with HIP
+    events on the library's stream; the per-view scene rebuild (sort,
+    bounds, BVH) is timed separately."""
+    import torch
+    from aarbf import scenes
+    from aarbf.diff import _Render
+    t0 = time.time()
+    xyzw, radii = scenes.gaussian_mixture(a.diff_particles, 32, 7)
+    attr = xyzw[:, 3].astype(np.float64)
+    xyzw[:, 3] = ((attr - attr.min()) / max(attr.max() - attr.min(), 1e-12)).astype(np.float32)
+    rng = np.random.default_rng(17)
+    tgt = xyzw.copy()
+    tgt[:, :3] += rng.normal(0.0, 0.002, (len(xyzw), 3)).astype(np.float32)
+    tgt[:, 3] = np.clip(tgt[:, 3] + rng.normal(0.0, 0.05, len(xyzw)), 0.0, 1.0).astype(np.float32)
+    tgt_r = (radii * rng.uniform(0.9, 1.1, len(radii))).astype(np.float32)
+    cm, dm, rm = scenes.ramp_table(256), scenes.ramp_table(256), scenes.constant_table(n=4)
+    dev = torch.device("cuda")
+    T = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    W = H = 512
+    spp, unit = 4, 0.1 * float(np.median(radii))
+    rbf = float(max(radii.max(), tgt_r.max()))
+    log(f"diff leg: workload generated in {time.time() - t0:.1f} s")
+    fw, bw, rebuild = [], [], []
+    for v in range(a.diff_views):
+        z = 2.0 * float(scenes.uniform(23, a.diff_views, 0)[v]) - 1.0
+        ph = 2.0 * np.pi * float(scenes.uniform(23, a.diff_views, 1)[v])
+        rr = np.sqrt(max(0.0, 1.0 - z * z))
+        lf = (0.5 + 1.6 * rr * np.cos(ph), 0.5 + 1.6 * z, 0.5 + 1.6 * rr * np.sin(ph))
+        cam = aarbf_camera(lf, (0.5, 0.5, 0.5), (0.0, 1.0, 0.0) if abs(z) < 0.95 else (1.0, 0.0, 0.0), W, H)
+        torch.cuda.synchronize()
+        tr = time.time()
+        target = _Render(ctx, cam, W, H, unit, spp, v, rbf, 1, 8, T(tgt[:, :3]), T(tgt[:, 3]), T(tgt_r), T(cm),
+                         T(dm), T(rm))
+        timg = target.forward()
+        target.close()
+        torch.cuda.synchronize()
+        tr = time.time()
+        r = _Render(ctx, cam, W, H, unit, spp, v, rbf, 1, 8, T(xyzw[:, :3]), T(xyzw[:, 3]), T(radii), T(cm),
+                    T(dm), T(rm))
+        torch.cuda.synchronize()
+        rebuild.append((time.time() - tr) * 1e3)
+        n = len(xyzw)
+        g = torch.zeros((H, W, 4), dtype=torch.float32, device=dev)
+        grads = [torch.zeros(k, dtype=torch.float32, device=dev) for k in (3 * n, n, n, 256 * 4, 256, 4)]
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        torch.cuda.synchronize()
+        e[0].record(stream)
+        ctx.diff_forward(r.rg, r.pc, W, H, spp, r.img, r.path)
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        g[..., :3] = 2.0 * (r.img[..., :3] - timg)  # dL/dimage of the L2 loss
+        torch.cuda.synchronize()
+        e2 = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        e2[0].record(stream)
+        ctx.diff_backward(r.rg, r.pc, W, H, spp, g, *grads, path=r.path)
+        e2[1].record(stream)
+        torch.cuda.synchronize()
+        fw.append(e[0].elapsed_time(e[1]))
+        bw.append(e2[0].elapsed_time(e2[1]))
+        r.close()
+        log(f"diff view {v}: forward {fw[-1]:.2f} ms, adjoint {bw[-1]:.2f} ms, rebuild {rebuild[-1]:.1f} ms")
+    f, b = float(np.mean(fw)), float(np.mean(bw))
+    return {"workload": f"mixture-{a.diff_particles} expected marching 512x512 4 spp, L2 loss vs a perturbed copy, "
+                        f"{a.diff_views} views on a seeded sphere",
+            "fwd_ms_per_view": round(f, 3), "fwd_bwd_ms_per_view": round(f + b, 3),
+            "ratio": round((f + b) / f, 3), "rebuild_ms_per_view": round(float(np.mean(rebuild)), 2),
+            "unit": unit, "spp": spp}
+
+
+def aarbf_camera(look_from, look_at, up, W, H):
+    import aarbf
+    from aarbf import scenes
+    return aarbf.camera_basis(look_from, look_at, up, scenes.fovy_scale(45.0), W, H)
+
+
+def batch_leg(ctx, stream, a):
+    """Batching: camera requests of uneven resolution (side log-uniform in
+    [128, 1024], aspect in [0.5, 2], 1-8 spp, half stochastic marching with
+    the synthetic blue-noise atlas, half delta tracking) over the
+    64-component mixture, FP16 payload by default; the schedule autotuned on
+    the first 32 requests; one aarbf_render_batch call timed against the
+    same requests rendered one call each (latency of a request = time from
+    the start of the set to its last sample)."""
+    import torch
+    import aarbf
+    from aarbf import scenes
+    t0 = time.time()
+    xyzw, radii = scenes.mixture_16m(n=a.batch_particles)
+    views = scenes.batch_views(n=a.batch_views)
+    log(f"batch leg: workload generated in {time.time() - t0:.1f} s")
+    ctx.set_payload(a.batch_payload)
+    sc = aarbf.Scene.prepare(ctx, xyzw, scenes.reference_tf("colormap"), scenes.reference_tf("radiusmap"),
+                             scenes.reference_tf("densitymap"), float(radii.max()), radii=radii, ppl=1, branching=8)
+    stbn = ctx.stbn(scenes.synthetic_stbn(seed=3))
+    unit = 0.0004
+    bufs, reqs = [], []
+    for v in views:
+        W, H = v["width"], v["height"]
+        img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+        acc = torch.zeros_like(img)
+        bufs.append((img, acc))
+        cam = aarbf_camera(v["look_from"], v["look_at"], v["up"], W, H)
+        pc = sc.push_constants(cam, unit, frame_id=v["frame"], accum_id=1, blue_noise=v["marching"])
+        reqs.append((sc.raygen_data(img, acc, W, H, stbn=stbn if v["marching"] else 0), pc, W, H, v["spp"]))
+    torch.cuda.synchronize()
+    tuned = ctx.autotune(reqs[:32], reps=1)
+    log(f"batch leg: autotuned {tuned}")
+    samples = sum(W * H * spp for (_, _, W, H, spp) in reqs)
+    ctx.render_batch(reqs)  # warm-up (graph capture, buffers)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    ctx.render_batch(reqs)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    batch_ms = e0.elapsed_time(e1)
+    lat = ctx.batch_latency(len(reqs))
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(reqs) + 1)]
+    ev[0].record(stream)
+    for k, (rg, pc, W, H, spp) in enumerate(reqs):
+        ctx.render(rg, pc, W, H, spp)
+        ev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    seq = np.array([ev[0].elapsed_time(e) for e in ev[1:]])
+    ctx.set_payload(0)
+    for k in (ctx.TUNE_STEP_CAP, ctx.TUNE_STEP_WAVES, ctx.TUNE_GRAPH_ROUNDS):
+        ctx.set_tuning(k, {ctx.TUNE_STEP_CAP: a.step_cap, ctx.TUNE_STEP_WAVES: a.step_waves,
+                           ctx.TUNE_GRAPH_ROUNDS: a.graph_rounds}[k])
+    return {"workload": f"mixture-{a.batch_particles} (64 components), {len(reqs)} requests, side log-U[128,1024], "
+                        "aspect U[0.5,2], spp U{1..8}, half marching / half delta tracking",
+            "payload": ("fp32", "fp16 leaf-local quantised")[a.batch_payload],
+            "samples": int(samples), "batch_ms": round(batch_ms, 2),
+            "msamples_per_s": round(samples / (batch_ms * 1e-3) / 1e6, 3),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+            "sequential_ms": round(float(seq[-1]), 2),
+            "sequential_msamples_per_s": round(samples / (seq[-1] * 1e-3) / 1e6, 3),
+            "sequential_latency_ms_p50": round(float(np.percentile(seq, 50)), 2),
+            "sequential_latency_ms_p99": round(float(np.percentile(seq, 99)), 2),
+            "autotuned": tuned}
 
 
 def copy_calibration(nbytes: int) -> int:

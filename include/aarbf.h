@@ -325,6 +325,22 @@ int aarbf_render_batch(aarbf_context *ctx, const aarbf_request *reqs, uint32_t n
  * start of the batch's GPU work to the end of the view's last sample (device
  * wall clock), or -1 if it did not finish.  Synchronises the stream. */
 int aarbf_batch_latency(aarbf_context *ctx, double *ms, uint32_t n);
+/* Schedule autotuner: renders the views (into its own buffers; the
+ * callers' images are untouched) under candidate schedule parameters --
+ * AARBF_TUNE_STEP_CAP over {1024, 2048, 4096, 8192}, AARBF_TUNE_STEP_WAVES
+ * over {4, 5, 6}, AARBF_TUNE_GRAPH_ROUNDS over {0, 8}, by coordinate descent
+ * from the current setting, each timed as the best of `reps` runs after a
+ * warm-up -- and keeps the fastest in the context.  The BVH leaf size is a
+ * scene property, tuned by rebuilding the scene per leaf size around this
+ * call.  No counterpart in the reference. */
+typedef struct aarbf_tune_result {
+  int32_t step_cap, step_waves, graph_rounds;
+  uint32_t candidates;  /* settings timed */
+  double baseline_ms;   /* the setting at entry */
+  double best_ms;       /* the setting kept */
+} aarbf_tune_result;
+int aarbf_autotune(aarbf_context *ctx, const aarbf_request *reqs, uint32_t n, uint32_t reps,
+                   aarbf_tune_result *out);
 /* ParticleSplatRayGen (deviceCodeSplat.hlsl:58-154). */
 int aarbf_render_splat(aarbf_context *ctx, const aarbf_raygen_data *rg,
                        const aarbf_push_constants *pc, int width, int height);

@@ -165,3 +165,44 @@ def test_graph_rounds_equal_host_rounds(gpu_ctx, rounds):
             _run_batch(gpu_ctx, sc, eager, stbn)
     finally:
         gpu_ctx.set_tuning(gpu_ctx.TUNE_GRAPH_ROUNDS, 0)
+
+
+def test_autotune_keeps_images_and_caller_buffers(gpu_ctx):
+    """The schedule autotuner leaves the callers' buffers alone, keeps a
+    setting from its candidate sets, and images rendered under the kept
+    setting equal those under the default one."""
+    from aarbf import tune
+    sc, tfs = _scene(gpu_ctx, n=2000, seed=18)
+    stbn = gpu_ctx.stbn(scenes.synthetic_stbn(seed=8))
+    ref = _views(gpu_ctx, sc, VIEWS, stbn)
+    _run_batch(gpu_ctx, sc, ref, stbn)
+    probe = _views(gpu_ctx, sc, VIEWS, stbn)
+    reqs = [(sc.raygen_data(img, acc, W, H, stbn=stbn if m else 0), pc, W, H, spp)
+            for (pc, img, acc, W, H, spp, m) in probe]
+    before = [(v[1].cpu().numpy().copy(), v[2].cpu().numpy().copy()) for v in probe]
+    try:
+        r = gpu_ctx.autotune(reqs, reps=1)
+        print("autotune:", r)
+        assert r["step_cap"] in (1024, 2048, 4096, 8192) and r["step_waves"] in (0, 4, 5, 6)
+        assert r["graph_rounds"] in (0, 8) and r["candidates"] >= 7 and 0 < r["best_ms"] <= r["baseline_ms"]
+        for v, (i0, a0) in zip(probe, before):  # untouched
+            assert np.array_equal(v[1].cpu().numpy(), i0) and np.array_equal(v[2].cpu().numpy(), a0)
+        got = _views(gpu_ctx, sc, VIEWS, stbn)
+        _run_batch(gpu_ctx, sc, got, stbn)
+        for va, vb in zip(ref, got):
+            assert np.array_equal(va[2].cpu().numpy(), vb[2].cpu().numpy())
+            assert np.array_equal(va[1].cpu().numpy(), vb[1].cpu().numpy())
+        # leaf size around it: rebuilds the scene per candidate
+        xyzw, radii = random_scene(2000, seed=18, extent=1.0, r=(0.03, 0.09))
+
+        def make(s):
+            return [(s.raygen_data(img, acc, W, H, stbn=stbn if m else 0),
+                     s.push_constants(default_camera(W, H, look_from=VIEWS[k][6]), 0.004, frame_id=VIEWS[k][4],
+                                      blue_noise=m), W, H, spp)
+                    for k, (pc, img, acc, W, H, spp, m) in enumerate(probe)]
+        best, info = tune.tune_leaf_size(gpu_ctx, xyzw, *tfs, 0.05, make, radii=radii, leaf_sizes=(1, 2))
+        print("leaf size:", info)
+        assert info["particles_per_leaf"] in (1, 2) and best.ppl == info["particles_per_leaf"]
+    finally:
+        for k in (gpu_ctx.TUNE_STEP_CAP, gpu_ctx.TUNE_STEP_WAVES, gpu_ctx.TUNE_GRAPH_ROUNDS):
+            gpu_ctx.set_tuning(k, 4096 if k == gpu_ctx.TUNE_STEP_CAP else 0)
