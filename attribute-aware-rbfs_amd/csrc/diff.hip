@@ -29,10 +29,11 @@ namespace aarbf {
 namespace {
 
 constexpr int kDBlock = 64;
-constexpr uint32_t kTexMax = 64;  // texels per table with LDS gradient sums (larger tables: global atomics)
+constexpr uint32_t kTexMax = 256; // texels per table with LDS gradient sums (larger tables: global atomics)
 constexpr uint32_t kCache = 4;    // per-lane particle adjoint cache entries
 constexpr uint32_t kSlot = 9;     // particle index + 8 adjoint sums
-constexpr uint32_t kRec = 16;     // covering particles a march point keeps for the adjoint
+constexpr uint32_t kDC = 12;      // candidates per march segment
+constexpr uint32_t kDense = 32;   // points queried one by one where candidates crowd
 
 struct DiffArgs {
   RenderArgs A;
@@ -45,6 +46,7 @@ struct DiffArgs {
   float4 *path;            // per sample: sum_k F'(L_k) (c_k - c_{k+1}) (forward writes, adjoint reads), or null
   const float4 *gimg;      // dLoss/dimage (adjoint)
   float *g_pos, *g_rad, *g_attr, *g_cm, *g_dm, *g_rm;
+  float *acc5;             // adjoint: per particle (r, alpha^3, rgb^2.2) sums, before the TF chain rule
 };
 
 __global__ __launch_bounds__(256) void diff_bake(const float4 *__restrict__ particles, const float *__restrict__ radii,
@@ -125,26 +127,32 @@ struct DPay {
   float rho, S[3];
 };
 
-__device__ inline void payload(const RenderArgs &A, float x, float y, float z, DPay &pl) {
-  pl.count = 0;
-  pl.rho = pl.S[0] = pl.S[1] = pl.S[2] = 0.0f;
-  for_covering(A, x, y, z, [&](uint32_t p, const float4 &q, float d) {
-    float4 k = A.pc[p];
-    float w = aa_rbf(d, q.w) * k.x;
-    pl.count += 1;
-    pl.rho += w;
-    pl.S[0] += k.y * w;
-    pl.S[1] += k.z * w;
-    pl.S[2] += k.w * w;
-  });
-}
 
-// the march of sample s of pixel (px, py): fn(x, y, z, payload) at every
-// point some particle covers.  With rec != null the covering particles of
-// the point (index, distance) are also kept there, up to kRec of them,
-// stride kDBlock.
+// Per-lane LDS columns of the march (stride kDBlock): the current
+// segment's candidates and the walk stack.
+struct DSeg {
+  uint32_t *idx;
+  float *tin, *tout;
+  uint16_t *stk;
+};
+
+// The march of sample s of pixel (px, py): fn(x, y, z, payload, cover) at
+// every point some particle covers, where cover(g) calls g(p, q, d) for
+// each covering particle p (baked q, distance d) in the order the payload
+// summed them; fn returns false to end the march.
+//
+// Points are t_k = t_{k-1} + unit (float recurrence from tenter + unit r1).
+// Which particles can cover them comes from a segment gather: one walk of
+// the quantised tree collects the kDC particles whose sphere (widened by a
+// rounding margin) the ray enters first, with entry / exit parameters; the
+// segment ends where the first particle left out would enter, so every
+// particle covering a point before that is a candidate.  Candidates enter
+// and leave an active set at their parameters; a point with an empty set
+// costs only the recurrence, the others test the active candidates
+// exactly (|P - x| < r).  Where more than kDC candidates crowd the segment
+// start the next kDense points use full point queries instead.
 template <class Fn>
-__device__ inline void march(const DiffArgs &D, int px, int py, uint32_t s, Fn fn, uint32_t *rec = nullptr) {
+__device__ inline void march(const DiffArgs &D, const DSeg &W, int px, int py, uint32_t s, Fn fn) {
   const RenderArgs &A = D.A;
   uint32_t fid = (A.frameID + s) & 0xffffu;
   uint32_t rng = aa_rng_seed(fid, (uint32_t)px, (uint32_t)py, (uint32_t)A.W);
@@ -161,32 +169,164 @@ __device__ inline void march(const DiffArgs &D, int px, int py, uint32_t s, Fn f
   float te, tx;
   aabb_intersection(A.pos, d, 0.0f, 10000.0f, A.lb, A.rt, te, tx);
   if (!(te < tx)) return;
+  const float3 o = A.pos;
+  const float m = 2e-5f * (1.0f + fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) + fmaxf(fabsf(te), fabsf(tx)));
+  const bool seg_ok = A.bvh.groups != nullptr;
+  uint32_t n = 0, act = 0, eptr = 0;
+  float seg_end = -AA_INF_F, t_enter = AA_INF_F, t_exit = AA_INF_F, dense_end = -AA_INF_F;
+  // the candidates of the segment from t0 on, sorted by entry
+  auto gather = [&](float t0) {
+    SegQuery q;
+    q.ax = o.x + t0 * d.x;
+    q.ay = o.y + t0 * d.y;
+    q.az = o.z + t0 * d.z;
+    q.dx = d.x;
+    q.dy = d.y;
+    q.dz = d.z;
+    q.ix = safe_inv(d.x);
+    q.iy = safe_inv(d.y);
+    q.iz = safe_inv(d.z);
+    q.m = m;
+    q.len = tx - t0 + 4.0f * m;
+    n = 0;
+    float maxtin = -AA_INF_F, t_cut = AA_INF_F;
+    uint32_t maxpos = 0;
+    auto take = [&](uint32_t p) {
+      const float4 P = A.pr[p];
+      if (!(P.w > 0.0f)) return;
+      const float vx = P.x - o.x, vy = P.y - o.y, vz = P.z - o.z;
+      const float tc = vx * d.x + vy * d.y + vz * d.z;
+      const float hx = vx - tc * d.x, hy = vy - tc * d.y, hz = vz - tc * d.z;
+      const float rr = P.w + m;
+      const float w2 = rr * rr - (hx * hx + hy * hy + hz * hz);
+      if (!(w2 >= 0.0f)) return;
+      const float w = sqrtf(w2);
+      const float tin = tc - w - m, tout = tc + w + m;
+      if (!(tout >= t0) || !(tin <= tx) || !(tin < t_cut)) {
+        if (tout >= t0 && tin <= tx) t_cut = fminf(t_cut, tin);
+        return;
+      }
+      if (n < kDC) {
+        W.idx[n * kDBlock] = p;
+        W.tin[n * kDBlock] = tin;
+        W.tout[n * kDBlock] = tout;
+        if (tin > maxtin) {
+          maxtin = tin;
+          maxpos = n;
+        }
+        ++n;
+      } else if (tin < maxtin) { // evict the latest-entering candidate
+        t_cut = fminf(t_cut, maxtin);
+        W.idx[maxpos * kDBlock] = p;
+        W.tin[maxpos * kDBlock] = tin;
+        W.tout[maxpos * kDBlock] = tout;
+        maxtin = -AA_INF_F;
+        for (uint32_t k = 0; k < kDC; ++k) {
+          const float v = W.tin[k * kDBlock];
+          if (v > maxtin) {
+            maxtin = v;
+            maxpos = k;
+          }
+        }
+      } else {
+        t_cut = fminf(t_cut, tin);
+      }
+      q.len = fminf(tx, t_cut + 4.0f * m) - t0;
+    };
+    Counters c = {};
+    walk_q<kDBlock, false>(
+        A, W.stk, [&](const float4 &lo, const float4 &hi) { return seg_box(q, lo, hi); },
+        [&](uint32_t base, uint32_t mask) {
+          for (; mask; mask &= mask - 1u) {
+            const uint32_t leaf = base + (uint32_t)__ffs(mask) - 1u;
+            const uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
+            for (uint32_t p = p0; p < p1; ++p) take(p);
+          }
+          return false;
+        },
+        c);
+    for (uint32_t a = 1; a < n; ++a) { // insertion sort by entry
+      const uint32_t pi = W.idx[a * kDBlock];
+      const float ti = W.tin[a * kDBlock], to = W.tout[a * kDBlock];
+      uint32_t b = a;
+      for (; b > 0 && W.tin[(b - 1) * kDBlock] > ti; --b) {
+        W.idx[b * kDBlock] = W.idx[(b - 1) * kDBlock];
+        W.tin[b * kDBlock] = W.tin[(b - 1) * kDBlock];
+        W.tout[b * kDBlock] = W.tout[(b - 1) * kDBlock];
+      }
+      W.idx[b * kDBlock] = pi;
+      W.tin[b * kDBlock] = ti;
+      W.tout[b * kDBlock] = to;
+    }
+    seg_end = t_cut;
+    act = 0;
+    eptr = 0;
+    t_exit = AA_INF_F;
+    t_enter = n ? W.tin[0] : AA_INF_F;
+  };
   float t = te + A.unit * r1;
   for (uint32_t i = 0; i < A.max_depth; ++i) {
     t = t + A.unit;
     if (t >= tx) return;
-    const float x = A.pos.x + t * d.x, y = A.pos.y + t * d.y, z = A.pos.z + t * d.z;
-    DPay pl;
-    if (rec) {
-      pl.count = 0;
-      pl.rho = pl.S[0] = pl.S[1] = pl.S[2] = 0.0f;
-      for_covering(A, x, y, z, [&](uint32_t p, const float4 &q, float dd) {
-        float4 k = A.pc[p];
-        float w = aa_rbf(dd, q.w) * k.x;
-        if (pl.count < kRec) {
-          rec[(2 * pl.count) * kDBlock] = p;
-          rec[(2 * pl.count + 1) * kDBlock] = __float_as_uint(dd);
-        }
-        pl.count += 1;
-        pl.rho += w;
-        pl.S[0] += k.y * w;
-        pl.S[1] += k.z * w;
-        pl.S[2] += k.w * w;
-      });
-    } else {
-      payload(A, x, y, z, pl);
+    bool dense = !seg_ok || t < dense_end;
+    if (!dense && !(t < seg_end)) {
+      gather(t);
+      if (!(t < seg_end)) { // crowded: point queries for a while
+        dense = true;
+        dense_end = t + (float)kDense * A.unit;
+      }
     }
-    if (pl.count > 0 && pl.rho > 0.0f) fn(x, y, z, pl);
+    if (!dense) {
+      if (t >= t_enter) {
+        while (eptr < n && W.tin[eptr * kDBlock] <= t) {
+          const float to = W.tout[eptr * kDBlock];
+          if (t <= to) {
+            act |= 1u << eptr;
+            t_exit = fminf(t_exit, to);
+          }
+          ++eptr;
+        }
+        t_enter = eptr < n ? W.tin[eptr * kDBlock] : AA_INF_F;
+      }
+      if (t > t_exit) {
+        t_exit = AA_INF_F;
+        for (uint32_t b = act; b; b &= b - 1u) {
+          const uint32_t k = (uint32_t)__ffs(b) - 1u;
+          const float to = W.tout[k * kDBlock];
+          if (to < t) act &= ~(1u << k);
+          else t_exit = fminf(t_exit, to);
+        }
+      }
+      if (act == 0) continue;
+    }
+    const float x = o.x + t * d.x, y = o.y + t * d.y, z = o.z + t * d.z;
+    // the covering particles, in one fixed order for the payload and cover()
+    auto cover = [&](auto g) {
+      if (dense) {
+        for_covering(A, x, y, z, g);
+        return;
+      }
+      for (uint32_t b = act; b; b &= b - 1u) {
+        const uint32_t p = W.idx[((uint32_t)__ffs(b) - 1u) * kDBlock];
+        const float4 q = A.pr[p];
+        const float dd = aa_dist3(q.x, q.y, q.z, x, y, z);
+        if (dd < q.w) g(p, q, dd);
+      }
+    };
+    DPay pl;
+    pl.count = 0;
+    pl.rho = pl.S[0] = pl.S[1] = pl.S[2] = 0.0f;
+    cover([&](uint32_t p, const float4 &q, float dd) {
+      const float4 k = A.pc[p];
+      const float w = aa_rbf(dd, q.w) * k.x;
+      pl.count += 1;
+      pl.rho += w;
+      pl.S[0] += k.y * w;
+      pl.S[1] += k.z * w;
+      pl.S[2] += k.w * w;
+    });
+    if (pl.count > 0 && pl.rho > 0.0f)
+      if (!fn(x, y, z, pl, cover)) return;
   }
 }
 
@@ -194,20 +334,23 @@ __device__ inline void colour(const DPay &pl, float c[3]) {
   for (int k = 0; k < 3; ++k) c[k] = aa_powf(pl.S[k] / pl.rho, 1.0f / 2.2f);
 }
 
+#define AARBF_DSEG_LDS                                                                                      \
+  __shared__ uint32_t s_cidx[kDC * kDBlock];                                                                  \
+  __shared__ float s_ctin[kDC * kDBlock], s_ctout[kDC * kDBlock];                                             \
+  __shared__ uint16_t s_stk[kStackLevels * kDBlock];                                                          \
+  const DSeg W{s_cidx + threadIdx.x, s_ctin + threadIdx.x, s_ctout + threadIdx.x, s_stk + threadIdx.x};
+
 __global__ __launch_bounds__(kDBlock) void diff_forward(DiffArgs D) {
   const RenderArgs &A = D.A;
-  if (threadIdx.x < kMaxLevels) {
-    g_lvl_ofs[threadIdx.x] = A.bvh.level_ofs[threadIdx.x];
-    g_lvl_cnt[threadIdx.x] = A.bvh.counts[threadIdx.x];
-  }
-  __syncthreads();
+  AARBF_DSEG_LDS
+  load_levels(A);
   const int px = (int)(blockIdx.x * 8u + (threadIdx.x & 7u)), py = (int)(blockIdx.y * 8u + (threadIdx.x >> 3));
   if (px >= A.W || py >= A.H) return;
   float acc[3] = {0.0f, 0.0f, 0.0f};
   for (uint32_t s = 0; s < D.spp; ++s) {
     float L = 0.0f, C[3] = {0.0f, 0.0f, 0.0f}, V[3] = {0.0f, 0.0f, 0.0f}, prev[3] = {0.0f, 0.0f, 0.0f};
     bool have = false;
-    march(D, px, py, s, [&](float, float, float, const DPay &pl) {
+    march(D, W, px, py, s, [&](float, float, float, const DPay &pl, auto) {
       float sl, f, dF;
       uint32_t j0, j1;
       float Dv = tf_grad(A.dmap, A.dm_n, 0, pl.rho, sl, j0, j1, f);
@@ -224,6 +367,7 @@ __global__ __launch_bounds__(kDBlock) void diff_forward(DiffArgs D) {
         C[k] += (F1 - F0) * c[k];
         prev[k] = c[k];
       }
+      return F1 < 1.0f; // once F saturates every later term is 0
     });
     {
       float dF;
@@ -240,34 +384,22 @@ __global__ __launch_bounds__(kDBlock) void diff_forward(DiffArgs D) {
 
 __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
   const RenderArgs &A = D.A;
-  __shared__ float s_gcm[4 * kTexMax], s_gdm[kTexMax], s_grm[kTexMax];
+  __shared__ float s_gdm[kTexMax];
   __shared__ uint32_t s_cache[kCache * kSlot * kDBlock];
-  __shared__ uint32_t s_rec[2 * kRec * kDBlock];
-  uint32_t *rec = s_rec + threadIdx.x;
-  if (threadIdx.x < kMaxLevels) {
-    g_lvl_ofs[threadIdx.x] = A.bvh.level_ofs[threadIdx.x];
-    g_lvl_cnt[threadIdx.x] = A.bvh.counts[threadIdx.x];
-  }
-  for (uint32_t j = threadIdx.x; j < 4 * kTexMax; j += kDBlock) s_gcm[j] = 0.0f;
-  for (uint32_t j = threadIdx.x; j < kTexMax; j += kDBlock) s_gdm[j] = s_grm[j] = 0.0f;
+  AARBF_DSEG_LDS
+  load_levels(A);
+  for (uint32_t j = threadIdx.x; j < kTexMax; j += kDBlock) s_gdm[j] = 0.0f;
   __syncthreads();
-  const bool lds_cm = A.cm_n <= kTexMax, lds_dm = A.dm_n <= kTexMax, lds_rm = D.rm_n <= kTexMax;
-  auto add_cm = [&](uint32_t j, float v) {
-    if (lds_cm) atomicAdd(&s_gcm[j], v);
-    else atomicAdd(&D.g_cm[j], v);
-  };
+  const bool lds_dm = A.dm_n <= kTexMax;
   auto add_dm = [&](uint32_t j, float v) {
     if (lds_dm) atomicAdd(&s_gdm[j], v);
     else atomicAdd(&D.g_dm[j], v);
   };
-  auto add_rm = [&](uint32_t j, float v) {
-    if (lds_rm) atomicAdd(&s_grm[j], v);
-    else atomicAdd(&D.g_rm[j], v);
-  };
   // A particle covers runs of consecutive march points, so its per-point
   // adjoints (position, radius, alpha^3, colour^2.2) are summed in a small
-  // direct-mapped per-lane cache and pushed through its radiusmap /
-  // colormap lookups and to global memory once per run.
+  // direct-mapped per-lane cache and added to global memory once per run;
+  // the chain rule through the radiusmap / colormap lookups follows per
+  // particle in diff_chain.
   uint32_t *cache = s_cache + threadIdx.x; // [kCache][kSlot] words, stride kDBlock
   auto word = [&](uint32_t e, uint32_t w) -> float & { return *(float *)&cache[(e * kSlot + w) * kDBlock]; };
   auto flush = [&](uint32_t e) {
@@ -277,33 +409,7 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
     atomicAdd(&D.g_pos[3 * p], word(e, 1));
     atomicAdd(&D.g_pos[3 * p + 1], word(e, 2));
     atomicAdd(&D.g_pos[3 * p + 2], word(e, 3));
-    const float a_r = word(e, 4), a_a3 = word(e, 5);
-    const float4 raw = D.particles[p];
-    const float4 k = A.pc[p];
-    float sl, f, a_attr = 0.0f;
-    uint32_t k0, k1;
-    // r = radius * radiusmap(attribute)
-    const float rmv = tf_grad(D.rmap, D.rm_n, 0, raw.w, sl, k0, k1, f);
-    atomicAdd(&D.g_rad[p], a_r * rmv);
-    const float a_rm = a_r * D.radii[p];
-    add_rm(k0, a_rm * (1.0f - f));
-    add_rm(k1, a_rm * f);
-    a_attr += a_rm * sl;
-    // alpha^3 and colour^2.2 from colormap(attribute)
-    for (int c2 = 0; c2 < 4; ++c2) {
-      const float cv = tf_grad(A.cmap, A.cm_n, c2, raw.w, sl, k0, k1, f);
-      float a_c;
-      if (c2 == 3) {
-        a_c = a_a3 * 3.0f * cv * cv;
-      } else {
-        const float c22 = c2 == 0 ? k.y : (c2 == 1 ? k.z : k.w);
-        a_c = cv > 0.0f ? word(e, 6 + c2) * 2.2f * (c22 / cv) : 0.0f; // d(c^2.2)/dc = 2.2 c^2.2 / c
-      }
-      add_cm(4 * k0 + c2, a_c * (1.0f - f));
-      add_cm(4 * k1 + c2, a_c * f);
-      a_attr += a_c * sl;
-    }
-    atomicAdd(&D.g_attr[p], a_attr);
+    for (uint32_t w = 0; w < 5; ++w) atomicAdd(&D.acc5[5 * (uint64_t)p + w], word(e, 4 + w));
   };
   // the cache entry of particle p (words strided by kDBlock: use word())
   auto slot = [&](uint32_t p) -> uint32_t {
@@ -330,7 +436,7 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
         const float4 V = D.path[((uint64_t)py * A.W + px) * D.spp + s];
         total = g[0] * V.x + g[1] * V.y + g[2] * V.z;
         have = true;
-      } else march(D, px, py, s, [&](float, float, float, const DPay &pl) {
+      } else march(D, W, px, py, s, [&](float, float, float, const DPay &pl, auto) {
         float sl, f, dF;
         uint32_t j0, j1;
         float Dv = tf_grad(A.dmap, A.dm_n, 0, pl.rho, sl, j0, j1, f);
@@ -345,6 +451,7 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
         prev[1] = c[1];
         prev[2] = c[2];
         have = true;
+        return Fcdf(L, dF) < 1.0f;
       });
       if (!have) continue;
       if (!D.path) {
@@ -355,7 +462,7 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
       // pass 2: dC/dD_j = total - prefix, and the chain rule at every point
       float Lp = 0.0f, prefix = 0.0f;
       have = false;
-      march(D, px, py, s, [&](float x, float y, float z, const DPay &pl) {
+      march(D, W, px, py, s, [&](float x, float y, float z, const DPay &pl, auto cover) {
         float sD, fD, dF;
         uint32_t j0, j1;
         float Dv = tf_grad(A.dmap, A.dm_n, 0, pl.rho, sD, j0, j1, fD);
@@ -396,32 +503,86 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
           word(e, 7) += a_S[1] * w;
           word(e, 8) += a_S[2] * w;
         };
-        if (pl.count <= kRec) { // the covering particles the march kept
-          for (uint32_t r = 0; r < pl.count; ++r) {
-            const uint32_t p = rec[(2 * r) * kDBlock];
-            scatter(p, A.pr[p], __uint_as_float(rec[(2 * r + 1) * kDBlock]));
-          }
-        } else {
-          for_covering(A, x, y, z, scatter);
-        }
+        cover(scatter);
         prev[0] = c[0];
         prev[1] = c[1];
         prev[2] = c[2];
         have = true;
-      }, rec);
+        return F1 < 1.0f; // later points: dC/dD and the step weights are 0
+      });
       for (uint32_t e = 0; e < kCache; ++e) flush(e);
     }
   }
   __syncthreads();
-  if (lds_cm)
-    for (uint32_t j = threadIdx.x; j < 4 * A.cm_n; j += kDBlock)
-      if (s_gcm[j] != 0.0f) atomicAdd(&D.g_cm[j], s_gcm[j]);
   if (lds_dm)
     for (uint32_t j = threadIdx.x; j < A.dm_n; j += kDBlock)
       if (s_gdm[j] != 0.0f) atomicAdd(&D.g_dm[j], s_gdm[j]);
+}
+
+// The chain rule of each particle's summed adjoints through its radiusmap
+// and colormap lookups: r = radius * radiusmap(w), alpha^3 and rgb^2.2 from
+// colormap(w).  Texel gradients are summed per block in LDS first.
+constexpr int kCBlock = 256;
+__global__ __launch_bounds__(kCBlock) void diff_chain(DiffArgs D) {
+  const RenderArgs &A = D.A;
+  __shared__ float s_gcm[4 * kTexMax], s_grm[kTexMax];
+  for (uint32_t j = threadIdx.x; j < 4 * kTexMax; j += kCBlock) s_gcm[j] = 0.0f;
+  for (uint32_t j = threadIdx.x; j < kTexMax; j += kCBlock) s_grm[j] = 0.0f;
+  __syncthreads();
+  const bool lds_cm = A.cm_n <= kTexMax, lds_rm = D.rm_n <= kTexMax;
+  auto add_cm = [&](uint32_t j, float v) {
+    if (lds_cm) atomicAdd(&s_gcm[j], v);
+    else atomicAdd(&D.g_cm[j], v);
+  };
+  auto add_rm = [&](uint32_t j, float v) {
+    if (lds_rm) atomicAdd(&s_grm[j], v);
+    else atomicAdd(&D.g_rm[j], v);
+  };
+  const uint32_t p = blockIdx.x * kCBlock + threadIdx.x;
+  if (p < A.numParticles) {
+    const float *a = D.acc5 + 5 * (uint64_t)p;
+    const float a_r = a[0], a_a3 = a[1];
+    if (a_r != 0.0f || a_a3 != 0.0f || a[2] != 0.0f || a[3] != 0.0f || a[4] != 0.0f) {
+      const float4 raw = D.particles[p];
+      const float4 k = A.pc[p];
+      float sl, f, a_attr = 0.0f;
+      uint32_t k0, k1;
+      // r = radius * radiusmap(attribute)
+      const float rmv = tf_grad(D.rmap, D.rm_n, 0, raw.w, sl, k0, k1, f);
+      D.g_rad[p] += a_r * rmv;
+      const float a_rm = a_r * D.radii[p];
+      add_rm(k0, a_rm * (1.0f - f));
+      add_rm(k1, a_rm * f);
+      a_attr += a_rm * sl;
+      // alpha^3 and colour^2.2 from colormap(attribute)
+      for (int c2 = 0; c2 < 4; ++c2) {
+        const float cv = tf_grad(A.cmap, A.cm_n, c2, raw.w, sl, k0, k1, f);
+        float a_c;
+        if (c2 == 3) {
+          a_c = a_a3 * 3.0f * cv * cv;
+        } else {
+          const float c22 = c2 == 0 ? k.y : (c2 == 1 ? k.z : k.w);
+          a_c = cv > 0.0f ? a[2 + c2] * 2.2f * (c22 / cv) : 0.0f; // d(c^2.2)/dc = 2.2 c^2.2 / c
+        }
+        add_cm(4 * k0 + c2, a_c * (1.0f - f));
+        add_cm(4 * k1 + c2, a_c * f);
+        a_attr += a_c * sl;
+      }
+      D.g_attr[p] += a_attr;
+    }
+  }
+  __syncthreads();
+  if (lds_cm)
+    for (uint32_t j = threadIdx.x; j < 4 * A.cm_n; j += kCBlock)
+      if (s_gcm[j] != 0.0f) atomicAdd(&D.g_cm[j], s_gcm[j]);
   if (lds_rm)
-    for (uint32_t j = threadIdx.x; j < D.rm_n; j += kDBlock)
+    for (uint32_t j = threadIdx.x; j < D.rm_n; j += kCBlock)
       if (s_grm[j] != 0.0f) atomicAdd(&D.g_rm[j], s_grm[j]);
+}
+
+__global__ __launch_bounds__(256) void diff_zero(float *p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
 }
 
 } // namespace
@@ -446,8 +607,9 @@ static int diff_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const aar
                 "aarbf_diff: the BVH has fewer leaves than numParticles / particlesPerLeaf");
   hipSetDevice(ctx->device);
   const uint32_t n = pc->numParticles;
-  float4 *pr = (float4 *)scratch(ctx, 32 * (size_t)(n ? n : 1));
+  float4 *pr = (float4 *)scratch(ctx, (32 + 20) * (size_t)(n ? n : 1));
   if (!pr) return set_error(ctx, AARBF_ERR_OUT_OF_MEMORY, "aarbf_diff: payload");
+  D.acc5 = (float *)(pr + 2 * (size_t)(n ? n : 1));
   float4 *pcol = pr + (n ? n : 1);
   if (n)
     diff_bake<<<(n + 255) / 256, 256, 0, ctx->stream>>>((const float4 *)pc->particles, (const float *)pc->radii, n,
@@ -476,6 +638,7 @@ static int diff_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const aar
   A.dmap = dm->table;
   A.dm_n = dm->n;
   A.bvh = make_view(*bvh);
+  if (bvh->num_levels > kStackLevels) A.bvh.groups = nullptr; // the march then queries every point
   D.particles = (const float4 *)pc->particles;
   D.radii = (const float *)pc->radii;
   D.rmap = rm->table;
@@ -519,8 +682,13 @@ int aarbf_diff_backward(aarbf_context *ctx, const aarbf_raygen_data *rg, const a
   D.g_dm = (float *)g_densitymap;
   D.g_rm = (float *)g_radiusmap;
   dim3 grid((W + 7) / 8, (H + 7) / 8);
+  const uint64_t n5 = 5 * (uint64_t)D.A.numParticles;
+  if (n5) diff_zero<<<(uint32_t)((n5 + 255) / 256), 256, 0, ctx->stream>>>(D.acc5, n5);
   diff_adjoint<<<grid, kDBlock, 0, ctx->stream>>>(D);
   AARBF_CHECK_LAUNCH(ctx, "diff_adjoint");
+  if (D.A.numParticles)
+    diff_chain<<<(D.A.numParticles + kCBlock - 1) / kCBlock, kCBlock, 0, ctx->stream>>>(D);
+  AARBF_CHECK_LAUNCH(ctx, "diff_chain");
   return AARBF_OK;
 }
 
