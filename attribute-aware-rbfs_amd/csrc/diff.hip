@@ -303,7 +303,27 @@ __device__ inline void march(const DiffArgs &D, const DSeg &W, int px, int py, u
     // the covering particles, in one fixed order for the payload and cover()
     auto cover = [&](auto g) {
       if (dense) {
-        for_covering(A, x, y, z, g);
+        if (!seg_ok) {
+          for_covering(A, x, y, z, g);
+          return;
+        }
+        // point query over the quantised tree, leaves in ascending order
+        Counters c = {};
+        walk_q<kDBlock, false>(
+            A, W.stk, [&](const float4 &lo, const float4 &hi) { return inside(lo, hi, x, y, z); },
+            [&](uint32_t base, uint32_t mask) {
+              for (; mask; mask &= mask - 1u) {
+                const uint32_t leaf = base + (uint32_t)__ffs(mask) - 1u;
+                const uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
+                for (uint32_t p = p0; p < p1; ++p) {
+                  const float4 q = A.pr[p];
+                  const float dd = aa_dist3(q.x, q.y, q.z, x, y, z);
+                  if (dd < q.w) g(p, q, dd);
+                }
+              }
+              return false;
+            },
+            c);
         return;
       }
       for (uint32_t b = act; b; b &= b - 1u) {
