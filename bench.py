@@ -49,7 +49,7 @@ def main() -> int:
     ap.add_argument("--step-cap", type=int, default=4096)
     ap.add_argument("--graph-rounds", type=int, default=0, help="rounds per captured hipGraph (0 = host-driven)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
-    ap.add_argument("--diff-views", type=int, default=4,
+    ap.add_argument("--diff-views", type=int, default=2,
                     help="views of the differentiable-rendering leg timed (0 = skip the leg)")
     ap.add_argument("--diff-particles", type=int, default=1048576)
     ap.add_argument("--batch-views", type=int, default=0,
