@@ -10,6 +10,7 @@ library's HIP kernels -- there is no host fallback.
 from __future__ import annotations
 
 import ctypes as C
+import functools
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -81,6 +82,22 @@ def camera_basis(look_from, look_at, look_up, cos_fovy: float, width: int, heigh
     return Camera(lf.copy(), d00, ddu, ddv)
 
 
+def _ordered(fn):
+    """Orders a call that reads or writes torch tensors with torch's own
+    work.  On the context's own stream torch's pending work finishes before
+    the library starts and the library's before the call returns; a context
+    given a torch stream (set_stream) is ordered by that stream instead."""
+    @functools.wraps(fn)
+    def wrap(self, *a, **k):
+        if self._own_stream:
+            _torch().cuda.current_stream(self.device).synchronize()
+        r = fn(self, *a, **k)
+        if self._own_stream:
+            self.sync()
+        return r
+    return wrap
+
+
 class Context:
     """One aarbf_context: a device, a HIP stream and the handles made on it."""
 
@@ -92,6 +109,7 @@ class Context:
             raise AarbfError(st, "aarbf_context_create: " + self.lib.aarbf_status_string(st).decode())
         self.h = h
         self.device = device
+        self._own_stream = True
         if stream is not None:
             self.set_stream(stream)
 
@@ -118,6 +136,7 @@ class Context:
         if not ptr:
             raise ValueError("pass a non-default stream (torch.cuda.Stream()); 0 selects the context's own stream")
         self.check(self.lib.aarbf_set_stream(self.h, C.c_void_p(ptr)), "aarbf_set_stream")
+        self._own_stream = False
 
     def sync(self):
         self.check(self.lib.aarbf_sync(self.h), "aarbf_sync")
@@ -140,6 +159,7 @@ class Context:
         """Particle payload format of the renders (PAYLOAD_FP32 / PAYLOAD_FP16)."""
         self.check(self.lib.aarbf_set_payload(self.h, int(fmt)), "aarbf_set_payload")
 
+    @_ordered
     def payload_read(self, n: int):
         """Decoded payload of the last render: float32 [n, 8] (x, y, z, r, a^3, rgb^2.2)."""
         torch = _torch()
@@ -177,12 +197,14 @@ class Context:
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.TIMING_KERNELS)}
 
     # -- pre-process (hostCode.cpp:183-250) ---------------------------------
+    @_ordered
     def spacetime_aabb(self, particles, aabb=None) -> np.ndarray:
         a = np.array(aabb if aabb is not None else [1e38, 1e38, 1e38, -1e38, -1e38, -1e38], dtype=np.float32)
         self.check(self.lib.aarbf_spacetime_aabb(self.h, particles.data_ptr(), particles.shape[0], _fptr(a)),
                    "aarbf_spacetime_aabb")
         return a
 
+    @_ordered
     def hilbert_codes(self, particles, aabb):
         torch = _torch()
         a = np.asarray(aabb, dtype=np.float32)
@@ -191,6 +213,7 @@ class Context:
                                                 codes.data_ptr()), "aarbf_hilbert_codes")
         return codes
 
+    @_ordered
     def sort_particles(self, particles, codes, want_perm: bool = True):
         torch = _torch()
         n = particles.shape[0]
@@ -200,12 +223,14 @@ class Context:
                    "aarbf_sort_particles")
         return perm
 
+    @_ordered
     def attribute_range(self, particles, rng=None) -> np.ndarray:
         r = np.array(rng if rng is not None else [1e20, -1e20], dtype=np.float32)
         self.check(self.lib.aarbf_attribute_range(self.h, particles.data_ptr(), particles.shape[0], _fptr(r)),
                    "aarbf_attribute_range")
         return r
 
+    @_ordered
     def normalize_attributes(self, particles, rng):
         r = np.asarray(rng, dtype=np.float32)
         self.check(self.lib.aarbf_normalize_attributes(self.h, particles.data_ptr(), particles.shape[0],
@@ -237,6 +262,7 @@ class Context:
         return h.value
 
     # -- bounds + BVH --------------------------------------------------------------
+    @_ordered
     def gen_bounds(self, particles, ppl: int, rbf_radius: float, radiusmap: int, radii=None, aabbs=None,
                    num_particles: int | None = None):
         torch = _torch()
@@ -249,12 +275,14 @@ class Context:
         self.check(self.lib.aarbf_gen_bounds(self.h, C.byref(bc)), "aarbf_gen_bounds")
         return aabbs
 
+    @_ordered
     def bvh_build(self, aabbs, branching: int = 8) -> int:
         h = C.c_uint64()
         self.check(self.lib.aarbf_bvh_build(self.h, aabbs.data_ptr(), aabbs.numel() // 6, branching,
                                             C.byref(h)), "aarbf_bvh_build")
         return h.value
 
+    @_ordered
     def bvh_refit(self, bvh: int, aabbs):
         self.check(self.lib.aarbf_bvh_refit(self.h, bvh, aabbs.data_ptr()), "aarbf_bvh_refit")
 
@@ -282,10 +310,12 @@ class Context:
         return out
 
     # -- render --------------------------------------------------------------------
+    @_ordered
     def render(self, rg: RayGenData, pc: PushConstants, width: int, height: int, spp: int = 1):
         self.check(self.lib.aarbf_render_rbf_spp(self.h, C.byref(rg), C.byref(pc), width, height, spp),
                    "aarbf_render_rbf_spp")
 
+    @_ordered
     def render_batch(self, requests):
         """Renders views of one scene in one schedule (aarbf_render_batch);
         `requests`: sequence of (rg, pc, width, height, spp)."""
@@ -301,6 +331,7 @@ class Context:
             arr[i].width, arr[i].height, arr[i].spp = int(w), int(h), int(spp)
         return arr
 
+    @_ordered
     def autotune(self, requests, reps: int = 1) -> dict:
         """Schedule autotuner (aarbf_autotune) over the views `requests`;
         keeps the fastest setting in the context and returns it."""
@@ -317,6 +348,7 @@ class Context:
                    "aarbf_batch_latency")
         return out
 
+    @_ordered
     def diff_forward(self, rg: RayGenData, pc: PushConstants, width: int, height: int, spp: int, image, path=None):
         """Expected stochastic-marching image into `image` (float32 [H, W, 4]);
         `path` (float32 [H, W, spp, 4]) optionally keeps what the backward needs."""
@@ -324,6 +356,7 @@ class Context:
                                                image.data_ptr(), path.data_ptr() if path is not None else 0),
                    "aarbf_diff_forward")
 
+    @_ordered
     def diff_backward(self, rg: RayGenData, pc: PushConstants, width: int, height: int, spp: int, grad_image,
                       g_pos, g_radii, g_attr, g_colormap, g_densitymap, g_radiusmap, path=None):
         """Adds the adjoint of diff_forward into the (zeroed) gradient tensors."""
@@ -333,10 +366,12 @@ class Context:
                                                 g_attr.data_ptr(), g_colormap.data_ptr(), g_densitymap.data_ptr(),
                                                 g_radiusmap.data_ptr()), "aarbf_diff_backward")
 
+    @_ordered
     def render_splat(self, rg: RayGenData, pc: PushConstants, width: int, height: int):
         self.check(self.lib.aarbf_render_splat(self.h, C.byref(rg), C.byref(pc), width, height),
                    "aarbf_render_splat")
 
+    @_ordered
     def taa_resolve(self, tc: TAAConstants):
         self.check(self.lib.aarbf_taa_resolve(self.h, C.byref(tc)), "aarbf_taa_resolve")
 
