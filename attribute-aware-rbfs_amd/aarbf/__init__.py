@@ -348,6 +348,13 @@ class Context:
                    "aarbf_batch_latency")
         return out
 
+    DIFF_MARCHING = 0
+    DIFF_RATIO_TRACKING = 1
+
+    def set_diff_estimator(self, estimator: int):
+        """0: expected stochastic marching; 1: ratio tracking of delta tracking."""
+        self.check(self.lib.aarbf_set_diff_estimator(self.h, estimator), "aarbf_set_diff_estimator")
+
     @_ordered
     def diff_forward(self, rg: RayGenData, pc: PushConstants, width: int, height: int, spp: int, image, path=None):
         """Expected stochastic-marching image into `image` (float32 [H, W, 4]);

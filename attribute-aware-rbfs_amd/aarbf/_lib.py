@@ -139,6 +139,7 @@ SIGNATURES = {
     "aarbf_render_batch": (C.c_int, [P, C.POINTER(Request), u32]),
     "aarbf_batch_latency": (C.c_int, [P, C.POINTER(C.c_double), u32]),
     "aarbf_autotune": (C.c_int, [P, C.POINTER(Request), u32, u32, C.POINTER(TuneResult)]),
+    "aarbf_set_diff_estimator": (C.c_int, [P, C.c_int]),
     "aarbf_taa_resolve": (C.c_int, [P, C.POINTER(TAAConstants)]),
     "aarbf_abi_layout": (u32, [C.POINTER(u32), u32]),
 }

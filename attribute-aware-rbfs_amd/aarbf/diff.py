@@ -27,10 +27,11 @@ class _Render:
     """Per-call device state: sorted particles, tables, BVH."""
 
     def __init__(self, ctx, camera, W, H, unit, spp, frame_id, rbf_radius, ppl, branching, positions, attributes,
-                 radii, colormap, densitymap, radiusmap):
+                 radii, colormap, densitymap, radiusmap, estimator=0):
         torch = _torch()
         from . import PushConstants, RayGenData, Int2, _f3
         self.ctx, self.W, self.H, self.spp = ctx, W, H, spp
+        self.estimator = estimator
         # The library works on its own stream: torch's work is finished before
         # each library call reads a tensor, and the library's before torch reads.
         p4 = torch.cat([positions.detach().float(), attributes.detach().float()[:, None]], 1).contiguous()
@@ -69,6 +70,7 @@ class _Render:
         self.rg = rg
 
     def forward(self):
+        self.ctx.set_diff_estimator(self.estimator)
         self.ctx.diff_forward(self.rg, self.pc, self.W, self.H, self.spp, self.img, self.path)
         self.ctx.sync()
         return self.img[..., :3].clone()
@@ -86,6 +88,7 @@ class _Render:
         gdm = torch.zeros(self.n_tex[1], dtype=torch.float32, device=dev)
         grm = torch.zeros(self.n_tex[2], dtype=torch.float32, device=dev)
         torch.cuda.synchronize()
+        self.ctx.set_diff_estimator(self.estimator)
         self.ctx.diff_backward(self.rg, self.pc, self.W, self.H, self.spp, g, gp, gr, ga, gcm, gdm, grm,
                                path=self.path)
         self.ctx.sync()
@@ -110,7 +113,8 @@ class _Fn:
         class ParticleRender(torch.autograd.Function):
             @staticmethod
             def forward(fctx, settings, positions, attributes, radii, colormap, densitymap, radiusmap):
-                r = _Render(*settings, positions, attributes, radii, colormap, densitymap, radiusmap)
+                r = _Render(*settings[:-1], positions, attributes, radii, colormap, densitymap, radiusmap,
+                            estimator=settings[-1])
                 img = r.forward()
                 fctx.r = r
                 return img
@@ -129,8 +133,10 @@ class _Fn:
 
 class DiffRenderer:
     def __init__(self, ctx, camera, width: int, height: int, unit: float, spp: int = 4, frame_id: int = 0,
-                 rbf_radius: float | None = None, ppl: int = 1, branching: int = 8):
-        self.settings = (ctx, camera, width, height, unit, spp, frame_id, rbf_radius, ppl, branching)
+                 rbf_radius: float | None = None, ppl: int = 1, branching: int = 8, estimator: int = 0):
+        """estimator 0: expected stochastic marching; 1: ratio tracking of
+        delta tracking (include/aarbf.h, aarbf_set_diff_estimator)."""
+        self.settings = (ctx, camera, width, height, unit, spp, frame_id, rbf_radius, ppl, branching, estimator)
         self.fn = _Fn.build()
 
     def __call__(self, positions, attributes, radii, colormap, densitymap, radiusmap):

@@ -1,9 +1,12 @@
-// diff.hip -- the differentiable renderer: the expected stochastic-marching
-// image and its adjoint with respect to particle positions, radii,
-// attributes and the colormap / densitymap / radiusmap texels.  No
-// counterpart in the reference; its forward is the expectation of
+// diff.hip -- the differentiable renderer: an expected image and its
+// adjoint with respect to particle positions, radii, attributes and the
+// colormap / densitymap / radiusmap texels.  No counterpart in the
+// reference.  Estimator 0 is the expectation of
 // ParticleTracker::stochasticMarching (deviceCodeRBF.hlsl:67-139, primary
-// ray, ambient 1) over the blue-noise threshold.
+// ray, ambient 1) over the blue-noise threshold; estimator 1 is ratio
+// tracking, the expectation of ParticleTracker::stochasticTracking
+// (':141-203') over its accept draws at the tracker's tentative points,
+// C = sum_i T_{i-1} D_i c_i with T_i = T_{i-1} (1 - D_i).
 //
 // A sample marches t_k = tenter + unit * (r1 + k) exactly as the marching
 // tracker does; at the points some particle covers it adds D_k =
@@ -47,7 +50,11 @@ struct DiffArgs {
   const float4 *gimg;      // dLoss/dimage (adjoint)
   float *g_pos, *g_rad, *g_attr, *g_cm, *g_dm, *g_rm;
   float *acc5;             // adjoint: per particle (r, alpha^3, rgb^2.2) sums, before the TF chain rule
+  uint32_t ratio;          // estimator: 0 expected marching, 1 ratio tracking
 };
+
+// Ratio tracking ends a sample after the first point with T < 2^-20.
+constexpr float kRatioTmin = 9.5367431640625e-07f;
 
 __global__ __launch_bounds__(256) void diff_bake(const float4 *__restrict__ particles, const float *__restrict__ radii,
                                                  uint32_t n, const float4 *__restrict__ rmap, uint32_t rm_n,
@@ -264,9 +271,23 @@ __device__ inline void march(const DiffArgs &D, const DSeg &W, int px, int py, u
     t_exit = AA_INF_F;
     t_enter = n ? W.tin[0] : AA_INF_F;
   };
+  // marching: t_k = t_{k-1} + unit from tenter + unit r1; ratio tracking:
+  // the delta tracker's free flights from t = 0, one LCG draw each after
+  // the sample's three set-up draws (r0, r1, r2)
+  const bool ratio = D.ratio != 0;
   float t = te + A.unit * r1;
+  if (ratio) {
+    aa_lcg_next(&rng);
+    aa_lcg_next(&rng);
+    t = 0.0f;
+  }
   for (uint32_t i = 0; i < A.max_depth; ++i) {
-    t = t + A.unit;
+    if (ratio) {
+      const float xi = aa_lcg_next(&rng);
+      t = t - aa_log_step(1.0f - xi) * A.unit;
+    } else {
+      t = t + A.unit;
+    }
     if (t >= tx) return;
     bool dense = !seg_ok || t < dense_end;
     if (!dense && !(t < seg_end)) {
@@ -369,6 +390,7 @@ __global__ __launch_bounds__(kDBlock) void diff_forward(DiffArgs D) {
   float acc[3] = {0.0f, 0.0f, 0.0f};
   for (uint32_t s = 0; s < D.spp; ++s) {
     float L = 0.0f, C[3] = {0.0f, 0.0f, 0.0f}, V[3] = {0.0f, 0.0f, 0.0f}, prev[3] = {0.0f, 0.0f, 0.0f};
+    float T = 1.0f;
     bool have = false;
     march(D, W, px, py, s, [&](float, float, float, const DPay &pl, auto) {
       float sl, f, dF;
@@ -376,6 +398,11 @@ __global__ __launch_bounds__(kDBlock) void diff_forward(DiffArgs D) {
       float Dv = tf_grad(A.dmap, A.dm_n, 0, pl.rho, sl, j0, j1, f);
       float c[3];
       colour(pl, c);
+      if (D.ratio) { // C = sum_i T_{i-1} D_i c_i
+        for (int k = 0; k < 3; ++k) C[k] += (T * Dv) * c[k];
+        T = T * (1.0f - Dv);
+        return T >= kRatioTmin;
+      }
       float F0 = Fcdf(L, dF);
       // the previous point's dC/dL term, now that c_k is known
       if (have)
@@ -389,7 +416,9 @@ __global__ __launch_bounds__(kDBlock) void diff_forward(DiffArgs D) {
       }
       return F1 < 1.0f; // once F saturates every later term is 0
     });
-    {
+    if (D.ratio) { // the adjoint's replay starts from the sample's colour
+      for (int k = 0; k < 3; ++k) V[k] = C[k];
+    } else {
       float dF;
       Fcdf(L, dF);
       for (int k = 0; k < 3; ++k) V[k] += dF * prev[k];
@@ -450,7 +479,7 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
       for (uint32_t e = 0; e < kCache; ++e) cache[e * kSlot * kDBlock] = 0xffffffffu;
       // pass 1: the total of dC/dL_k over the path (kept by the forward when
       // it had a path buffer)
-      float L = 0.0f, total = 0.0f, prev[3] = {0, 0, 0};
+      float L = 0.0f, total = 0.0f, prev[3] = {0, 0, 0}, T1 = 1.0f;
       bool have = false;
       if (D.path) {
         const float4 V = D.path[((uint64_t)py * A.W + px) * D.spp + s];
@@ -462,6 +491,12 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
         float Dv = tf_grad(A.dmap, A.dm_n, 0, pl.rho, sl, j0, j1, f);
         float c[3];
         colour(pl, c);
+        if (D.ratio) { // total = g . C
+          total += (T1 * Dv) * (g[0] * c[0] + g[1] * c[1] + g[2] * c[2]);
+          T1 = T1 * (1.0f - Dv);
+          have = true;
+          return T1 >= kRatioTmin;
+        }
         if (have) {
           Fcdf(L, dF);
           total += dF * (g[0] * (prev[0] - c[0]) + g[1] * (prev[1] - c[1]) + g[2] * (prev[2] - c[2]));
@@ -474,13 +509,17 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
         return Fcdf(L, dF) < 1.0f;
       });
       if (!have) continue;
-      if (!D.path) {
+      if (!D.path && !D.ratio) {
         float dF;
         Fcdf(L, dF);
         total += dF * (g[0] * prev[0] + g[1] * prev[1] + g[2] * prev[2]);
       }
-      // pass 2: dC/dD_j = total - prefix, and the chain rule at every point
-      float Lp = 0.0f, prefix = 0.0f;
+      // pass 2: the chain rule at every point.  Marching: dC/dD_j = total -
+      // prefix.  Ratio tracking: with q the g-weighted colour still to come
+      // (q = g . C before the first point), q_j = (q_{j-1} - D_j g.c_j) /
+      // (1 - D_j) and dC/dD_j = T_{j-1} (g.c_j - q_j); q_j = 0 at the point
+      // that ends the sample.
+      float Lp = 0.0f, prefix = 0.0f, T = 1.0f, q = total;
       have = false;
       march(D, W, px, py, s, [&](float x, float y, float z, const DPay &pl, auto cover) {
         float sD, fD, dF;
@@ -488,15 +527,29 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
         float Dv = tf_grad(A.dmap, A.dm_n, 0, pl.rho, sD, j0, j1, fD);
         float c[3];
         colour(pl, c);
-        if (have) {
-          Fcdf(Lp, dF);
-          prefix += dF * (g[0] * (prev[0] - c[0]) + g[1] * (prev[1] - c[1]) + g[2] * (prev[2] - c[2]));
+        float aD, pj;
+        bool more;
+        if (D.ratio) {
+          const float gc = g[0] * c[0] + g[1] * c[1] + g[2] * c[2];
+          const float Tn = T * (1.0f - Dv);
+          more = Tn >= kRatioTmin;
+          const float qn = more ? (q - Dv * gc) / (1.0f - Dv) : 0.0f;
+          aD = T * (gc - qn);
+          pj = T * Dv;
+          T = Tn;
+          q = qn;
+        } else {
+          if (have) {
+            Fcdf(Lp, dF);
+            prefix += dF * (g[0] * (prev[0] - c[0]) + g[1] * (prev[1] - c[1]) + g[2] * (prev[2] - c[2]));
+          }
+          aD = total - prefix;
+          float F0 = Fcdf(Lp, dF);
+          Lp += Dv;
+          float F1 = Fcdf(Lp, dF);
+          pj = F1 - F0;
+          more = F1 < 1.0f; // later points: dC/dD and the step weights are 0
         }
-        const float aD = total - prefix;
-        float F0 = Fcdf(Lp, dF);
-        Lp += Dv;
-        float F1 = Fcdf(Lp, dF);
-        const float pj = F1 - F0;
         add_dm(j0, aD * (1.0f - fD));
         add_dm(j1, aD * fD);
         float a_rho = aD * sD, a_S[3];
@@ -528,7 +581,7 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
         prev[1] = c[1];
         prev[2] = c[2];
         have = true;
-        return F1 < 1.0f; // later points: dC/dD and the step weights are 0
+        return more;
       });
       for (uint32_t e = 0; e < kCache; ++e) flush(e);
     }
@@ -664,10 +717,19 @@ static int diff_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const aar
   D.rmap = rm->table;
   D.rm_n = rm->n;
   D.spp = spp;
+  D.ratio = ctx->diff_estimator == 1 ? 1u : 0u;
   return AARBF_OK;
 }
 
 extern "C" {
+
+int aarbf_set_diff_estimator(aarbf_context *ctx, int estimator) {
+  if (!ctx) return AARBF_ERR_INVALID_ARGUMENT;
+  AARBF_REQUIRE(ctx, estimator == 0 || estimator == 1,
+                "aarbf_set_diff_estimator: 0 (expected marching) or 1 (ratio tracking)");
+  ctx->diff_estimator = estimator;
+  return AARBF_OK;
+}
 
 int aarbf_diff_forward(aarbf_context *ctx, const aarbf_raygen_data *rg, const aarbf_push_constants *pc, int W, int H,
                        uint32_t spp, uint64_t image, uint64_t path) {

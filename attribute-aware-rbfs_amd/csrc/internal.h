@@ -76,6 +76,7 @@ struct aarbf_context {
   uint64_t *d_stats = nullptr; // 8 counters
   uint32_t *d_counter = nullptr; // work counter of the persistent render kernel
   int render_mode = 0;           // 0 wavefront gather/step, 1 per-pixel point queries, 2 majorant tracking
+  int diff_estimator = 0;        // differentiable renderer: 0 expected marching, 1 ratio tracking
   int cus = 0;                   // compute units of the device
   int tune_step_waves = 0;       // occupancy target of the step kernel (0 = default, 4 waves per SIMD)
   int tune_step_cap = 4096;      // tracker steps per ray per wavefront round

@@ -367,6 +367,16 @@ int aarbf_diff_backward(aarbf_context *ctx, const aarbf_raygen_data *rg, const a
                         int width, int height, uint32_t spp, uint64_t path, uint64_t grad_image,
                         uint64_t g_pos, uint64_t g_radii, uint64_t g_attr, uint64_t g_colormap,
                         uint64_t g_densitymap, uint64_t g_radiusmap);
+/* Estimator of the differentiable renderer (default 0): 0 the expectation
+ * of stochastic marching over its threshold, as described above; 1 ratio
+ * tracking, the expectation of delta tracking (deviceCodeRBF.hlsl:141-203,
+ * primary ray, ambient 1) over its accept draws: tentative points at the
+ * tracker's free flights t_i = t_{i-1} - ln(1 - xi_i) unit from t = 0 (one
+ * LCG draw each after the sample's three set-up draws), pixel = mean over
+ * samples of sum_i T_{i-1} D_i c_i, T_i = T_{i-1} (1 - D_i), each sample
+ * ending after the first point with T_i < 2^-20.  path then holds each
+ * sample's colour.  No counterpart in the reference. */
+int aarbf_set_diff_estimator(aarbf_context *ctx, int estimator);
 /* CompositeGui's TAA resolve and BGRA8 packing (deviceCodeCommon.hlsl:67-115)
  * plus the taa -> taaPrev history copy (hostCode.cpp:819-830). */
 int aarbf_taa_resolve(aarbf_context *ctx, const aarbf_taa_constants *tc);

@@ -359,15 +359,17 @@ class OrcDiff(C.Structure):
         ("spp", C.c_uint32), ("frameID", C.c_uint32), ("max_depth", C.c_uint32),
         ("unit", C.c_float), ("rbfRadius", C.c_float),
         ("aabb", C.c_float * 6),
+        ("estimator", C.c_uint32),
     ]
 
 
 class Diff:
-    """FP64 expected-marching image and its adjoint (oracle/adjoint.c).
+    """FP64 differentiable image (estimator 0: expected stochastic marching,
+    1: ratio tracking of delta tracking) and its adjoint (oracle/adjoint.c).
     pos [N,3], attr [N], radii [N], TF tables [n,4]; all float64."""
 
     def __init__(self, pos, attr, radii, colormap, densitymap, radiusmap, camera, width, height, unit, rbf_radius,
-                 aabb, spp=1, frame_id=0, max_depth=0):
+                 aabb, spp=1, frame_id=0, max_depth=0, estimator=0):
         self.keep = []
 
         def arr(a):
@@ -387,6 +389,7 @@ class Diff:
         p.width, p.height, p.spp, p.frameID, p.max_depth = width, height, spp, frame_id, max_depth
         p.unit, p.rbfRadius = unit, rbf_radius
         p.aabb[:] = [float(x) for x in aabb[:6]]
+        p.estimator = estimator
         self.p = p
         self.sizes = (len(attr), len(cm), len(dm), len(rm))
         L = lib()

@@ -25,6 +25,19 @@
  * pushes it, with dC/dc_j = F(L_j) - F(L_{j-1}), through the densitymap,
  * the colour mean and each covering particle's RBF weight, radius,
  * colormap and radiusmap lookups.
+ *
+ * Estimator 1, ratio tracking: the expectation of the reference's delta
+ * tracking (deviceCodeRBF.hlsl:141-203, primary ray, ambient 1) over its
+ * accept draws.  Tentative points follow the tracker's free flights,
+ * t_i = t_{i-1} - ln(1 - xi_i) unit from t = 0 (one LCG draw each, after
+ * the sample's three set-up draws), and point i is accepted with
+ * probability D_i, so
+ *     C = sum_i T_{i-1} D_i c_i,   T_i = T_{i-1} (1 - D_i),
+ * the sum ending after the first point with T_i < 2^-20 (the remaining
+ * terms are below that fraction of the colour).  The adjoint here stores
+ * the sample's points and runs the reverse recursion
+ *     R_J = 0,  R_{j-1} = D_j c_j + (1 - D_j) R_j,
+ *     dC/dD_j = T_{j-1} (c_j - R_j),  dC/dc_j = T_{j-1} D_j.
  */
 #include <math.h>
 #include <stdint.h>
@@ -47,7 +60,10 @@ typedef struct orc_diff {
   uint32_t spp, frameID, max_depth;
   float unit, rbfRadius;
   float aabb[6];
+  uint32_t estimator; /* 0 expected marching, 1 ratio tracking */
 } orc_diff;
+
+#define ORC_RT_TMIN (1.0 / 1048576.0)
 
 /* linear texel filter with clamp-to-edge (aa_tf_coords in double): value,
  * slope d/du, and the two texels with their weights */
@@ -160,42 +176,63 @@ static void sample_ray(const orc_diff *p, int px, int py, uint32_t s, dray *r) {
 }
 
 /* visits the march points that some particle covers: fn(k-th point, x, payload) */
-typedef void (*point_fn)(const orc_diff *p, const dpart *pp, const double x[3], const dpay *pl, void *u);
+typedef int (*point_fn)(const orc_diff *p, const dpart *pp, const double x[3], const dpay *pl, void *u);
 
+/* estimator 0: the march points; estimator 1: the tracker's tentative
+ * points.  fn returns 0 to end the sample. */
 static void march(const orc_diff *p, const dpart *pp, int px, int py, uint32_t s, point_fn fn, void *u) {
   dray r;
   sample_ray(p, px, py, s, &r);
   if (!r.live) return;
   float t = r.t;
+  uint32_t rng = 0;
+  if (p->estimator == 1) {
+    uint32_t fid = (p->frameID + s) & 0xffffu;
+    rng = aa_rng_seed(fid, (uint32_t)px, (uint32_t)py, (uint32_t)p->width);
+    for (int k = 0; k < 3; ++k) aa_lcg_next(&rng); /* r0, r1, r2 of the sample's set-up */
+    t = 0.0f;
+  }
   uint32_t maxd = p->max_depth ? p->max_depth : 10000u;
   for (uint32_t i = 0; i < maxd; ++i) {
-    t = t + p->unit;
+    if (p->estimator == 1) {
+      float xi = aa_lcg_next(&rng);
+      t = t - aa_log_step(1.0f - xi) * p->unit;
+    } else {
+      t = t + p->unit;
+    }
     if (t >= r.tmax) return;
     float xf[3] = {r.o[0] + t * r.d[0], r.o[1] + t * r.d[1], r.o[2] + t * r.d[2]};
     double x[3] = {xf[0], xf[1], xf[2]};
     dpay pl;
     query(p, pp, x, &pl);
-    if (pl.count > 0 && pl.rho > 0.0) fn(p, pp, x, &pl, u);
+    if (pl.count > 0 && pl.rho > 0.0)
+      if (!fn(p, pp, x, &pl, u)) return;
   }
 }
 
 typedef struct {
-  double L, C[3];
+  double L, C[3], T;
 } fwd_state;
 
-static void fwd_point(const orc_diff *p, const dpart *pp, const double x[3], const dpay *pl, void *u) {
+static int fwd_point(const orc_diff *p, const dpart *pp, const double x[3], const dpay *pl, void *u) {
   (void)pp;
   (void)x;
   fwd_state *st = (fwd_state *)u;
   double s, f, dF;
   uint32_t j0, j1;
   double D = tfd(p->dm, p->dm_n, 0, pl->rho, &s, &j0, &j1, &f);
+  double c[3];
+  colour(pl, c);
+  if (p->estimator == 1) {
+    for (int k = 0; k < 3; ++k) st->C[k] += st->T * D * c[k];
+    st->T *= 1.0 - D;
+    return st->T >= ORC_RT_TMIN;
+  }
   double F0 = Fcdf(st->L, &dF);
   st->L += D;
   double F1 = Fcdf(st->L, &dF);
-  double c[3];
-  colour(pl, c);
   for (int k = 0; k < 3; ++k) st->C[k] += (F1 - F0) * c[k];
+  return 1;
 }
 
 static dpart *parts(const orc_diff *p) {
@@ -213,7 +250,7 @@ ORC_EXPORT int orc_diff_forward(const orc_diff *p, double *image) {
     for (int px = 0; px < p->width; ++px) {
       double acc[3] = {0, 0, 0};
       for (uint32_t s = 0; s < p->spp; ++s) {
-        fwd_state st = {0.0, {0, 0, 0}};
+        fwd_state st = {0.0, {0, 0, 0}, 1.0};
         march(p, pp, px, py, s, fwd_point, &st);
         for (int k = 0; k < 3; ++k) acc[k] += st.C[k];
       }
@@ -234,7 +271,7 @@ typedef struct {
   double *g_pos, *g_rad, *g_attr, *g_cm, *g_dm, *g_rm;
 } adj_state;
 
-static void pass1(const orc_diff *p, const dpart *pp, const double x[3], const dpay *pl, void *u) {
+static int pass1(const orc_diff *p, const dpart *pp, const double x[3], const dpay *pl, void *u) {
   (void)pp;
   (void)x;
   adj_state *st = (adj_state *)u;
@@ -252,34 +289,24 @@ static void pass1(const orc_diff *p, const dpart *pp, const double x[3], const d
   st->L += D;
   memcpy(st->prev_c, c, sizeof(c));
   st->have_prev = 1;
+  return 1;
 }
 
-static void pass2(const orc_diff *p, const dpart *pp, const double x[3], const dpay *pl, void *u) {
-  adj_state *st = (adj_state *)u;
-  double sD, fD, dF;
+/* pushes the adjoints of one point's density (aD = dC/dD) and colour
+ * (pj = dC/dc, the same for the three channels) through the densitymap, the
+ * colour mean and every covering particle */
+static void point_adjoint(const orc_diff *p, const dpart *pp, const double x[3], const dpay *pl, const double *g,
+                          double aD, double pj, double *g_pos, double *g_rad, double *g_attr, double *g_cm,
+                          double *g_dm, double *g_rm) {
+  double sD, fD;
   uint32_t j0, j1;
-  double D = tfd(p->dm, p->dm_n, 0, pl->rho, &sD, &j0, &j1, &fD);
-  double c[3];
-  colour(pl, c);
-  /* the term dC/dL_{j-1} belongs to the prefix once c_j is known */
-  if (st->have_prev) {
-    Fcdf(st->Lp, &dF);
-    double gd = 0;
-    for (int k = 0; k < 3; ++k) gd += st->g[k] * (st->cprev[k] - c[k]);
-    st->prefix += dF * gd;
-  }
-  double aD = st->total - st->prefix; /* dC/dD_j */
-  double F0 = Fcdf(st->Lp, &dF);
-  st->Lp += D;
-  double F1 = Fcdf(st->Lp, &dF);
-  double pj = F1 - F0;
-  /* densitymap texels and slope */
-  st->g_dm[j0] += aD * (1.0 - fD);
-  st->g_dm[j1] += aD * fD;
+  tfd(p->dm, p->dm_n, 0, pl->rho, &sD, &j0, &j1, &fD);
+  g_dm[j0] += aD * (1.0 - fD);
+  g_dm[j1] += aD * fD;
   double a_rho = aD * sD, a_S[3];
   /* colour mean c = (S / rho)^(1/2.2) */
   for (int k = 0; k < 3; ++k) {
-    double gc = st->g[k] * pj;
+    double gc = g[k] * pj;
     double m = pl->S[k] / pl->rho;
     double dcdm = m > 0.0 ? (1.0 / 2.2) * pow(m, 1.0 / 2.2 - 1.0) : 0.0;
     a_S[k] = gc * dcdm / pl->rho;
@@ -303,36 +330,111 @@ static void pass2(const orc_diff *p, const dpart *pp, const double x[3], const d
     double drbf_dd = -9.0 * d / (q->r * q->r) * rbf, drbf_dr = 9.0 * d * d / (q->r * q->r * q->r) * rbf;
     if (d > 0.0) {
       double k = a_rbf * drbf_dd / d;
-      st->g_pos[3 * i] += k * dx;
-      st->g_pos[3 * i + 1] += k * dy;
-      st->g_pos[3 * i + 2] += k * dz;
+      g_pos[3 * i] += k * dx;
+      g_pos[3 * i + 1] += k * dy;
+      g_pos[3 * i + 2] += k * dz;
     }
     double a_r = a_rbf * drbf_dr;
     /* r = radius * radiusmap(attr) */
     double s, f;
     uint32_t k0, k1;
     tfd(p->rm, p->rm_n, 0, p->attr[i], &s, &k0, &k1, &f);
-    st->g_rad[i] += a_r * q->rm;
+    g_rad[i] += a_r * q->rm;
     double a_rm = a_r * p->radii[i];
-    st->g_rm[k0] += a_rm * (1.0 - f);
-    st->g_rm[k1] += a_rm * f;
-    st->g_attr[i] += a_rm * s;
+    g_rm[k0] += a_rm * (1.0 - f);
+    g_rm[k1] += a_rm * f;
+    g_attr[i] += a_rm * s;
     /* a3 = alpha^3, c22 = c^2.2, from colormap(attr) */
     double a_alpha = a_a3 * 3.0 * q->alpha * q->alpha;
     tfd(p->cm, p->cm_n, 3, p->attr[i], &s, &k0, &k1, &f);
-    st->g_cm[4 * k0 + 3] += a_alpha * (1.0 - f);
-    st->g_cm[4 * k1 + 3] += a_alpha * f;
-    st->g_attr[i] += a_alpha * s;
+    g_cm[4 * k0 + 3] += a_alpha * (1.0 - f);
+    g_cm[4 * k1 + 3] += a_alpha * f;
+    g_attr[i] += a_alpha * s;
     for (int c2 = 0; c2 < 3; ++c2) {
       double a_c = q->c[c2] > 0.0 ? a_c22[c2] * 2.2 * pow(q->c[c2], 1.2) : 0.0;
       tfd(p->cm, p->cm_n, c2, p->attr[i], &s, &k0, &k1, &f);
-      st->g_cm[4 * k0 + c2] += a_c * (1.0 - f);
-      st->g_cm[4 * k1 + c2] += a_c * f;
-      st->g_attr[i] += a_c * s;
+      g_cm[4 * k0 + c2] += a_c * (1.0 - f);
+      g_cm[4 * k1 + c2] += a_c * f;
+      g_attr[i] += a_c * s;
     }
   }
+}
+
+static int pass2(const orc_diff *p, const dpart *pp, const double x[3], const dpay *pl, void *u) {
+  adj_state *st = (adj_state *)u;
+  double sD, fD, dF;
+  uint32_t j0, j1;
+  double D = tfd(p->dm, p->dm_n, 0, pl->rho, &sD, &j0, &j1, &fD);
+  double c[3];
+  colour(pl, c);
+  /* the term dC/dL_{j-1} belongs to the prefix once c_j is known */
+  if (st->have_prev) {
+    Fcdf(st->Lp, &dF);
+    double gd = 0;
+    for (int k = 0; k < 3; ++k) gd += st->g[k] * (st->cprev[k] - c[k]);
+    st->prefix += dF * gd;
+  }
+  double aD = st->total - st->prefix; /* dC/dD_j */
+  double F0 = Fcdf(st->Lp, &dF);
+  st->Lp += D;
+  double F1 = Fcdf(st->Lp, &dF);
+  point_adjoint(p, pp, x, pl, st->g, aD, F1 - F0, st->g_pos, st->g_rad, st->g_attr, st->g_cm, st->g_dm, st->g_rm);
   memcpy(st->cprev, c, sizeof(c));
   st->have_prev = 1;
+  return 1;
+}
+
+/* ratio tracking: the sample's points, kept for the reverse recursion */
+typedef struct {
+  double x[3], D, c[3];
+  dpay pl;
+} rt_point;
+
+typedef struct {
+  rt_point *pts;
+  size_t n, cap;
+  double T;
+} rt_state;
+
+static int rt_keep(const orc_diff *p, const dpart *pp, const double x[3], const dpay *pl, void *u) {
+  (void)pp;
+  rt_state *st = (rt_state *)u;
+  if (st->n == st->cap) {
+    size_t cap = st->cap ? 2 * st->cap : 256;
+    rt_point *q = (rt_point *)realloc(st->pts, cap * sizeof(rt_point));
+    if (!q) return 0;
+    st->pts = q;
+    st->cap = cap;
+  }
+  rt_point *e = st->pts + st->n++;
+  double s, f;
+  uint32_t j0, j1;
+  memcpy(e->x, x, sizeof(e->x));
+  e->D = tfd(p->dm, p->dm_n, 0, pl->rho, &s, &j0, &j1, &f);
+  colour(pl, e->c);
+  e->pl = *pl;
+  st->T *= 1.0 - e->D;
+  return st->T >= ORC_RT_TMIN;
+}
+
+static void rt_adjoint(const orc_diff *p, const dpart *pp, const rt_state *st, const double *g, double *g_pos,
+                       double *g_rad, double *g_attr, double *g_cm, double *g_dm, double *g_rm) {
+  double *R = (double *)malloc(sizeof(double) * 3 * (st->n + 1));
+  if (!R) return;
+  R[3 * st->n] = R[3 * st->n + 1] = R[3 * st->n + 2] = 0.0;
+  for (size_t j = st->n; j-- > 0;) /* R[j] = remaining colour after point j (index j: point j+1) */
+    for (int k = 0; k < 3; ++k)
+      R[3 * j + k] = j + 1 < st->n ? st->pts[j + 1].D * st->pts[j + 1].c[k] + (1.0 - st->pts[j + 1].D) * R[3 * (j + 1) + k]
+                                   : 0.0;
+  double T = 1.0;
+  for (size_t j = 0; j < st->n; ++j) {
+    const rt_point *e = st->pts + j;
+    double aD = 0.0;
+    for (int k = 0; k < 3; ++k) aD += g[k] * T * (e->c[k] - R[3 * j + k]);
+    point_adjoint(p, pp, e->x, &e->pl, g, aD, T * e->D, g_pos, g_rad, g_attr, g_cm, g_dm, g_rm);
+    T *= 1.0 - e->D;
+  }
+  free(R);
 }
 
 /* gimage: dLoss/dimage, W*H*3.  Gradients are accumulated (callers zero
@@ -346,6 +448,13 @@ ORC_EXPORT int orc_diff_adjoint(const orc_diff *p, const double *gimage, double 
       double g[3];
       for (int k = 0; k < 3; ++k) g[k] = gimage[3 * ((size_t)py * p->width + px) + k] / p->spp;
       for (uint32_t s = 0; s < p->spp; ++s) {
+        if (p->estimator == 1) {
+          rt_state rs = {NULL, 0, 0, 1.0};
+          march(p, pp, px, py, s, rt_keep, &rs);
+          rt_adjoint(p, pp, &rs, g, g_pos, g_rad, g_attr, g_cm, g_dm, g_rm);
+          free(rs.pts);
+          continue;
+        }
         adj_state st;
         memset(&st, 0, sizeof(st));
         st.g = g;

@@ -35,13 +35,17 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_diff_forward_and_adjoint_match_oracle(gpu_ctx):
+@pytest.mark.parametrize("estimator", [0, 1])
+def test_diff_forward_and_adjoint_match_oracle(gpu_ctx, estimator):
+    """estimator 0: expected marching (path replay, total - prefix);
+    1: ratio tracking (path replay with the remaining-colour recursion,
+    against the oracle's stored-points reverse recursion)."""
     import torch
     W = H = 24
     spp, unit = 2, 0.01
     pos, attr, radii, cm, dm, rm = _scene()
     cam = default_camera(W, H, look_from=(0.2, 0.1, 1.3))
-    R = DiffRenderer(gpu_ctx, cam, W, H, unit, spp=spp, frame_id=3, rbf_radius=0.2)
+    R = DiffRenderer(gpu_ctx, cam, W, H, unit, spp=spp, frame_id=3, rbf_radius=0.2, estimator=estimator)
     t = lambda a: torch.tensor(a, device="cuda", requires_grad=True)
     P, A, Rd, CM, DM, RM = (t(a) for a in (pos, attr, radii, cm, dm, rm))
     img = R(P, A, Rd, CM, DM, RM)
@@ -51,7 +55,7 @@ def test_diff_forward_and_adjoint_match_oracle(gpu_ctx):
     torch.cuda.synchronize()
     aabb = np.concatenate([pos.min(0), pos.max(0)])
     D = oracle.Diff(pos.astype(np.float64), attr.astype(np.float64), radii.astype(np.float64), cm, dm, rm, cam, W, H,
-                    unit, 0.2, aabb, spp=spp, frame_id=3)
+                    unit, 0.2, aabb, spp=spp, frame_id=3, estimator=estimator)
     oimg = D.forward()
     assert (oimg > 0.01).mean() > 0.3
     np.testing.assert_allclose(img.detach().cpu().numpy(), oimg, atol=1e-4)
@@ -64,7 +68,8 @@ def test_diff_forward_and_adjoint_match_oracle(gpu_ctx):
         assert e < 1e-3, k
 
 
-def test_diff_backward_cost(gpu_ctx):
+@pytest.mark.parametrize("estimator", [0, 1])
+def test_diff_backward_cost(gpu_ctx, estimator):
     """forward + backward per view against forward alone (<= 3x): the
     library's forward and adjoint calls, each timed to the end of its work
     (the per-view scene rebuild, common to both, is not part of either),
@@ -77,7 +82,9 @@ def test_diff_backward_cost(gpu_ctx):
     radii *= 0.15
     cam = default_camera(W, H, look_from=(0.2, 0.1, 1.4))
     T = lambda a: torch.tensor(a, device="cuda")
-    r = _Render(gpu_ctx, cam, W, H, 0.002, 4, 1, 0.05, 1, 8, T(pos), T(attr), T(radii), T(cm), T(dm), T(rm))
+    r = _Render(gpu_ctx, cam, W, H, 0.002, 4, 1, 0.05, 1, 8, T(pos), T(attr), T(radii), T(cm), T(dm), T(rm),
+                estimator=estimator)
+    gpu_ctx.set_diff_estimator(estimator)
     n = len(pos)
     g = torch.full((H, W, 4), 0.01, dtype=torch.float32, device="cuda")
     grads = [torch.zeros(k, dtype=torch.float32, device="cuda") for k in (3 * n, n, n, cm.size, len(dm), len(rm))]
@@ -100,7 +107,7 @@ def test_diff_backward_cost(gpu_ctx):
     print(f"forward {f:.2f} ms, forward + backward {f + b:.2f} ms per view (ratio {(f + b) / f:.2f})")
     assert (f + b) / f < 3.0
     # the autograd wrapper end to end
-    R = DiffRenderer(gpu_ctx, cam, W, H, 0.002, spp=4, frame_id=1, rbf_radius=0.05)
+    R = DiffRenderer(gpu_ctx, cam, W, H, 0.002, spp=4, frame_id=1, rbf_radius=0.05, estimator=estimator)
     args = [torch.tensor(a, device="cuda", requires_grad=True) for a in (pos, attr, radii, cm, dm, rm)]
     img = R(*args)
     img.sum().backward()
