@@ -27,14 +27,16 @@ __global__ __launch_bounds__(256) void bake_kernel(const float4 *__restrict__ pa
                                                    float rbfRadius, const float4 *__restrict__ rmap,
                                                    uint32_t rm_n, const float4 *__restrict__ cmap,
                                                    uint32_t cm_n, float4 *__restrict__ pr,
-                                                   float4 *__restrict__ pc) {
+                                                   float4 *__restrict__ pc, float *__restrict__ pm) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float4 q = particles[i];
   float rp = tf_r(rmap, rm_n, q.w);
   float radius = (radii ? radii[i] : rbfRadius) * rp;
   float4 c = tf4(cmap, cm_n, q.w);
-  pr[i] = make_float4(q.x, q.y, q.z, c.w == 0.0f ? 0.0f : radius);
+  const float r = c.w == 0.0f ? 0.0f : radius;
+  pr[i] = make_float4(q.x, q.y, q.z, r);
+  pm[i] = aa_rbf_coef(r);
   pc[i] = make_float4(aa_cube(c.w), aa_powf(c.x, 2.2f), aa_powf(c.y, 2.2f), aa_powf(c.z, 2.2f));
 }
 
@@ -72,7 +74,8 @@ __global__ __launch_bounds__(256) void quant_chunks(const float4 *__restrict__ p
 __global__ __launch_bounds__(256) void bake_q(const float4 *__restrict__ particles, const float *__restrict__ radii,
                                               uint32_t n, float rbfRadius, const float4 *__restrict__ rmap,
                                               uint32_t rm_n, const float4 *__restrict__ cmap, uint32_t cm_n,
-                                              const float4 *__restrict__ qc, uint4 *__restrict__ qp) {
+                                              const float4 *__restrict__ qc, uint4 *__restrict__ qp,
+                                              float *__restrict__ pm) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 q = particles[i];
@@ -95,6 +98,7 @@ __global__ __launch_bounds__(256) void bake_q(const float4 *__restrict__ particl
   const uint32_t rh = r > 0.0f && rr > 0.0 ? normal(aa_f2h_rz((float)rr)) : 0u;
   uint32_t ah = aa_f2h_rn(c.w);
   if (ah < 0x400u && c.w > 0.0f) ah = 0x400u;
+  pm[i] = aa_rbf_coef(aa_h2f(rh)); // of the decoded radius
   qp[i] = make_uint4(qa[0] | (qa[1] << 16), qa[2] | (rh << 16), ah | (normal(aa_f2h_rn(aa_powf(c.x, 2.2f))) << 16),
                      normal(aa_f2h_rn(aa_powf(c.y, 2.2f))) | (normal(aa_f2h_rn(aa_powf(c.z, 2.2f))) << 16));
 }
@@ -338,6 +342,9 @@ static int render_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const a
   size_t need = 2 * (size_t)(n ? n : 1);
   const size_t need16 = (size_t)(n ? n : 1) + 2 * (size_t)((n + 31u) >> kQChunkLog2);
   if (need16 > need) need = need16;
+  // then the RBF coefficients, one float per particle
+  const size_t pm_off = need;
+  need += ((size_t)(n ? n : 1) + 3) / 4;
   if (ctx->baked_count < need) {
     if (ctx->baked) {
       hipStreamSynchronize(ctx->stream);
@@ -349,18 +356,19 @@ static int render_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const a
     ctx->baked_count = need;
   }
   float4 *pr = ctx->baked, *pcol = ctx->baked + (n ? n : 1);
+  float *pm = (float *)(ctx->baked + pm_off);
   uint4 *qp = (uint4 *)ctx->baked;
   float4 *qc = ctx->baked + (n ? n : 1);
   const bool q16 = ctx->payload == 1;
   if (n && !q16)
     bake_kernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>((const float4 *)pc->particles, (const float *)pc->radii,
                                                           n, pc->rbfRadius, rm->table, rm->n, cm->table, cm->n, pr,
-                                                          pcol);
+                                                          pcol, pm);
   if (n && q16) {
     const uint32_t nc = (n + 31u) >> kQChunkLog2;
     quant_chunks<<<(nc + 255) / 256, 256, 0, ctx->stream>>>((const float4 *)pc->particles, n, qc);
     bake_q<<<(n + 255) / 256, 256, 0, ctx->stream>>>((const float4 *)pc->particles, (const float *)pc->radii, n,
-                                                     pc->rbfRadius, rm->table, rm->n, cm->table, cm->n, qc, qp);
+                                                     pc->rbfRadius, rm->table, rm->n, cm->table, cm->n, qc, qp, pm);
   }
   AARBF_CHECK_LAUNCH(ctx, "bake");
   ctx->baked_fmt = q16 ? 1 : 0;
@@ -387,6 +395,7 @@ static int render_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const a
   A.lb = make_float3(rg->globalAABBMin.x - m2, rg->globalAABBMin.y - m2, rg->globalAABBMin.z - m2);
   A.pr = pr;
   A.pc = pcol;
+  A.pm = pm;
   A.qp = qp;
   A.qc = qc;
   A.q16 = q16 ? 1u : 0u;

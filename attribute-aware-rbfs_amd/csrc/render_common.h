@@ -38,6 +38,7 @@ struct RenderArgs {
   float3 light, pos, d00, du, dv, lb, rt;
   const float4 *pr; // baked x, y, z, radius (0 = never covers)
   const float4 *pc; // baked alpha^3, rgb^2.2
+  const float *pm;  // baked RBF coefficient -4.5 / r^2 (both payload formats)
   // FP16 payload (q16 != 0): per particle 16 bytes -- x, y, z as 16-bit
   // fixed point in the frame of its chunk of 32 particles, radius, alpha^3
   // and rgb^2.2 as binary16; per chunk qc[2c] = origin, qc[2c+1] = step
@@ -169,10 +170,10 @@ __device__ inline void point_query(const RenderArgs &A, float x, float y, float 
         for (uint32_t p = p0; p < p1; ++p) {
           float4 q = ld_pr(A, p);
           if (STATS) c.particles++;
-          float d = aa_dist3(q.x, q.y, q.z, x, y, z);
-          if (d < q.w) {
+          const float e = aa_dist2(q.x, q.y, q.z, x, y, z) * A.pm[p];
+          if (e > -4.5f) {
             float4 k = ld_pc(A, p);
-            float wa = aa_rbf(d, q.w) * k.x;
+            float wa = aa_expf(e) * k.x;
             if (STATS) c.covering++;
             pl.count += 1;
             pl.dens += wa;
@@ -408,10 +409,10 @@ __device__ inline void point_query_q(const RenderArgs &A, uint16_t *stk, float x
   if (STATS) c.queries++;
   auto test = [&](const float4 &lo, const float4 &hi) { return inside(lo, hi, x, y, z); };
   auto particle = [&](uint32_t p, const float4 &q) {
-    float d = aa_dist3(q.x, q.y, q.z, x, y, z);
-    if (d < q.w) {
+    const float e = aa_dist2(q.x, q.y, q.z, x, y, z) * A.pm[p];
+    if (e > -4.5f) {
       float4 k = ld_pc(A, p);
-      float wa = aa_rbf(d, q.w) * k.x;
+      float wa = aa_expf(e) * k.x;
       if (STATS) c.covering++;
       pl.count += 1;
       pl.dens += wa;
@@ -460,10 +461,10 @@ __device__ inline void point_query_b(const RenderArgs &A, uint16_t *stk, float x
   if (STATS) c.queries++;
   auto test = [&](const float4 &lo, const float4 &hi) { return inside(lo, hi, x, y, z); };
   auto particle = [&](uint32_t p, const float4 &q) {
-    float d = aa_dist3(q.x, q.y, q.z, x, y, z);
-    if (d < q.w) {
+    const float e = aa_dist2(q.x, q.y, q.z, x, y, z) * A.pm[p];
+    if (e > -4.5f) {
       float4 k = ld_pc(A, p);
-      float wa = aa_rbf(d, q.w) * k.x;
+      float wa = aa_expf(e) * k.x;
       if (STATS) c.covering++;
       pl.count += 1;
       pl.dens += wa;

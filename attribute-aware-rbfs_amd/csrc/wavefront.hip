@@ -873,9 +873,9 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
             float4 qq = ld_pr(A, p);
             float4 kk = ld_pc(A, p);
             if (STATS) c.particles++;
-            float dd = aa_dist3(qq.x, qq.y, qq.z, x, y, z);
-            if (dd < qq.w) {
-              float wa = aa_rbf(dd, qq.w) * kk.x;
+            const float e = aa_dist2(qq.x, qq.y, qq.z, x, y, z) * A.pm[p];
+            if (e > -4.5f) {
+              float wa = aa_expf(e) * kk.x;
               if (STATS) c.covering++;
               pl.count += 1;
               pl.dens += wa;
@@ -1135,9 +1135,9 @@ __global__ __launch_bounds__(kSBlock, 4) void wf_step_m(RenderArgs A, WFState S,
             float4 qq = ld_pr(A, p);
             float4 kk = ld_pc(A, p);
             if (STATS) c.particles++;
-            float dd = aa_dist3(qq.x, qq.y, qq.z, x, y, z);
-            if (dd < qq.w) {
-              float wa = aa_rbf(dd, qq.w) * kk.x;
+            const float e = aa_dist2(qq.x, qq.y, qq.z, x, y, z) * A.pm[p];
+            if (e > -4.5f) {
+              float wa = aa_expf(e) * kk.x;
               if (STATS) c.covering++;
               pl.count += 1;
               pl.dens += wa;
@@ -1301,10 +1301,10 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
         const uint32_t p = cur[j / ppl] * ppl + j % ppl;
         if (p < A.numParticles) {
           const float4 q = ld_pr(A, p);
-          const float d = aa_dist3(q.x, q.y, q.z, x, y, z);
-          if (d < q.w) {
+          const float e = aa_dist2(q.x, q.y, q.z, x, y, z) * A.pm[p];
+          if (e > -4.5f) {
             const float4 k = ld_pc(A, p);
-            wa = aa_rbf(d, q.w) * k.x;
+            wa = aa_expf(e) * k.x;
             wr = k.y * wa;
             wg = k.z * wa;
             wb = k.w * wa;
@@ -1734,6 +1734,7 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A0, const ReqArgs *vi
     A.rt = A0.rt;
     A.pr = A0.pr;
     A.pc = A0.pc;
+    A.pm = A0.pm;
     A.qp = A0.qp;
     A.qc = A0.qc;
     A.q16 = A0.q16;

@@ -178,8 +178,23 @@ AA_FN float aa_dist3(float ax, float ay, float az, float bx, float by, float bz)
   return AA_SQRTF((dx * dx + dy * dy) + dz * dz);
 }
 
+/* |a - b|^2 with the summation order of aa_dist3. */
+AA_FN float aa_dist2(float ax, float ay, float az, float bx, float by, float bz) {
+  float dx = ax - bx, dy = ay - by, dz = az - bz;
+  return (dx * dx + dy * dy) + dz * dz;
+}
+
+/* The renderer's form of the RBF of sharedCode.h:137-140 (sigma = 3):
+ * exp(-0.5 (3 d / r)^2) = exp(d^2 m) with the per-particle coefficient
+ * m = -4.5 / r^2, and the particle covers a point where (3 d / r)^2 < 9,
+ * i.e. d^2 m > -4.5 (the reference's distance(X, P) < r).  The point
+ * queries evaluate e = aa_dist2(...) * m, test e > -4.5 and weight exp(e):
+ * no square root and no division per test.  m = -inf for r = 0 (never
+ * covers). */
+AA_FN float aa_rbf_coef(float r) { return -4.5f / (r * r); }
+
 /* Gaussian RBF of sharedCode.h:137-140 with sigma = 3:
- * exp(-0.5 * ((d * 3) / r)^2). */
+ * exp(-0.5 * ((d * 3) / r)^2) (the splat path's form). */
 AA_FN float aa_rbf(float d, float r) {
   float q = (d * 3.0f) / r;
   return aa_expf(-0.5f * (q * q));

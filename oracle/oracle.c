@@ -344,6 +344,7 @@ typedef struct {
   float x, y, z, r;  /* position and effective radius (0 = never covers) */
   float a3;          /* colormap(w).a ^ 3 */
   float c22[3];      /* colormap(w).rgb ^ 2.2 */
+  float m;           /* RBF coefficient -4.5 / r^2 (aa_rbf_coef) */
 } baked_particle;
 
 typedef struct {
@@ -380,6 +381,7 @@ static void bake(const orc_params *p, baked_particle *bp) {
       bp[i].r = b[3];
       bp[i].a3 = b[4];
       for (int k = 0; k < 3; ++k) bp[i].c22[k] = b[5 + k];
+      bp[i].m = aa_rbf_coef(b[3]);
     }
     return;
   }
@@ -396,6 +398,7 @@ static void bake(const orc_params *p, baked_particle *bp) {
     b->r = (c[3] == 0.0f) ? 0.0f : radius; /* transparent: ignoreHit ':369-372' */
     b->a3 = aa_cube(c[3]);
     for (int k = 0; k < 3; ++k) b->c22[k] = aa_powf(c[k], 2.2f);
+    b->m = aa_rbf_coef(b->r);
   }
 }
 
@@ -512,9 +515,10 @@ static void point_query(const orc_scene *s, const float x[3], orc_payload *pl) {
   int clamp1 = !s->p->visualizeAttributes;
   for (uint32_t k = s->cell_start[cell]; k < s->cell_start[cell + 1]; ++k) {
     const baked_particle *b = s->bp + s->cell_items[k];
-    float dist = aa_dist3(b->x, b->y, b->z, x[0], x[1], x[2]);
-    if (!(dist < b->r)) continue;
-    float rbf = aa_rbf(dist, b->r);
+    /* distance(X, P) < r and the RBF in the form of aa_rbf_coef */
+    float e = aa_dist2(b->x, b->y, b->z, x[0], x[1], x[2]) * b->m;
+    if (!(e > -4.5f)) continue;
+    float rbf = aa_expf(e);
     float wa = rbf * b->a3;
     pl->count += 1;
     pl->density += wa;
