@@ -1723,29 +1723,17 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A0, const ReqArgs *vi
   RenderArgs A = A0;
   if (batch) {
     // per-view fields live in the table; cleared here so that the kernel
-    // arguments (and a captured graph's key) do not change from frame to frame
-    const uint32_t blue = A.blue;
-    A = RenderArgs{};
-    A.blue = blue;
-    A.visAttr = A0.visAttr;
-    A.max_depth = A0.max_depth;
-    A.unit = A0.unit;
-    A.lb = A0.lb;
-    A.rt = A0.rt;
-    A.pr = A0.pr;
-    A.pc = A0.pc;
-    A.pm = A0.pm;
-    A.qp = A0.qp;
-    A.qc = A0.qc;
-    A.q16 = A0.q16;
-    A.numParticles = A0.numParticles;
-    A.ppl = A0.ppl;
-    A.cmap = A0.cmap;
-    A.dmap = A0.dmap;
-    A.cm_n = A0.cm_n;
-    A.dm_n = A0.dm_n;
-    A.stats = A0.stats;
-    A.bvh = A0.bvh;
+    // arguments (and a captured graph's key) do not change from frame to
+    // frame.  blue stays: the host picks mode 2 only when no view marches.
+    A.frameID = A.accumID = A.spp = 0;
+    A.showNoise = 0;
+    A.W = A.H = 0;
+    A.amb = 0.0f;
+    A.light = A.pos = A.d00 = A.du = A.dv = make_float3(0.0f, 0.0f, 0.0f);
+    A.stbn = nullptr;
+    A.stbn_w = A.stbn_h = 0;
+    A.image = A.accum = nullptr;
+    A.v0 = ReqArgs{};
   } else {
     A.v0 = V[0];
   }
