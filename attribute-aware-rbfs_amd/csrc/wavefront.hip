@@ -59,7 +59,7 @@ constexpr uint32_t kDmLds = 64;       // density-map texels cached in LDS
 // steps, so taking several per iteration cuts the iterations (and the
 // per-iteration overhead every lane pays) roughly by this factor.
 #ifndef AARBF_FAST_STEPS
-#define AARBF_FAST_STEPS 2
+#define AARBF_FAST_STEPS 8
 #endif
 constexpr int kFastSteps = AARBF_FAST_STEPS;
 
@@ -1631,15 +1631,15 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
       hipGraph_t graph = nullptr;
       const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
       if (rc != AARBF_OK) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return rc;
       }
       AARBF_HIP(ctx, e);
       const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      hipGraphDestroy(graph);
+      (void)hipGraphDestroy(graph);
       AARBF_HIP(ctx, ei);
       if (ctx->graphs.size() >= 8) {
-        hipGraphExecDestroy(ctx->graphs.front().second);
+        (void)hipGraphExecDestroy(ctx->graphs.front().second);
         ctx->graphs.erase(ctx->graphs.begin());
       }
       ctx->graphs.emplace_back(kb, exec);
@@ -1786,8 +1786,8 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A0, const ReqArgs *vi
                                                                               (uint32_t)ntiles);
     if (ctx->lat_cap < 2 * nv + 1) {
       if (ctx->lat_buf) {
-        hipStreamSynchronize(ctx->stream);
-        hipFree(ctx->lat_buf);
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(ctx->lat_buf);
         ctx->lat_buf = nullptr;
         ctx->lat_cap = 0;
       }
