@@ -432,7 +432,9 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
   __shared__ uint32_t s_idx[kCand * kGBlock];
   __shared__ float s_tin[kCand * kGBlock];
   __shared__ float s_tout[kCand * kGBlock];
-  __shared__ uint32_t s_stack[kStackLevels * kGBlock];
+  // the walk's per-level visiting words, top levels of kGBlock words
+  // (sized at launch to the tree, which keeps more blocks resident)
+  extern __shared__ uint32_t s_stack[];
   const uint32_t n_g = *countG;
   if (blockIdx.x * kGBlock >= n_g) return; // the whole block: no barrier is skipped
   load_levels(A);
@@ -1523,10 +1525,11 @@ static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &
   const bool tm = ctx->timing && db == 0;
   if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[0], ctx->stream));
   if (gb > 0) {
+    const size_t stack_bytes = sizeof(uint32_t) * kGBlock * std::max<uint32_t>(A.bvh.top, 1u);
     if (st)
-      wf_gather<true><<<gb, kGBlock, 0, ctx->stream>>>(A, S, lG, cA + 1);
+      wf_gather<true><<<gb, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1);
     else
-      wf_gather<false><<<gb, kGBlock, 0, ctx->stream>>>(A, S, lG, cA + 1);
+      wf_gather<false><<<gb, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1);
   }
   wf_zero<<<1, 64, 0, ctx->stream>>>(cN, 2);
   wf_zero<<<1, 64, 0, ctx->stream>>>(cnt + 4, 1);
