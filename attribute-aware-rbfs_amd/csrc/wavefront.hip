@@ -431,7 +431,6 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
                                                      const uint32_t *__restrict__ countG) {
   __shared__ uint32_t s_idx[kCand * kGBlock];
   __shared__ float s_tin[kCand * kGBlock];
-  __shared__ float s_tout[kCand * kGBlock];
   // the walk's per-level visiting words, top levels of kGBlock words
   // (sized at launch to the tree, which keeps more blocks resident)
   extern __shared__ uint32_t s_stack[];
@@ -445,7 +444,7 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
   if (gid < n_g) {
     const uint32_t slot = listG[gid];
     uint32_t *gi = s_idx + threadIdx.x;
-    float *gtin = s_tin + threadIdx.x, *gtout = s_tout + threadIdx.x;
+    float *gtin = s_tin + threadIdx.x;
     uint32_t *stk = s_stack + threadIdx.x;
     float4 a0 = S.od0[slot], a1 = S.od1[slot];
     const float3 o = make_float3(a0.x, a0.y, a0.z), d = make_float3(a1.x, a1.y, a1.z);
@@ -471,17 +470,26 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
     float maxtin = -AA_INF_F, t_cut = AA_INF_F;
     if (STATS) c.queries++;
     const uint32_t nodes0 = c.nodes;
-    // conservative entry/exit of the particle's sphere widened by m
-    auto take = [&](uint32_t p, const float4 &P) {
-      if (!(P.w > 0.0f)) return;
+    // conservative entry/exit of the particle's sphere widened by m; false
+    // when the line misses it.  Exit parameters are not kept during the
+    // walk (LDS for more resident blocks): the output recomputes them with
+    // the same operations.
+    auto chord = [&](const float4 &P, float &tin, float &tout) -> bool {
       float vx = P.x - o.x, vy = P.y - o.y, vz = P.z - o.z;
       float tc = vx * d.x + vy * d.y + vz * d.z;
       float hx = vx - tc * d.x, hy = vy - tc * d.y, hz = vz - tc * d.z;
       float rr = P.w + m;
       float w2 = rr * rr - (hx * hx + hy * hy + hz * hz);
-      if (!(w2 >= 0.0f)) return;
+      if (!(w2 >= 0.0f)) return false;
       float w = sqrtf(w2);
-      float tin = tc - w - m, tout = tc + w + m;
+      tin = tc - w - m;
+      tout = tc + w + m;
+      return true;
+    };
+    auto take = [&](uint32_t p, const float4 &P) {
+      if (!(P.w > 0.0f)) return;
+      float tin, tout;
+      if (!chord(P, tin, tout)) return;
       if (!(tout >= t0) || !(tin <= tmax) || !(tin < t_cut)) {
         if (tout >= t0 && tin <= tmax) t_cut = fminf(t_cut, tin);
         return;
@@ -489,7 +497,6 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
       if (n < kCand) {
         gi[n * kGBlock] = p;
         gtin[n * kGBlock] = tin;
-        gtout[n * kGBlock] = tout;
         if (tin > maxtin) {
           maxtin = tin;
           maxpos = n;
@@ -501,7 +508,6 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
         t_cut = fminf(t_cut, maxtin);
         gi[maxpos * kGBlock] = p;
         gtin[maxpos * kGBlock] = tin;
-        gtout[maxpos * kGBlock] = tout;
         maxtin = -AA_INF_F;
         for (uint32_t k = 0; k < kCand; ++k) {
           float v = gtin[k * kGBlock];
@@ -647,17 +653,15 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
       // ascending particle order (insertion sort, n <= kCand)
       for (uint32_t a = 1; a < n; ++a) {
         uint32_t pi = gi[a * kGBlock];
-        float ti = gtin[a * kGBlock], to = gtout[a * kGBlock];
+        float ti = gtin[a * kGBlock];
         uint32_t b = a;
         while (b > 0 && gi[(b - 1) * kGBlock] > pi) {
           gi[b * kGBlock] = gi[(b - 1) * kGBlock];
           gtin[b * kGBlock] = gtin[(b - 1) * kGBlock];
-          gtout[b * kGBlock] = gtout[(b - 1) * kGBlock];
           --b;
         }
         gi[b * kGBlock] = pi;
         gtin[b * kGBlock] = ti;
-        gtout[b * kGBlock] = to;
       }
       // entry order: position of rank r in tin order (ties by position)
       for (uint32_t k = 0; k < n; ++k) {
@@ -667,10 +671,13 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
           float tj = gtin[j * kGBlock];
           r += (tj < tk) || (tj == tk && j < k);
         }
+        const uint32_t pk = gi[k * kGBlock];
+        float tin, tout;
+        chord(ld_pr(A, pk), tin, tout); // the exit parameter take() computed
         oeo[r * stride + row] = (uint8_t)k;
-        oidx[k * stride + row] = gi[k * kGBlock];
+        oidx[k * stride + row] = pk;
         otin[k * stride + row] = tk;
-        otout[k * stride + row] = gtout[k * kGBlock];
+        otout[k * stride + row] = tout;
       }
     }
     if (head) {
