@@ -51,6 +51,9 @@ namespace {
 // segments but three waves, and was slower).
 constexpr uint32_t kCand = 12;
 constexpr int kGBlock = 128; // gather kernel block
+#ifndef AARBF_GATHER_WAVES
+#define AARBF_GATHER_WAVES 1 // occupancy target of the gather kernel (waves per SIMD)
+#endif
 constexpr int kSBlock = 64;  // step kernel block (one wave)
 constexpr uint64_t kPassSamples = 64; // samples per colour-buffer pass
 constexpr uint32_t kDmLds = 64;       // density-map texels cached in LDS
@@ -427,12 +430,13 @@ __global__ __launch_bounds__(256) void wf_fold(RenderArgs A, WFState S) {
 
 // ---- gather --------------------------------------------------------------------
 template <bool STATS>
-__global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, const uint32_t *__restrict__ listG,
+__global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderArgs A, WFState S, const uint32_t *__restrict__ listG,
                                                      const uint32_t *__restrict__ countG) {
   __shared__ uint32_t s_idx[kCand * kGBlock];
   __shared__ float s_tin[kCand * kGBlock];
-  // the walk's per-level visiting words, top levels of kGBlock words
-  // (sized at launch to the tree, which keeps more blocks resident)
+  // the walk's per-level visiting words for levels 1 .. top-1, kGBlock
+  // words each (sized at launch to the tree, which keeps more blocks
+  // resident)
   extern __shared__ uint32_t s_stack[];
   const uint32_t n_g = *countG;
   if (blockIdx.x * kGBlock >= n_g) return; // the whole block: no barrier is skipped
@@ -619,13 +623,13 @@ __global__ __launch_bounds__(kGBlock) void wf_gather(RenderArgs A, WFState S, co
           if (nib == 0) {
             if (lvl == top - 1) break;
             ++lvl;
-            ord = stk[lvl * kGBlock];
+            ord = stk[(lvl - 1u) * kGBlock];
             base = ((base >> log2K) >> log2K) << log2K;
             continue;
           }
           ord >>= 4;
           uint32_t node = base + nib - 1u;
-          stk[lvl * kGBlock] = ord;
+          stk[(lvl - 1u) * kGBlock] = ord; // lvl >= 1 here
           base = node << log2K;
           ord = ordered(lvl, node);
           --lvl;
@@ -1532,7 +1536,7 @@ static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &
   const bool tm = ctx->timing && db == 0;
   if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[0], ctx->stream));
   if (gb > 0) {
-    const size_t stack_bytes = sizeof(uint32_t) * kGBlock * std::max<uint32_t>(A.bvh.top, 1u);
+    const size_t stack_bytes = sizeof(uint32_t) * kGBlock * std::max<uint32_t>(A.bvh.top, 2u) - sizeof(uint32_t) * kGBlock;
     if (st)
       wf_gather<true><<<gb, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1);
     else
