@@ -66,8 +66,8 @@ def main() -> int:
         return copy_calibration(a.copy_calibration)
 
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); the render path shards "
-              "by image tiles / requests with a full particle replica per GPU and no exchange, see DESIGN.md",
+        print(f"bench.py: this renderer is measured on one GPU only (asked for --gpus {a.gpus}); the render "
+              "path would shard by image tiles / requests with a full particle replica per GPU and no exchange",
               file=sys.stderr)
         return 2
 

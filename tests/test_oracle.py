@@ -127,7 +127,7 @@ def _stream(frame, px, py, w, k):
 
 
 def test_rng_seed_and_first_draw():
-    # get_rng(0, (0,0), 512x512) recorded by compiling rng.h for the survey
+    # get_rng(0, (0,0), 512x512): the value rng.h (rng.h:20-70) gives for pixel (0, 0) of a 512-wide frame
     s, d = _stream(0, 0, 0, 512, 1)
     assert s == 256347020
     assert abs(float(d[0]) - 0.145855993) < 1e-9
