@@ -35,7 +35,10 @@ constexpr int kDBlock = 64;
 constexpr uint32_t kTexMax = 256; // texels per table with LDS gradient sums (larger tables: global atomics)
 constexpr uint32_t kCache = 4;    // per-lane particle adjoint cache entries
 constexpr uint32_t kSlot = 9;     // particle index + 8 adjoint sums
-constexpr uint32_t kDC = 12;      // candidates per march segment
+#ifndef AARBF_DIFF_CAND
+#define AARBF_DIFF_CAND 12
+#endif
+constexpr uint32_t kDC = AARBF_DIFF_CAND; // candidates per march segment (<= 32: the active set is a bitmask)
 constexpr uint32_t kDense = 32;   // points queried one by one where candidates crowd
 
 struct DiffArgs {

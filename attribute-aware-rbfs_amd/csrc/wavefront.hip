@@ -708,28 +708,23 @@ __device__ inline uint32_t round_cap(uint32_t base_cap, uint32_t nslots, uint32_
 }
 
 // ---- step ------------------------------------------------------------------------
-// DENSE = false: rays whose segment has a candidate list; rays on a dense
-// segment are handed to the dense queue untouched.  DENSE = true: the dense
-// queue, stepped with point queries over the quantised tree.  Keeping the
-// two apart keeps the BVH walk's registers out of the common kernel.
-template <bool STATS, bool DENSE, int WPE, bool B>
+// Rays whose segment has a candidate list; rays on a dense segment are
+// handed to the dense queue (wf_dense_wave) untouched, which keeps the BVH
+// walk's registers out of this kernel.
+template <bool STATS, int WPE, bool B>
 __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S, const uint32_t *__restrict__ listA,
                                                    const uint32_t *__restrict__ countA, uint32_t *__restrict__ outA,
                                                    uint32_t *__restrict__ outG, uint32_t *__restrict__ counts,
                                                    uint32_t *__restrict__ outD, uint32_t *__restrict__ countD,
                                                    uint32_t step_cap) {
-  constexpr uint32_t NC = DENSE ? 1u : kCand;
-  constexpr uint32_t NS = DENSE ? kStackLevels : 1u;
-  __shared__ uint32_t s_idx[NC * kSBlock];
-  __shared__ float s_tin[NC * kSBlock];
-  __shared__ float s_tout[NC * kSBlock];
-  __shared__ uint8_t s_eo[NC * kSBlock];
-  __shared__ uint16_t s_stack[NS * kSBlock];
+  __shared__ uint32_t s_idx[kCand * kSBlock];
+  __shared__ float s_tin[kCand * kSBlock];
+  __shared__ float s_tout[kCand * kSBlock];
+  __shared__ uint8_t s_eo[kCand * kSBlock];
   __shared__ float s_dm[kDmLds];
   const uint32_t n_a = *countA;
   if (blockIdx.x * kSBlock >= n_a) return; // the whole block: no barrier is skipped
   step_cap = round_cap(step_cap, S.nslots, n_a);
-  if (DENSE) load_levels(A);
   // density map red channel, read at every accepted-candidate step
   const bool dm_lds = A.dm_n <= kDmLds;
   if (dm_lds)
@@ -751,7 +746,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
   uint32_t slot = 0;
   if (gid < n_a) {
     slot = listA[gid];
-    if (!DENSE && (S.u[slot].z >> 16) & F_DENSE) {
+    if ((S.u[slot].z >> 16) & F_DENSE) {
       outcome = 3;
     }
   }
@@ -761,7 +756,6 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
     uint32_t *li = s_idx + threadIdx.x;
     float *ltin = s_tin + threadIdx.x, *ltout = s_tout + threadIdx.x;
     uint8_t *leo = s_eo + threadIdx.x;
-    uint16_t *stk = s_stack + threadIdx.x;
     uint32_t n = 0;
     // active set at the current t
     uint32_t act = 0, eptr = 0;
@@ -826,7 +820,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
           simt[0] += 1;
           simt[1] += (uint32_t)__popcll(live);
         }
-        uint64_t touch = __ballot(act != 0 || ((L.flags & F_PENDING) != 0 && DENSE));
+        uint64_t touch = __ballot(act != 0);
         if (touch && (uint32_t)__ffsll((unsigned long long)live) - 1u == (threadIdx.x & 63u)) simt[2] += 1;
       }
       const bool pending = (L.flags & F_PENDING) != 0;
@@ -842,7 +836,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
         tn = L.t - aa_log_step(1.0f - xi) * A.unit;
       }
       const bool can = !pending && L.i < A.max_depth && steps < step_cap;
-      if (!DENSE && can && act == 0u && tn < lim) {
+      if (can && act == 0u && tn < lim) {
         L.trng = ns;
         L.t = tn;
         ++L.i;
@@ -895,37 +889,33 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
         }
         Payload pl;
         float x = L.o.x + L.t * L.d.x, y = L.o.y + L.t * L.d.y, z = L.o.z + L.t * L.d.z;
-        if (DENSE) {
-          point_query_q<kSBlock, STATS>(A, stk, x, y, z, pl, c);
-        } else {
-          if (L.t >= t_enter) {
-            enter(L.t);
-            lim = fminf(lim0, t_enter);
-          }
-          if (L.t > t_exit) leave(L.t);
-          pl.count = 0;
-          pl.dens = 0.0f;
-          pl.r = pl.g = pl.b = 0.0f;
-          uint32_t m = act;
-          while (m) {
-            uint32_t k = (uint32_t)__ffs(m) - 1u;
-            m &= m - 1u;
-            uint32_t p = li[k * kSBlock];
-            // both halves of the payload in one round trip
-            float4 qq = ld_pr(A, p);
-            float4 kk = ld_pc(A, p);
-            if (STATS) c.particles++;
-            const float e = aa_dist2(qq.x, qq.y, qq.z, x, y, z) * A.pm[p];
-            if (e > -4.5f) {
-              float wa = aa_expf(e) * kk.x;
-              if (STATS) c.covering++;
-              pl.count += 1;
-              pl.dens += wa;
-              pl.r += kk.y * wa;
-              pl.g += kk.z * wa;
-              pl.b += kk.w * wa;
-              if (!A.visAttr) pl.dens = fminf(pl.dens, 1.0f);
-            }
+        if (L.t >= t_enter) {
+          enter(L.t);
+          lim = fminf(lim0, t_enter);
+        }
+        if (L.t > t_exit) leave(L.t);
+        pl.count = 0;
+        pl.dens = 0.0f;
+        pl.r = pl.g = pl.b = 0.0f;
+        uint32_t m = act;
+        while (m) {
+          uint32_t k = (uint32_t)__ffs(m) - 1u;
+          m &= m - 1u;
+          uint32_t p = li[k * kSBlock];
+          // both halves of the payload in one round trip
+          float4 qq = ld_pr(A, p);
+          float4 kk = ld_pc(A, p);
+          if (STATS) c.particles++;
+          const float e = aa_dist2(qq.x, qq.y, qq.z, x, y, z) * A.pm[p];
+          if (e > -4.5f) {
+            float wa = aa_expf(e) * kk.x;
+            if (STATS) c.covering++;
+            pl.count += 1;
+            pl.dens += wa;
+            pl.r += kk.y * wa;
+            pl.g += kk.z * wa;
+            pl.b += kk.w * wa;
+            if (!A.visAttr) pl.dens = fminf(pl.dens, 1.0f);
           }
         }
         L.flags &= ~F_PENDING;
@@ -986,7 +976,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
       float vis = (L.flags & F_SHADOW) ? 1.0f - L.shw : 1.0f;
       finish_sample<STATS, B>(A, S, L, vis, c);
       outcome = next_ray<STATS, B>(A, S, L, c);
-      if (!DENSE && outcome == 1 && !(L.flags & F_DENSE)) {
+      if (outcome == 1 && !(L.flags & F_DENSE)) {
         // the pixel's next sample can step at once: go on with it here
         setup();
         outcome = 0;
@@ -1007,7 +997,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
       uint64_t sum = simt[k];
       for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
       if ((threadIdx.x & 63) == 0 && sum)
-        atomicAdd((unsigned long long *)&A.stats[(DENSE ? 29 : 26) + k], (unsigned long long)sum);
+        atomicAdd((unsigned long long *)&A.stats[26 + k], (unsigned long long)sum);
     }
 }
 
@@ -1552,17 +1542,17 @@ static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &
     else
       wf_step_m<false, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
   } else if (st) {
-    wf_step<true, false, 1, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+    wf_step<true, 1, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
   } else {
     switch (ctx->tune_step_waves) {
     case 5:
-      wf_step<false, false, 5, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+      wf_step<false, 5, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
       break;
     case 6:
-      wf_step<false, false, 6, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+      wf_step<false, 6, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
       break;
     default: // 4 waves per SIMD
-      wf_step<false, false, 4, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
+      wf_step<false, 4, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
     }
   }
   AARBF_CHECK_LAUNCH(ctx, "wf_step");
