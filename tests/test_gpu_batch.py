@@ -206,3 +206,29 @@ def test_autotune_keeps_images_and_caller_buffers(gpu_ctx):
     finally:
         for k in (gpu_ctx.TUNE_STEP_CAP, gpu_ctx.TUNE_STEP_WAVES, gpu_ctx.TUNE_GRAPH_ROUNDS):
             gpu_ctx.set_tuning(k, 4096 if k == gpu_ctx.TUNE_STEP_CAP else 0)
+
+
+@pytest.mark.parametrize("rounds", [0, 4])
+def test_batch_views_spanning_several_passes(gpu_ctx, rounds):
+    """Views whose spp ends in different 64-sample passes (70, 3 and 130
+    spp): every view equals its own call, host-driven and captured rounds."""
+    sc, _ = _scene(gpu_ctx, n=1200, seed=19)
+    stbn = gpu_ctx.stbn(scenes.synthetic_stbn(seed=9))
+    views = [(12, 9, 70, False, 5, 1, (0.4, 0.3, 2.2), (0.3, 0.2, 0.4), False),
+             (16, 16, 3, True, 9, 2, (-0.5, 0.2, 2.0), (0.1, 0.6, 0.5), False),
+             (7, 13, 130, False, 65500, 1, (0.0, 1.8, 1.0), (0.7, 0.3, 0.2), False)]
+    a = _views(gpu_ctx, sc, views, stbn, seed_acc=3)
+    b = _views(gpu_ctx, sc, views, stbn, seed_acc=3)
+    for (pc, img, acc, W, H, spp, m) in a:
+        gpu_ctx.render(sc.raygen_data(img, acc, W, H, stbn=stbn if m else 0), pc, W, H, spp)
+    gpu_ctx.sync()
+    try:
+        gpu_ctx.set_tuning(gpu_ctx.TUNE_GRAPH_ROUNDS, rounds)
+        _run_batch(gpu_ctx, sc, b, stbn)
+    finally:
+        gpu_ctx.set_tuning(gpu_ctx.TUNE_GRAPH_ROUNDS, 0)
+    for k, (va, vb) in enumerate(zip(a, b)):
+        assert np.array_equal(va[2].cpu().numpy(), vb[2].cpu().numpy()), f"view {k}: accum"
+        assert np.array_equal(va[1].cpu().numpy(), vb[1].cpu().numpy()), f"view {k}: image"
+    lat = gpu_ctx.batch_latency(len(views))
+    assert np.all(lat >= 0)
