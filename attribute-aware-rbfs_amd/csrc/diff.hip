@@ -40,6 +40,10 @@ constexpr uint32_t kSlot = 9;     // particle index + 8 adjoint sums
 #endif
 constexpr uint32_t kDC = AARBF_DIFF_CAND; // candidates per march segment (<= 32: the active set is a bitmask)
 constexpr uint32_t kDense = 32;   // points queried one by one where candidates crowd
+#ifndef AARBF_DIFF_MINSEG
+#define AARBF_DIFF_MINSEG 8
+#endif
+constexpr uint32_t kMinSeg = AARBF_DIFF_MINSEG; // shorter segments (in points) count as crowded
 
 struct DiffArgs {
   RenderArgs A;
@@ -143,7 +147,7 @@ struct DPay {
 struct DSeg {
   uint32_t *idx;
   float *tin, *tout;
-  uint16_t *stk;
+  uint32_t *stk; // per-level words of the walks (visit order / hit masks)
 };
 
 // The march of sample s of pixel (px, py): fn(x, y, z, payload, cover) at
@@ -243,18 +247,39 @@ __device__ inline void march(const DiffArgs &D, const DSeg &W, int px, int py, u
       }
       q.len = fminf(tx, t_cut + 4.0f * m) - t0;
     };
-    Counters c = {};
-    walk_q<kDBlock, false>(
-        A, W.stk, [&](const float4 &lo, const float4 &hi) { return seg_box(q, lo, hi); },
-        [&](uint32_t base, uint32_t mask) {
-          for (; mask; mask &= mask - 1u) {
-            const uint32_t leaf = base + (uint32_t)__ffs(mask) - 1u;
-            const uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
-            for (uint32_t p = p0; p < p1; ++p) take(p);
+    // front-to-back walk: the children of a hit node in order of entry, so
+    // the nearest particles come first and t_cut prunes the rest early
+    {
+      const uint32_t log2K = A.bvh.log2K, top = A.bvh.top;
+      const float4 *root = A.bvh.nodes + g_lvl_ofs[top];
+      if (top >= 1 && seg_box(q, root[0], root[1])) {
+        uint32_t lvl = top - 1, base = 0;
+        uint32_t ord = ftb_children(A, q, top, 0);
+        for (;;) {
+          if (lvl == 0) { // a whole leaf group, in entry order
+            for (; ord; ord >>= 4) {
+              const uint32_t leaf = base + (ord & 15u) - 1u;
+              const uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
+              for (uint32_t p = p0; p < p1; ++p) take(p);
+            }
           }
-          return false;
-        },
-        c);
+          const uint32_t nib = ord & 15u;
+          if (nib == 0) {
+            if (lvl == top - 1) break;
+            ++lvl;
+            ord = W.stk[(lvl - 1u) * kDBlock];
+            base = ((base >> log2K) >> log2K) << log2K;
+            continue;
+          }
+          ord >>= 4;
+          const uint32_t node = base + nib - 1u;
+          W.stk[(lvl - 1u) * kDBlock] = ord;
+          base = node << log2K;
+          ord = ftb_children(A, q, lvl, node);
+          --lvl;
+        }
+      }
+    }
     for (uint32_t a = 1; a < n; ++a) { // insertion sort by entry
       const uint32_t pi = W.idx[a * kDBlock];
       const float ti = W.tin[a * kDBlock], to = W.tout[a * kDBlock];
@@ -295,9 +320,12 @@ __device__ inline void march(const DiffArgs &D, const DSeg &W, int px, int py, u
     bool dense = !seg_ok || t < dense_end;
     if (!dense && !(t < seg_end)) {
       gather(t);
-      if (!(t < seg_end)) { // crowded: point queries for a while
+      // crowded (the list is full within a few points): point queries for
+      // a while instead of a gather every few points
+      if (!(t + (float)kMinSeg * A.unit < seg_end)) {
         dense = true;
         dense_end = t + (float)kDense * A.unit;
+        seg_end = -AA_INF_F;
       }
     }
     if (!dense) {
@@ -381,7 +409,7 @@ __device__ inline void colour(const DPay &pl, float c[3]) {
 #define AARBF_DSEG_LDS                                                                                      \
   __shared__ uint32_t s_cidx[kDC * kDBlock];                                                                  \
   __shared__ float s_ctin[kDC * kDBlock], s_ctout[kDC * kDBlock];                                             \
-  __shared__ uint16_t s_stk[kStackLevels * kDBlock];                                                          \
+  __shared__ uint32_t s_stk[kStackLevels * kDBlock];                                                          \
   const DSeg W{s_cidx + threadIdx.x, s_ctin + threadIdx.x, s_ctout + threadIdx.x, s_stk + threadIdx.x};
 
 __global__ __launch_bounds__(kDBlock) void diff_forward(DiffArgs D) {

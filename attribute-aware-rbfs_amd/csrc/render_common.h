@@ -344,8 +344,8 @@ __device__ inline uint32_t qbyte(const QGroup &g, uint32_t field, uint32_t k) {
   return (g.q[2 * field + (k >> 2)] >> ((k & 3u) * 8u)) & 255u;
 }
 
-template <int STRIDE, bool STATS, class BoxTest, class Leaves>
-__device__ inline void walk_q(const RenderArgs &A, uint16_t *stk, BoxTest test, Leaves leaves, Counters &c) {
+template <int STRIDE, bool STATS, class BoxTest, class Leaves, class StkT>
+__device__ inline void walk_q(const RenderArgs &A, StkT *stk, BoxTest test, Leaves leaves, Counters &c) {
   const QGroup *__restrict__ Q = A.bvh.groups;
   const uint32_t log2K = A.bvh.log2K, K = 1u << log2K, top = A.bvh.top;
   const float4 *root = A.bvh.nodes + g_lvl_ofs[top];
@@ -396,6 +396,61 @@ __device__ inline void walk_q(const RenderArgs &A, uint16_t *stk, BoxTest test, 
     mask = group(lvl, node);
     --lvl;
   }
+}
+
+// Children of quantised node `node` at level `lvl` that the segment q
+// enters, in order of where it enters their boxes: nibbles (child + 1),
+// first child in the lowest nibble (K <= 8).  The front-to-back walks use
+// it to reach the nearest particles first, so their t_cut prunes the rest.
+__device__ inline uint32_t ftb_children(const RenderArgs &A, const SegQuery &q, uint32_t lvl, uint32_t node) {
+  const QGroup g = A.bvh.groups[g_lvl_qofs[lvl] + node];
+  const uint32_t log2K = A.bvh.log2K, K = 1u << log2K;
+  const uint32_t first = node << log2K;
+  const uint32_t mcnt = min(K, g_lvl_cnt[lvl - 1] - first);
+  float key[8];
+  uint32_t id[8];
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) {
+    key[k] = AA_INF_F;
+    id[k] = k;
+    const uint32_t lx = qbyte(g, 0, k), hx = qbyte(g, 3, k);
+    if (k < mcnt && lx <= hx) {
+      float4 lo, hi;
+      lo.x = g.a.x + (float)lx * g.a.w;
+      lo.y = g.a.y + (float)qbyte(g, 1, k) * g.b.x;
+      lo.z = g.a.z + (float)qbyte(g, 2, k) * g.b.y;
+      hi.x = g.a.x + (float)hx * g.a.w;
+      hi.y = g.a.y + (float)qbyte(g, 4, k) * g.b.x;
+      hi.z = g.a.z + (float)qbyte(g, 5, k) * g.b.y;
+      float tn;
+      if (seg_box_t(q, lo, hi, tn)) key[k] = tn;
+    }
+  }
+  // Batcher's odd-even merge sort network for 8 keys (fixed indices: the
+  // keys stay in registers)
+#define AARBF_FTB_CSWAP(i, j)                                                                                \
+  if (key[j] < key[i]) {                                                                                     \
+    const float tk = key[i];                                                                                 \
+    key[i] = key[j];                                                                                         \
+    key[j] = tk;                                                                                             \
+    const uint32_t ti = id[i];                                                                               \
+    id[i] = id[j];                                                                                           \
+    id[j] = ti;                                                                                              \
+  }
+  AARBF_FTB_CSWAP(0, 1) AARBF_FTB_CSWAP(2, 3) AARBF_FTB_CSWAP(4, 5) AARBF_FTB_CSWAP(6, 7)
+  AARBF_FTB_CSWAP(0, 2) AARBF_FTB_CSWAP(1, 3) AARBF_FTB_CSWAP(4, 6) AARBF_FTB_CSWAP(5, 7)
+  AARBF_FTB_CSWAP(1, 2) AARBF_FTB_CSWAP(5, 6) AARBF_FTB_CSWAP(0, 4) AARBF_FTB_CSWAP(3, 7)
+  AARBF_FTB_CSWAP(1, 5) AARBF_FTB_CSWAP(2, 6) AARBF_FTB_CSWAP(1, 4) AARBF_FTB_CSWAP(3, 6)
+  AARBF_FTB_CSWAP(2, 4) AARBF_FTB_CSWAP(3, 5) AARBF_FTB_CSWAP(3, 4)
+#undef AARBF_FTB_CSWAP
+  uint32_t ord = 0, cnt = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < 8; ++r)
+    if (key[r] < AA_INF_F) {
+      ord |= (id[r] + 1u) << (4u * cnt);
+      ++cnt;
+    }
+  return ord;
 }
 
 // Point query over the quantised tree: leaves whose (dequantised) box holds
