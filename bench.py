@@ -195,6 +195,15 @@ def main() -> int:
             traffic = json.load(open(a.traffic)).get("bytes_per_launch")
         except Exception:
             traffic = None
+    # the step kernel's issue limit, from the profiled SQ counters of this workload
+    valu = None
+    vpath = os.path.join(ROOT, "profiles", "render_valu.json")
+    if os.path.exists(vpath):
+        try:
+            v = json.load(open(vpath))
+            valu = {"kernel": v["kernel"], "busy": v["valu_busy"], "source": v["source"]}
+        except Exception:
+            valu = None
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -235,6 +244,7 @@ def main() -> int:
             "algorithmic_bytes_per_launch": int(bytes_launch),
             "algorithmic_bytes_per_sample": round(bytes_launch / samples, 1),
             "kernels": kernels,
+            "valu": valu,
         },
         "cpu_baseline": cpu,
         "diff": diff,
