@@ -410,7 +410,13 @@ class Scene:
                 device: str = "cuda"):
         torch = _torch()
         dev = torch.device(device, ctx.device) if device == "cuda" else torch.device(device)
-        p = torch.from_numpy(np.ascontiguousarray(xyzw, dtype=np.float32).reshape(-1, 4)).to(dev)
+        # numpy arrays or tensors (a tensor already on the device is copied there)
+        def dev_f32(a, shape):
+            if isinstance(a, torch.Tensor):
+                return a.detach().to(device=dev, dtype=torch.float32).reshape(shape).contiguous().clone()
+            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32).reshape(shape)).to(dev)
+
+        p = dev_f32(xyzw, (-1, 4))
         cm = colormap if isinstance(colormap, int) else ctx.tf(colormap)
         rm = radiusmap if isinstance(radiusmap, int) else ctx.tf(radiusmap)
         dm = densitymap if isinstance(densitymap, int) else ctx.tf(densitymap)
@@ -419,7 +425,7 @@ class Scene:
         perm = ctx.sort_particles(p, codes)
         r = None
         if radii is not None:
-            r = torch.from_numpy(np.ascontiguousarray(radii, dtype=np.float32)).to(dev)
+            r = dev_f32(radii, (-1,))
             r = r[perm.long()].contiguous()
         rng = ctx.attribute_range(p)
         if normalize:

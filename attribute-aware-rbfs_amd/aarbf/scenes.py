@@ -196,6 +196,45 @@ def gaussian_mixture(n: int, components: int, seed: int, sigma_range=(0.02, 0.15
     return xyzw, radii.astype(np.float32)
 
 
+def hierarchical_clusters(n: int = 67108864, seed: int = 13, levels: int = 3, children: int = 16,
+                          sigmas=(0.12, 0.035, 0.01), leaf_sigma: float = 0.004, r_range=(0.0005, 0.01),
+                          r_power: float = 2.0, device: str = "cuda"):
+    """Precision workload scene: a `levels`-level Gaussian mixture with
+    `children` children per node (16 x 16 x 16 = 4,096 leaf clusters at the
+    defaults), the root at the centre of the unit cube, level l offsets
+    N(0, sigmas[l]); particles N(leaf centre, leaf_sigma); radii power-law
+    p(r) ~ r^-r_power on r_range; attribute = distance to the particle's
+    top-level centre plus N(0, 0.02).  Drawn with torch's seeded generator
+    on `device` (67M particles in seconds); returns (xyzw [n, 4] float32,
+    radii [n] float32) tensors on `device`."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    f64 = torch.float64
+    centres = torch.full((1, 3), 0.5, dtype=f64, device=device)
+    tops = None
+    for lvl in range(levels):
+        k = centres.shape[0]
+        off = torch.randn((k, children, 3), generator=g, device=device, dtype=f64) * sigmas[lvl]
+        centres = (centres[:, None, :] + off).reshape(-1, 3)
+        if lvl == 0:
+            tops = centres
+    leaf_per_top = centres.shape[0] // tops.shape[0]
+    comp = torch.randint(0, centres.shape[0], (n,), generator=g, device=device)
+    pos = centres[comp] + torch.randn((n, 3), generator=g, device=device, dtype=f64) * leaf_sigma
+    a, b = r_range
+    u = torch.rand((n,), generator=g, device=device, dtype=f64)
+    if r_power == 1.0:
+        radii = a * (b / a) ** u
+    else:
+        e = 1.0 - r_power
+        radii = (a ** e + u * (b ** e - a ** e)) ** (1.0 / e)
+    attr = torch.linalg.vector_norm(pos - tops[comp // leaf_per_top], dim=1) + 0.02 * torch.randn(
+        (n,), generator=g, device=device, dtype=f64)
+    xyzw = torch.cat([pos, attr[:, None]], dim=1).to(torch.float32)
+    return xyzw, radii.to(torch.float32)
+
+
 def orbit_camera(frame: int, frames: int, radius: float, centre=(0.5, 0.5, 0.5), seed: int = 2,
                  elevation_deg: float = 20.0):
     """Seeded orbit: azimuth starts at a seeded angle and sweeps one full

@@ -41,66 +41,98 @@ __global__ __launch_bounds__(256) void bake_kernel(const float4 *__restrict__ pa
 }
 
 
-// FP16 payload: per chunk of 32 particles (Hilbert order, so spatially
-// compact) the centres' bounding box gives an origin and a per-axis step
-// (computed in double, stored as float) for 16-bit fixed-point positions.
-__global__ __launch_bounds__(256) void quant_chunks(const float4 *__restrict__ particles, uint32_t n,
-                                                    float4 *__restrict__ qc) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= (n + 31u) >> kQChunkLog2) return;
-  float lo[3] = {AA_INF_F, AA_INF_F, AA_INF_F}, hi[3] = {-AA_INF_F, -AA_INF_F, -AA_INF_F};
-  for (uint32_t i = c << kQChunkLog2; i < min(n, (c + 1u) << kQChunkLog2); ++i) {
-    const float4 q = particles[i];
-    const float v[3] = {q.x, q.y, q.z};
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = fminf(lo[d], v[d]);
-      hi[d] = fmaxf(hi[d], v[d]);
-    }
-  }
-  float s[3];
-  for (int d = 0; d < 3; ++d) s[d] = (float)(((double)hi[d] - (double)lo[d]) / 65535.0);
-  qc[2 * c] = make_float4(lo[0], lo[1], lo[2], 0.0f);
-  qc[2 * c + 1] = make_float4(s[0], s[1], s[2], 0.0f);
+// FP16 payload, one thread per particle, 32 lanes per chunk of 32
+// Hilbert-consecutive (so spatially compact) particles:
+//  * positions: the chunk's centre box gives an origin and a per-axis step
+//    ((max - min) / 65535, in double, stored as float); each coordinate
+//    rounds to the nearest grid point;
+//  * radius: shrunk by the particle's rounding shift (so the decoded sphere
+//    lies inside the original one, and so inside its FP32 leaf box: the BVH
+//    stays valid), then coded against the chunk's range of shrunk radii
+//    (code 1 = the smallest, 65535 = the largest; 0 = radius 0), rounded
+//    down -- the encoder evaluates the decoder's own expression;
+//  * alpha (cubed when decoded) as 16-bit unorm, rounded to nearest, a
+//    positive alpha at least one step (a zero alpha would leave points with
+//    count > 0 and density 0, whose colour divides 0 by 0);
+//  * rgb^2.2 as binary16, rounded to nearest (subnormal halves included:
+//    dark channels keep an absolute precision of 2^-25).
+__device__ inline float chunk_min(float v) {
+  for (int m = 16; m > 0; m >>= 1) v = fminf(v, __shfl_xor(v, m, 32));
+  return v;
+}
+__device__ inline float chunk_max(float v) {
+  for (int m = 16; m > 0; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 32));
+  return v;
+}
+__device__ inline double chunk_min(double v) {
+  for (int m = 16; m > 0; m >>= 1) v = fmin(v, __shfl_xor(v, m, 32));
+  return v;
+}
+__device__ inline double chunk_max(double v) {
+  for (int m = 16; m > 0; m >>= 1) v = fmax(v, __shfl_xor(v, m, 32));
+  return v;
 }
 
-// Encodes particle i.  Positions round to the nearest grid point of the
-// chunk; the radius is reduced by the rounding shift (and 2^-10 more, then
-// rounded towards zero to binary16), so every decoded sphere lies inside the
-// original one and so inside its leaf box: the BVH built from the FP32
-// bounds stays valid.  alpha (cubed when decoded) and rgb^2.2 round to
-// nearest.  No half is subnormal: a radius or rgb^2.2 below 2^-14 becomes
-// 0, and a positive alpha is at least 2^-14 (a zero alpha would leave points
-// with count > 0 and density 0, whose colour divides 0 by 0).
 __global__ __launch_bounds__(256) void bake_q(const float4 *__restrict__ particles, const float *__restrict__ radii,
                                               uint32_t n, float rbfRadius, const float4 *__restrict__ rmap,
                                               uint32_t rm_n, const float4 *__restrict__ cmap, uint32_t cm_n,
-                                              const float4 *__restrict__ qc, uint4 *__restrict__ qp,
+                                              float4 *__restrict__ qc, uint4 *__restrict__ qp,
                                               float *__restrict__ pm) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float4 q = particles[i];
-  const float rp = tf_r(rmap, rm_n, q.w);
-  const float radius = (radii ? radii[i] : rbfRadius) * rp;
-  const float4 c = tf4(cmap, cm_n, q.w);
-  const float r = c.w == 0.0f ? 0.0f : radius;
-  const float4 o = qc[2 * (i >> kQChunkLog2)], s = qc[2 * (i >> kQChunkLog2) + 1];
-  const float v[3] = {q.x, q.y, q.z}, ov[3] = {o.x, o.y, o.z}, sv[3] = {s.x, s.y, s.z};
+  const bool live = i < n; // every lane of a chunk takes part in its reductions
+  const float4 q = live ? particles[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float v[3] = {q.x, q.y, q.z};
+  float ov[3], sv[3];
+  for (int d = 0; d < 3; ++d) {
+    ov[d] = chunk_min(live ? v[d] : AA_INF_F);
+    const float hi = chunk_max(live ? v[d] : -AA_INF_F);
+    sv[d] = (float)(((double)hi - (double)ov[d]) / 65535.0);
+  }
   uint32_t qa[3];
   double shift2 = 0.0;
   for (int d = 0; d < 3; ++d) {
-    double t = sv[d] > 0.0f ? rint(((double)v[d] - (double)ov[d]) / (double)sv[d]) : 0.0;
+    const double t = sv[d] > 0.0f ? rint(((double)v[d] - (double)ov[d]) / (double)sv[d]) : 0.0;
     qa[d] = (uint32_t)fmin(fmax(t, 0.0), 65535.0);
-    const float dec = ov[d] + (float)qa[d] * sv[d];
-    shift2 += ((double)dec - v[d]) * ((double)dec - v[d]);
+    const double e = (double)fx16(qa[d], sv[d], ov[d]) - (double)v[d];
+    shift2 += e * e;
   }
-  const double rr = ((double)r - sqrt(shift2)) * (1.0 - 1.0 / 1024.0);
-  auto normal = [](uint32_t h) { return h < 0x400u ? 0u : h; };
-  const uint32_t rh = r > 0.0f && rr > 0.0 ? normal(aa_f2h_rz((float)rr)) : 0u;
-  uint32_t ah = aa_f2h_rn(c.w);
-  if (ah < 0x400u && c.w > 0.0f) ah = 0x400u;
-  pm[i] = aa_rbf_coef(aa_h2f(rh)); // of the decoded radius
-  qp[i] = make_uint4(qa[0] | (qa[1] << 16), qa[2] | (rh << 16), ah | (normal(aa_f2h_rn(aa_powf(c.x, 2.2f))) << 16),
-                     normal(aa_f2h_rn(aa_powf(c.y, 2.2f))) | (normal(aa_f2h_rn(aa_powf(c.z, 2.2f))) << 16));
+  float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  double rr = 0.0;
+  if (live) {
+    const float rp = tf_r(rmap, rm_n, q.w);
+    const float radius = (radii ? radii[i] : rbfRadius) * rp;
+    c = tf4(cmap, cm_n, q.w);
+    const float r = c.w == 0.0f ? 0.0f : radius;
+    if (r > 0.0f) rr = ((double)r - sqrt(shift2)) * (1.0 - 0x1p-30);
+  }
+  const bool has_r = rr > 0.0;
+  // chunk radius range: base + code * step, code 1 .. 65535
+  const double rmin = chunk_min(has_r ? rr : (double)AA_INF_F);
+  const double rmax = chunk_max(has_r ? rr : -(double)AA_INF_F);
+  float r_step = 0.0f, r_base = 0.0f;
+  if (rmin < (double)AA_INF_F) {
+    r_step = (float)((rmax - rmin) / 65534.0);
+    r_base = (float)(rmin - (double)r_step);
+    // code 1 must not decode above the smallest shrunk radius
+    while ((double)fx16(1u, r_step, r_base) > rmin) r_base = nextafterf(r_base, -AA_INF_F);
+  }
+  uint32_t qr = 0;
+  if (has_r) {
+    double t = r_step > 0.0f ? floor((rr - (double)r_base) / (double)r_step) : 1.0;
+    qr = (uint32_t)fmin(fmax(t, 1.0), 65535.0);
+    while (qr > 1u && (double)fx16(qr, r_step, r_base) > rr) --qr;
+    if ((double)fx16(qr, r_step, r_base) > rr || !(fx16(qr, r_step, r_base) > 0.0f)) qr = 0u; // cannot shrink: drop
+  }
+  if ((i & 31u) == 0u && live) {
+    qc[2 * (i >> kQChunkLog2)] = make_float4(ov[0], ov[1], ov[2], r_base);
+    qc[2 * (i >> kQChunkLog2) + 1] = make_float4(sv[0], sv[1], sv[2], r_step);
+  }
+  if (!live) return;
+  uint32_t au = (uint32_t)fminf(rintf(c.w * 65535.0f), 65535.0f);
+  if (au == 0u && c.w > 0.0f) au = 1u;
+  pm[i] = aa_rbf_coef(fx16_r(qr, r_step, r_base)); // of the decoded radius
+  qp[i] = make_uint4(qa[0] | (qa[1] << 16), qa[2] | (qr << 16), au | (aa_f2h_rn(aa_powf(c.x, 2.2f)) << 16),
+                     aa_f2h_rn(aa_powf(c.y, 2.2f)) | (aa_f2h_rn(aa_powf(c.z, 2.2f)) << 16));
 }
 
 // The decoded payload (x, y, z, r, alpha^3, rgb^2.2 per particle) of either format.
@@ -364,12 +396,9 @@ static int render_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const a
     bake_kernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>((const float4 *)pc->particles, (const float *)pc->radii,
                                                           n, pc->rbfRadius, rm->table, rm->n, cm->table, cm->n, pr,
                                                           pcol, pm);
-  if (n && q16) {
-    const uint32_t nc = (n + 31u) >> kQChunkLog2;
-    quant_chunks<<<(nc + 255) / 256, 256, 0, ctx->stream>>>((const float4 *)pc->particles, n, qc);
+  if (n && q16)
     bake_q<<<(n + 255) / 256, 256, 0, ctx->stream>>>((const float4 *)pc->particles, (const float *)pc->radii, n,
                                                      pc->rbfRadius, rm->table, rm->n, cm->table, cm->n, qc, qp, pm);
-  }
   AARBF_CHECK_LAUNCH(ctx, "bake");
   ctx->baked_fmt = q16 ? 1 : 0;
   ctx->baked_n = n;

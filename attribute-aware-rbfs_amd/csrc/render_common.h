@@ -39,9 +39,10 @@ struct RenderArgs {
   const float4 *pr; // baked x, y, z, radius (0 = never covers)
   const float4 *pc; // baked alpha^3, rgb^2.2
   const float *pm;  // baked RBF coefficient -4.5 / r^2 (both payload formats)
-  // FP16 payload (q16 != 0): per particle 16 bytes -- x, y, z as 16-bit
-  // fixed point in the frame of its chunk of 32 particles, radius, alpha^3
-  // and rgb^2.2 as binary16; per chunk qc[2c] = origin, qc[2c+1] = step
+  // FP16 payload (q16 != 0): per particle 16 bytes -- x, y, z and radius
+  // as 16-bit fixed point in the frame of its chunk of 32 particles, alpha
+  // as 16-bit unorm, rgb^2.2 as binary16; per chunk qc[2c] = origin (xyz,
+  // radius base), qc[2c+1] = per-axis step (xyz, radius step)
   const uint4 *qp;
   const float4 *qc;
   uint32_t q16;
@@ -70,22 +71,30 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A, const ReqArgs *vie
 
 constexpr uint32_t kQChunkLog2 = 5; // particles per FP16 quantisation chunk: 32
 
-// binary16 -> binary32 by the hardware conversion (exact; the FP16 payload
-// holds no subnormal halves, so the denormal mode does not matter).
+// binary16 -> binary32 by the hardware conversion (exact, subnormal halves
+// included: HIP kernels run with FP16 denormals enabled).
 __device__ inline float h16(uint32_t h) { return (float)__builtin_bit_cast(_Float16, (uint16_t)h); }
+
+// 16-bit fixed point of a chunk: origin + q * step with one rounding (the
+// encoder evaluates the same expression, so it knows every decoded value).
+__device__ inline float fx16(uint32_t q, float step, float origin) { return __builtin_fmaf((float)q, step, origin); }
+// radius code 0 = radius 0 (never covers)
+__device__ inline float fx16_r(uint32_t q, float step, float origin) { return q ? fx16(q, step, origin) : 0.0f; }
+// alpha as 16-bit unorm
+__device__ inline float un16(uint32_t q) { return (float)q * (1.0f / 65535.0f); }
 
 // Baked particle p: position and radius, and (alpha^3, rgb^2.2).
 __device__ inline float4 ld_pr(const RenderArgs &A, uint32_t p) {
   if (!A.q16) return A.pr[p];
   const uint4 w = A.qp[p];
   const float4 o = A.qc[2 * (p >> kQChunkLog2)], s = A.qc[2 * (p >> kQChunkLog2) + 1];
-  return make_float4(o.x + (float)(w.x & 0xffffu) * s.x, o.y + (float)(w.x >> 16) * s.y,
-                     o.z + (float)(w.y & 0xffffu) * s.z, h16(w.y >> 16));
+  return make_float4(fx16(w.x & 0xffffu, s.x, o.x), fx16(w.x >> 16, s.y, o.y), fx16(w.y & 0xffffu, s.z, o.z),
+                     fx16_r(w.y >> 16, s.w, o.w));
 }
 __device__ inline float4 ld_pc(const RenderArgs &A, uint32_t p) {
   if (!A.q16) return A.pc[p];
   const uint4 w = A.qp[p];
-  return make_float4(aa_cube(h16(w.z & 0xffffu)), h16(w.z >> 16), h16(w.w & 0xffffu), h16(w.w >> 16));
+  return make_float4(aa_cube(un16(w.z & 0xffffu)), h16(w.z >> 16), h16(w.w & 0xffffu), h16(w.w >> 16));
 }
 
 namespace {

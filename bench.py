@@ -56,6 +56,9 @@ def main() -> int:
                     help="camera requests of the batching leg (0 = skip the leg; the batching workload has 256)")
     ap.add_argument("--batch-particles", type=int, default=16777216)
     ap.add_argument("--batch-payload", type=int, default=1)
+    ap.add_argument("--frame-batch", type=int, default=4,
+                    help="orbit frames of the main workload per aarbf_render_batch call in the frame-batch leg "
+                         "(FP16 and FP32 payload; 0 = skip)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "render_traffic.json"),
                     help="HBM bytes per render launch measured with rocprofv3 --pmc")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -211,6 +214,8 @@ def main() -> int:
     ctx.set_timing(False)
     ctx.set_render_mode(0)
     ctx.set_payload(0)
+    fbatch = frame_batch_leg(ctx, stream, a, sc, wl, cams, msps if a.payload == 0 and a.mode == 0 else None) \
+        if a.frame_batch > 0 else None
     diff = diff_leg(ctx, stream, a) if a.diff_views > 0 else None
     batch = batch_leg(ctx, stream, a) if a.batch_views > 0 else None
 
@@ -247,6 +252,7 @@ def main() -> int:
             "valu": valu,
         },
         "cpu_baseline": cpu,
+        "frame_batch": fbatch,
         "diff": diff,
         "batch": batch,
         "wall_s": round(wall, 3),
@@ -254,6 +260,44 @@ def main() -> int:
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
+
+
+def frame_batch_leg(ctx, stream, a, sc, wl, cams, plain_msps):
+    """The main workload's orbit frames 1..K (1920x1080, 16 spp delta
+    tracking each) rendered by one aarbf_render_batch call, with the FP16
+    leaf-local payload and with the FP32 payload, against the plain FP32
+    path (one aarbf_render_rbf_spp call per frame, the line's value).  Each
+    view's image equals its own call's on the same payload bit for bit."""
+    import torch
+    W, H, S, K = wl.width, wl.height, wl.spp, a.frame_batch
+    bufs = [(torch.zeros((H, W, 4), dtype=torch.float32, device="cuda"),
+             torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")) for _ in range(K)]
+    reqs = []
+    for k in range(K):
+        f = 1 + k
+        pc = sc.push_constants(cams[f % len(cams)], wl.unit, frame_id=(f * S) & 0xFFFF, accum_id=1)
+        reqs.append((sc.raygen_data(bufs[k][0], bufs[k][1], W, H), pc, W, H, S))
+    out = {"workload": f"orbit frames 1..{K} of the main workload, one aarbf_render_batch call", "frames": K}
+    for payload, name in ((1, "fp16"), (0, "fp32")):
+        ctx.set_payload(payload)
+        ctx.render_batch(reqs)  # warm-up (slot pool, payload)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ctx.render_batch(reqs)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        out[f"{name}_batched_ms"] = round(ms, 2)
+        out[f"{name}_batched_msamples_per_s"] = round(W * H * S * K / (ms * 1e-3) / 1e6, 3)
+        log(f"frame batch ({name} payload): {K} frames in {ms:.1f} ms")
+    ctx.set_payload(0)
+    if plain_msps:
+        out["plain_fp32_msamples_per_s"] = round(plain_msps, 3)
+        out["fp16_batched_over_plain_fp32"] = round(out["fp16_batched_msamples_per_s"] / plain_msps, 3)
+    del bufs
+    torch.cuda.synchronize()
+    return out
 
 
 def diff_leg(ctx, stream, a):

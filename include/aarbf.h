@@ -293,11 +293,13 @@ int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg,
                          const aarbf_push_constants *pc, int width, int height, uint32_t spp);
 /* Particle payload the renderer bakes from the particles and TFs: 0 (default)
  * FP32, 32 bytes per particle; 1 FP16 leaf-local, 16 bytes per particle --
- * positions as 16-bit fixed point in the frame of their chunk of 32 particles
- * (origin and per-axis step computed in double), radius, alpha^3 and rgb^2.2
- * as binary16, the radius shrunk so each decoded sphere stays inside the
- * original one.  FP16 renders a slightly different scene; accumulation stays
- * FP32.  No counterpart in the reference. */
+ * positions and radius as 16-bit fixed point in the frame of their chunk of
+ * 32 particles (origin and per-axis step computed in double, stored as float;
+ * the radius shrunk by the position's rounding shift, so each decoded sphere
+ * stays inside the original one, and coded over the chunk's radius range),
+ * alpha as 16-bit unorm, rgb^2.2 as binary16.  FP16 renders a slightly
+ * different scene; accumulation stays FP32.  No counterpart in the
+ * reference. */
 int aarbf_set_payload(aarbf_context *ctx, int format);
 /* The decoded payload of the last render (x, y, z, r, alpha^3, r^2.2, g^2.2,
  * b^2.2 per particle) into device float out[8 * n]. */

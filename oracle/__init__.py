@@ -83,7 +83,7 @@ def lib() -> C.CDLL:
         L.orc_taa_resolve.argtypes = [f32p, f32p, f32p, C.c_int, C.c_int, C.c_int]
         L.orc_expected.restype = C.c_int
         L.orc_expected.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int32), C.c_uint64,
-                                   C.POINTER(C.c_double)]
+                                   C.POINTER(C.c_double), C.c_int]
         L.orc_splat.restype = C.c_int
         L.orc_splat.argtypes = [C.POINTER(OrcParams), f32p, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32),
                                 C.c_uint64, f32p]
@@ -299,18 +299,19 @@ class Render:
 
         return _Prepared()
 
-    def expected(self, pixels=None) -> np.ndarray:
+    def expected(self, pixels=None, threads: int = 8) -> np.ndarray:
         """FP64 quadrature of the delta-tracking expectation: [H, W, 3] for
-        the whole image, or [n, 3] for the given (x, y) pixels."""
+        the whole image, or [n, 3] for the given (x, y) pixels (pixels
+        shared among `threads` workers; the result does not depend on it)."""
         W, H = self.p.width, self.p.height
         if pixels is not None:
             px = np.ascontiguousarray(np.asarray(pixels, dtype=np.int32).reshape(-1, 2))
             out = np.zeros((px.shape[0], 3), np.float64)
             st = lib().orc_expected(C.byref(self.p), px.ctypes.data_as(C.POINTER(C.c_int32)), px.shape[0],
-                                    out.ctypes.data_as(C.POINTER(C.c_double)))
+                                    out.ctypes.data_as(C.POINTER(C.c_double)), threads)
         else:
             out = np.zeros((H, W, 3), np.float64)
-            st = lib().orc_expected(C.byref(self.p), None, 0, out.ctypes.data_as(C.POINTER(C.c_double)))
+            st = lib().orc_expected(C.byref(self.p), None, 0, out.ctypes.data_as(C.POINTER(C.c_double)), threads)
         if st != 0:
             raise RuntimeError("orc_expected failed (delta tracking only)")
         return out

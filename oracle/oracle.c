@@ -736,15 +736,17 @@ static void raygen(const orc_scene *s, int px, int py, uint32_t frameID, uint32_
   memcpy(image + 4 * ofs, fin, 16);
 }
 
-static int scene_init(orc_scene *s, const orc_params *p) {
+static int scene_init_grid(orc_scene *s, const orc_params *p, int grid) {
   memset(s, 0, sizeof(*s));
   s->p = p;
   s->bp = (baked_particle *)malloc(sizeof(baked_particle) * (p->numParticles ? p->numParticles : 1));
   if (!s->bp) return -1;
   bake(p, s->bp);
   orc_light_direction(p->light_azimuth, p->light_elevation, s->light);
-  return build_grid(s);
+  return grid ? build_grid(s) : 0;
 }
+
+static int scene_init(orc_scene *s, const orc_params *p) { return scene_init_grid(s, p, 1); }
 
 static void scene_free(orc_scene *s) {
   free(s->bp);
@@ -1127,8 +1129,42 @@ static int cmp_double(const void *a, const void *b) {
   return (x > y) - (x < y);
 }
 
-/* sphere spans of all particles along the ray over [0, tmax]; returns count */
-static uint32_t q_spans(const orc_scene *s, const double o[3], const double d[3], double tmax, q_span *sp) {
+/* per-ray work arrays, grown on demand */
+typedef struct {
+  q_span *sp;
+  double *cuts;
+  uint32_t *act;
+  uint64_t cap;
+  int oom;
+} q_scratch;
+
+static int q_reserve(q_scratch *w, uint64_t n) {
+  if (n <= w->cap) return 1;
+  uint64_t c = w->cap ? w->cap : 1024;
+  while (c < n) c *= 2;
+  q_span *sp = (q_span *)realloc(w->sp, sizeof(q_span) * c);
+  if (sp) w->sp = sp;
+  double *cuts = (double *)realloc(w->cuts, sizeof(double) * 2 * c);
+  if (cuts) w->cuts = cuts;
+  uint32_t *act = (uint32_t *)realloc(w->act, sizeof(uint32_t) * c);
+  if (act) w->act = act;
+  if (!sp || !cuts || !act) {
+    w->oom = 1;
+    return 0;
+  }
+  w->cap = c;
+  return 1;
+}
+
+static void q_scratch_free(q_scratch *w) {
+  free(w->sp);
+  free(w->cuts);
+  free(w->act);
+}
+
+/* sphere spans of all particles along the ray over [0, tmax]; returns
+ * count (0 with ws->oom set when the arrays cannot grow) */
+static uint32_t q_spans(const orc_scene *s, const double o[3], const double d[3], double tmax, q_scratch *ws) {
   const orc_params *p = s->p;
   uint32_t n = 0;
   for (uint32_t i = 0; i < p->numParticles; ++i) {
@@ -1142,9 +1178,10 @@ static uint32_t q_spans(const orc_scene *s, const double o[3], const double d[3]
     double w = sqrt(w2);
     double a = tc - w, c = tc + w;
     if (c < 0.0 || a > tmax) continue;
-    sp[n].t0 = a < 0.0 ? 0.0 : a;
-    sp[n].t1 = c > tmax ? tmax : c;
-    sp[n].p = i;
+    if (!q_reserve(ws, (uint64_t)n + 1)) return 0;
+    ws->sp[n].t0 = a < 0.0 ? 0.0 : a;
+    ws->sp[n].t1 = c > tmax ? tmax : c;
+    ws->sp[n].p = i;
     ++n;
   }
   return n;
@@ -1154,8 +1191,11 @@ static uint32_t q_spans(const orc_scene *s, const double o[3], const double d[3]
  * non-empty covering set, calling f(piece r, t0, t1, panel length) */
 typedef void (*q_piece_fn)(q_ray *r, double a, double b, double h, void *user);
 
-static void q_pieces(q_ray *r, double tmax, q_span *sp, double *cuts, uint32_t *act, q_piece_fn f, void *user) {
-  uint32_t ns = q_spans(r->s, r->o, r->d, tmax, sp);
+static void q_pieces(q_ray *r, double tmax, q_scratch *w, q_piece_fn f, void *user) {
+  uint32_t ns = q_spans(r->s, r->o, r->d, tmax, w);
+  q_span *sp = w->sp;
+  double *cuts = w->cuts;
+  uint32_t *act = w->act;
   uint32_t nc = 0;
   for (uint32_t k = 0; k < ns; ++k) {
     cuts[nc++] = sp[k].t0;
@@ -1194,12 +1234,6 @@ static void q_depth_piece(q_ray *r, double a, double b, double h, void *user) {
   }
 }
 
-typedef struct {
-  q_span *sp;
-  double *cuts;
-  uint32_t *act;
-} q_scratch;
-
 /* optical depth from x along dir to where it leaves the clip box */
 static double q_shadow_depth(const orc_scene *s, const double x[3], q_scratch *w) {
   const orc_params *p = s->p;
@@ -1217,7 +1251,7 @@ static double q_shadow_depth(const orc_scene *s, const double x[3], q_scratch *w
     r.d[d] = s->light[d];
   }
   q_depth u = {0.0};
-  q_pieces(&r, (double)tx, w->sp, w->cuts, w->act, q_depth_piece, &u);
+  q_pieces(&r, (double)tx, w, q_depth_piece, &u);
   return u.tau;
 }
 
@@ -1267,54 +1301,91 @@ static void q_primary_piece(q_ray *r, double a, double b, double h, void *user) 
   }
 }
 
+typedef struct {
+  const orc_scene *s;
+  const int32_t *pixels;
+  uint64_t n;
+  double *out;
+  uint64_t next;
+  int oom;
+  pthread_mutex_t lock;
+} q_job;
+
+static void q_pixel(const orc_scene *s, int px, int py, q_scratch *w, q_scratch *w2, double *e) {
+  const orc_params *p = s->p;
+  float sx = ((float)px + 0.5f) / (float)p->width, sy = ((float)py + 0.5f) / (float)p->height;
+  float rt[3], lb[3], dir[3];
+  for (int d = 0; d < 3; ++d) {
+    rt[d] = p->aabb_max[d] + 2.0f * p->rbfRadius;
+    lb[d] = p->aabb_min[d] - 2.0f * p->rbfRadius;
+    dir[d] = (p->cam_d00[d] + sx * p->cam_du[d]) + sy * p->cam_dv[d];
+  }
+  float len = sqrtf((dir[0] * dir[0] + dir[1] * dir[1]) + dir[2] * dir[2]);
+  for (int d = 0; d < 3; ++d) dir[d] = dir[d] / len;
+  float tenter, texit;
+  aabb_intersection(p->cam_pos, dir, 0.0f, 10000.0f, lb, rt, &tenter, &texit);
+  e[0] = e[1] = e[2] = 0.0;
+  if (!(tenter < texit)) return;
+  q_ray r;
+  r.s = s;
+  for (int d = 0; d < 3; ++d) {
+    r.o[d] = p->cam_pos[d];
+    r.d[d] = dir[d];
+  }
+  q_primary u = {1.0, {0.0, 0.0, 0.0}, w2};
+  q_pieces(&r, (double)texit, w, q_primary_piece, &u);
+  for (int c = 0; c < 3; ++c) e[c] = u.e[c];
+}
+
+static void *q_worker(void *arg) {
+  q_job *job = (q_job *)arg;
+  const orc_params *p = job->s->p;
+  q_scratch w = {0}, w2 = {0};
+  for (;;) {
+    pthread_mutex_lock(&job->lock);
+    uint64_t i = job->next++;
+    pthread_mutex_unlock(&job->lock);
+    if (i >= job->n) break;
+    int px = job->pixels ? job->pixels[2 * i] : (int)(i % (uint64_t)p->width);
+    int py = job->pixels ? job->pixels[2 * i + 1] : (int)(i / (uint64_t)p->width);
+    q_pixel(job->s, px, py, &w, &w2, job->out + 3 * i);
+    if (w.oom || w2.oom) break;
+  }
+  pthread_mutex_lock(&job->lock);
+  if (w.oom || w2.oom) job->oom = 1;
+  pthread_mutex_unlock(&job->lock);
+  q_scratch_free(&w);
+  q_scratch_free(&w2);
+  return NULL;
+}
+
 /* Expected RGB of the given pixels (all when pixels == NULL) for delta
- * tracking (enableBlueNoise = 0); out = 3 doubles per pixel. */
-ORC_EXPORT int orc_expected(const orc_params *p, const int32_t *pixels, uint64_t npix, double *out) {
+ * tracking (enableBlueNoise = 0); out = 3 doubles per pixel; pixels are
+ * shared among `threads` workers (each pixel's sum is sequential, so the
+ * result does not depend on the thread count). */
+ORC_EXPORT int orc_expected(const orc_params *p, const int32_t *pixels, uint64_t npix, double *out, int threads) {
   if (p->enableBlueNoise) return -1;
   orc_scene s;
-  if (scene_init(&s, p) != 0) return -1;
-  uint64_t np = (uint64_t)p->numParticles + 1;
-  q_span *sp = (q_span *)malloc(sizeof(q_span) * np), *sp2 = (q_span *)malloc(sizeof(q_span) * np);
-  double *cuts = (double *)malloc(sizeof(double) * 2 * np), *cuts2 = (double *)malloc(sizeof(double) * 2 * np);
-  uint32_t *act = (uint32_t *)malloc(sizeof(uint32_t) * np), *act2 = (uint32_t *)malloc(sizeof(uint32_t) * np);
-  int rc = (sp && sp2 && cuts && cuts2 && act && act2) ? 0 : -1;
-  const uint64_t n = pixels ? npix : (uint64_t)p->width * p->height;
-  for (uint64_t i = 0; rc == 0 && i < n; ++i) {
-    int px = pixels ? pixels[2 * i] : (int)(i % (uint64_t)p->width);
-    int py = pixels ? pixels[2 * i + 1] : (int)(i / (uint64_t)p->width);
-    float sx = ((float)px + 0.5f) / (float)p->width, sy = ((float)py + 0.5f) / (float)p->height;
-    float rt[3], lb[3], dir[3];
-    for (int d = 0; d < 3; ++d) {
-      rt[d] = p->aabb_max[d] + 2.0f * p->rbfRadius;
-      lb[d] = p->aabb_min[d] - 2.0f * p->rbfRadius;
-      dir[d] = (p->cam_d00[d] + sx * p->cam_du[d]) + sy * p->cam_dv[d];
-    }
-    float len = sqrtf((dir[0] * dir[0] + dir[1] * dir[1]) + dir[2] * dir[2]);
-    for (int d = 0; d < 3; ++d) dir[d] = dir[d] / len;
-    float tenter, texit;
-    aabb_intersection(p->cam_pos, dir, 0.0f, 10000.0f, lb, rt, &tenter, &texit);
-    double *e = out + 3 * i;
-    e[0] = e[1] = e[2] = 0.0;
-    if (!(tenter < texit)) continue;
-    q_ray r;
-    r.s = &s;
-    for (int d = 0; d < 3; ++d) {
-      r.o[d] = p->cam_pos[d];
-      r.d[d] = dir[d];
-    }
-    q_scratch sh = {sp2, cuts2, act2};
-    q_primary u = {1.0, {0.0, 0.0, 0.0}, &sh};
-    q_pieces(&r, (double)texit, sp, cuts, act, q_primary_piece, &u);
-    for (int c = 0; c < 3; ++c) e[c] = u.e[c];
-  }
-  free(sp);
-  free(sp2);
-  free(cuts);
-  free(cuts2);
-  free(act);
-  free(act2);
+  if (scene_init_grid(&s, p, 0) != 0) return -1;
+  q_job job;
+  job.s = &s;
+  job.pixels = pixels;
+  job.n = pixels ? npix : (uint64_t)p->width * p->height;
+  job.out = out;
+  job.next = 0;
+  job.oom = 0;
+  pthread_mutex_init(&job.lock, NULL);
+  if (threads < 1) threads = 1;
+  if (threads > 64) threads = 64;
+  pthread_t th[64];
+  int started = 0;
+  for (int k = 1; k < threads; ++k)
+    if (pthread_create(&th[started], NULL, q_worker, &job) == 0) ++started;
+  q_worker(&job);
+  for (int k = 0; k < started; ++k) pthread_join(th[k], NULL);
+  pthread_mutex_destroy(&job.lock);
   scene_free(&s);
-  return rc;
+  return job.oom ? -1 : 0;
 }
 
 /* ======================================================================

@@ -1,8 +1,10 @@
 """FP16 leaf-local payload (aarbf_set_payload(ctx, 1)): the decoded payload
 keeps every sphere inside the original one (so the FP32-built BVH stays
-valid) with the stated precision, the renderer's images equal the oracle's
-bit for bit on the decoded scene, and converged images stay within the
-stated RMSE of the FP64 quadrature of the original scene."""
+valid) with the stated precision (positions within half a chunk step, the
+radius within one step of its chunk's radius code, alpha within half a
+16-bit unorm step, rgb^2.2 within 2^-11 relative), the renderer's images
+equal the oracle's bit for bit on the decoded scene, and converged images
+stay within the stated RMSE of the FP64 quadrature of the original scene."""
 import numpy as np
 import pytest
 
@@ -55,11 +57,21 @@ def test_fp16_payload_bounds(gpu_ctx):
     assert sc.n % 32 != 0 and np.all(shift <= bound)
     live = ref[:, 3] > 0
     assert np.all(q[live, 3] + shift[live] <= ref[live, 3])          # decoded sphere inside the original
-    assert np.all(q[live, 3] >= (ref[live, 3] - shift[live]) * (1 - 2 ** -10) * (1 - 2 ** -10) - 1e-7)
-    # alpha is stored (alpha^3 = its cube, so 3x the relative error), rgb^2.2 directly
-    # (no subnormal halves: values below 2^-14 flush to 0, alpha is at least 2^-14)
-    np.testing.assert_allclose(q[:, 4], ref[:, 4], rtol=3.01 * 2 ** -11, atol=2.0 ** -42)
-    np.testing.assert_allclose(q[:, 5:], ref[:, 5:], rtol=2 ** -11, atol=2 ** -14)
+    assert np.all(q[~live, 3] == 0)
+    # radius: 16-bit code over the chunk's range of shrunk radii, rounded down
+    shrunk = np.where(live, ref[:, 3].astype(np.float64) - shift, np.nan)
+    lo = np.empty(sc.n)
+    for c0 in range(0, sc.n, 32):
+        c = shrunk[c0:c0 + 32]
+        step = (np.nanmax(c) - np.nanmin(c)) / 65534.0 if np.isfinite(c).any() else 0.0
+        lo[c0:c0 + 32] = c - 1.01 * step - 1e-6 * c
+    assert np.all(q[live, 3] >= lo[live]) and np.all(q[live, 3] > 0)
+    # alpha as 16-bit unorm (alpha^3 = its cube), rgb^2.2 as binary16 with
+    # subnormals (the decode must not flush them: absolute error 2^-25 below 2^-14)
+    np.testing.assert_allclose(np.cbrt(q[:, 4]), np.cbrt(ref[:, 4]), rtol=1e-6, atol=0.51 / 65535)
+    np.testing.assert_allclose(q[:, 5:], ref[:, 5:], rtol=2 ** -11, atol=2 ** -25)
+    dark = (ref[:, 5:] > 0) & (ref[:, 5:] < 2 ** -14)
+    assert np.all(q[:, 5:][dark] > 0)
     assert np.all((q[:, 4] > 0) == (ref[:, 4] > 0))  # no weight rounds to zero
 
 
