@@ -57,6 +57,14 @@ constexpr int kGBlock = 128; // gather kernel block
 constexpr int kSBlock = 64;  // step kernel block (one wave)
 constexpr uint64_t kPassSamples = 64; // samples per colour-buffer pass
 constexpr uint32_t kDmLds = 64;       // density-map texels cached in LDS
+// Gather queues up to this length go to wf_gather_wave (one wave per ray):
+// late in a call the few rays left each need one long tree walk, which one
+// lane alone runs at memory latency.
+#ifndef AARBF_WAVE_GATHER
+#define AARBF_WAVE_GATHER 16384
+#endif
+constexpr uint32_t kWaveGather = AARBF_WAVE_GATHER;
+constexpr uint32_t kWaveFront = 256; // breadth-first frontier entries
 // Empty-space steps a lane may take per iteration of the step loop: a wave
 // iterates as long as its slowest lane, and most steps are empty-space
 // steps, so taking several per iteration cuts the iterations (and the
@@ -439,6 +447,7 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
   // resident)
   extern __shared__ uint32_t s_stack[];
   const uint32_t n_g = *countG;
+  if (n_g <= kWaveGather) return;            // wf_gather_wave takes short queues
   if (blockIdx.x * kGBlock >= n_g) return; // the whole block: no barrier is skipped
   load_levels(A);
   Counters c = {};
@@ -705,6 +714,329 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
 __device__ inline uint32_t round_cap(uint32_t base_cap, uint32_t nslots, uint32_t n_active) {
   const uint64_t b = base_cap;
   return (uint32_t)min(65536ull, max(b, b * nslots / (4ull * max(n_active, 1u))));
+}
+
+
+// ---- gather, one wave per ray ---------------------------------------------------
+// The same candidate list and segment end as wf_gather, for short queues.
+// The wave first expands the tree breadth first from the root (64 child
+// boxes per pass) until at least 32 subtrees meet the segment, then every
+// lane walks its share of them front to back as wf_gather does, keeping its
+// own kCand nearest entries.  The lanes share a bound on the result's
+// segment end (the smallest of their own cuts, an LDS atomic): a lane's cut
+// is the entry of a particle that does not fit among its own kCand, so at
+// least kCand + 1 particles enter by then and no particle entering later
+// can be in the result or end its segment; boxes entered after it (with the
+// rounding margin) are skipped.  A merge then takes the kCand smallest
+// entries over all lanes; the segment ends at the next entry or the
+// smallest cut, whichever comes first -- the (kCand + 1)-th smallest entry
+// overall, as in wf_gather.  Lists that differ only among equal entries
+// give the same image: no particle entering at the segment end can cover a
+// step before it.
+__device__ inline int ord_key(float f) {
+  const int i = __float_as_int(f);
+  return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ inline float ord_val(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
+
+template <bool STATS>
+__global__ __launch_bounds__(64) void wf_gather_wave(RenderArgs A, WFState S, const uint32_t *__restrict__ listG,
+                                                     const uint32_t *__restrict__ countG) {
+  __shared__ uint32_t s_idx[kCand * 64];
+  __shared__ float s_tin[kCand * 64];
+  __shared__ uint32_t s_stack[kMaxLevels * 64];
+  __shared__ uint32_t s_front[2][kWaveFront];
+  __shared__ uint32_t s_ridx[kCand];
+  __shared__ float s_rtin[kCand];
+  __shared__ int s_bound;
+  const uint32_t n_g = *countG;
+  if (n_g > kWaveGather || blockIdx.x >= n_g) return; // the whole block
+  load_levels(A);
+  const uint32_t lane = threadIdx.x;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  Counters c = {};
+  const QGroup *__restrict__ Qg = A.bvh.groups;
+  const uint32_t log2K = A.bvh.log2K, K = 1u << log2K, top = A.bvh.top;
+  uint32_t *gi = s_idx + lane, *stk = s_stack + lane;
+  float *gtin = s_tin + lane;
+  for (uint32_t item = blockIdx.x; item < n_g; item += gridDim.x) {
+    const uint32_t slot = listG[item];
+    const float4 a0 = S.od0[slot], a1 = S.od1[slot];
+    const float3 o = make_float3(a0.x, a0.y, a0.z), d = make_float3(a1.x, a1.y, a1.z);
+    const uint4 u = S.u[slot];
+    uint32_t flags = u.z >> 16;
+    const bool head = (flags & F_HEADGATHER) != 0;
+    const uint32_t pix = S.pix[slot];
+    const float t0 = head ? S.seg[slot].x : a0.w, tmax = a1.w;
+    SegQuery q;
+    q.ax = o.x + t0 * d.x;
+    q.ay = o.y + t0 * d.y;
+    q.az = o.z + t0 * d.z;
+    q.dx = d.x;
+    q.dy = d.y;
+    q.dz = d.z;
+    q.ix = safe_inv(d.x);
+    q.iy = safe_inv(d.y);
+    q.iz = safe_inv(d.z);
+    const float m = 2e-5f * (1.0f + fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) + fmaxf(fabsf(t0), fabsf(tmax)));
+    q.m = m;
+    q.len = tmax - t0 + 4.0f * m;
+    uint32_t n = 0, maxpos = 0;
+    float maxtin = -AA_INF_F, t_cut = AA_INF_F;
+    if (STATS && lane == 0) c.queries++;
+    if (lane == 0) s_bound = ord_key(AA_INF_F);
+    // breadth first from the root
+    const float4 *root = A.bvh.nodes + g_lvl_ofs[top];
+    uint32_t nf = 0, lvl = top, cur = 0;
+    if (STATS && lane == 0) c.nodes++;
+    if (seg_box(q, root[0], root[1])) {
+      if (lane == 0) s_front[0][0] = 0;
+      nf = 1;
+    }
+    __syncthreads();
+    while (nf > 0 && nf < 32 && lvl > 1) {
+      const uint32_t total = nf << log2K; // <= 31 * 8 < kWaveFront
+      uint32_t nn = 0;
+      for (uint32_t base = 0; base < total; base += 64u) {
+        const uint32_t j = base + lane;
+        bool hit = false;
+        uint32_t child = 0;
+        if (j < total) {
+          const uint32_t node = s_front[cur][j >> log2K], k = j & (K - 1u);
+          const QGroup g = Qg[g_lvl_qofs[lvl] + node];
+          const uint32_t first = node << log2K, mcnt = min(K, g_lvl_cnt[lvl - 1] - first);
+          const uint32_t lx = qbyte(g, 0, k), hx = qbyte(g, 3, k);
+          child = first + k;
+          if (k < mcnt && lx <= hx) {
+            float4 lo, hi;
+            lo.x = g.a.x + (float)lx * g.a.w;
+            lo.y = g.a.y + (float)qbyte(g, 1, k) * g.b.x;
+            lo.z = g.a.z + (float)qbyte(g, 2, k) * g.b.y;
+            hi.x = g.a.x + (float)hx * g.a.w;
+            hi.y = g.a.y + (float)qbyte(g, 4, k) * g.b.x;
+            hi.z = g.a.z + (float)qbyte(g, 5, k) * g.b.y;
+            float tn;
+            hit = seg_box_t(q, lo, hi, tn);
+          }
+          if (STATS && k < mcnt) c.nodes++;
+        }
+        const uint64_t mk = __ballot(hit);
+        if (hit) s_front[cur ^ 1u][nn + (uint32_t)__popcll(mk & lt)] = child;
+        nn += (uint32_t)__popcll(mk);
+      }
+      __syncthreads();
+      cur ^= 1u;
+      nf = nn;
+      --lvl;
+    }
+    // each lane's subtrees, front to back
+    auto cut_now = [&]() { return fminf(t_cut, ord_val(__hip_atomic_load(&s_bound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))); };
+    auto chord = [&](const float4 &P, float &tin, float &tout) -> bool {
+      float vx = P.x - o.x, vy = P.y - o.y, vz = P.z - o.z;
+      float tc = vx * d.x + vy * d.y + vz * d.z;
+      float hx = vx - tc * d.x, hy = vy - tc * d.y, hz = vz - tc * d.z;
+      float rr = P.w + m;
+      float w2 = rr * rr - (hx * hx + hy * hy + hz * hz);
+      if (!(w2 >= 0.0f)) return false;
+      float w = sqrtf(w2);
+      tin = tc - w - m;
+      tout = tc + w + m;
+      return true;
+    };
+    auto lower_cut = [&](float v) {
+      if (v < t_cut) {
+        t_cut = v;
+        atomicMin(&s_bound, ord_key(v));
+      }
+    };
+    auto take = [&](uint32_t p, const float4 &P) {
+      if (!(P.w > 0.0f)) return;
+      float tin, tout;
+      if (!chord(P, tin, tout)) return;
+      if (!(tout >= t0) || !(tin <= tmax) || !(tin < t_cut)) {
+        if (tout >= t0 && tin <= tmax) lower_cut(tin);
+        return;
+      }
+      if (n < kCand) {
+        gi[n * 64] = p;
+        gtin[n * 64] = tin;
+        if (tin > maxtin) {
+          maxtin = tin;
+          maxpos = n;
+        }
+        ++n;
+        return;
+      }
+      if (tin < maxtin) { // evict the latest-entering candidate
+        lower_cut(maxtin);
+        gi[maxpos * 64] = p;
+        gtin[maxpos * 64] = tin;
+        maxtin = -AA_INF_F;
+        for (uint32_t k = 0; k < kCand; ++k) {
+          float v = gtin[k * 64];
+          if (v > maxtin) {
+            maxtin = v;
+            maxpos = k;
+          }
+        }
+      } else {
+        lower_cut(tin);
+      }
+    };
+    auto leaf_batch = [&](uint32_t lb, uint32_t ord) {
+      if (A.ppl == 1) {
+        float4 P[8];
+        uint32_t pid[8];
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r) {
+          uint32_t nib = (ord >> (4u * r)) & 15u;
+          pid[r] = nib ? lb + nib - 1u : 0xffffffffu;
+          if (nib) P[r] = ld_pr(A, pid[r]);
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r)
+          if (pid[r] != 0xffffffffu) {
+            if (STATS) c.particles++;
+            take(pid[r], P[r]);
+          }
+        return;
+      }
+      for (; ord; ord >>= 4) {
+        uint32_t leaf = lb + (ord & 15u) - 1u;
+        uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
+        for (uint32_t p = p0; p < p1; ++p) {
+          if (STATS) c.particles++;
+          take(p, ld_pr(A, p));
+        }
+      }
+    };
+    auto children = [&](uint32_t l, uint32_t node) {
+      q.len = fminf(tmax, cut_now() + 4.0f * m) - t0;
+      if (STATS) c.nodes += K;
+      return ftb_children(A, q, l, node);
+    };
+    for (uint32_t fi = lane; fi < nf; fi += 64u) {
+      const uint32_t R = s_front[cur][fi], L = lvl - 1u; // R at level lvl >= 1
+      uint32_t l = L, base = R << log2K;
+      uint32_t ord = children(lvl, R);
+      for (;;) {
+        if (l == 0) { // a whole leaf group, in entry order
+          leaf_batch(base, ord);
+          ord = 0;
+        }
+        const uint32_t nib = ord & 15u;
+        if (nib == 0) {
+          if (l == L) break;
+          ++l;
+          ord = stk[(l - 1u) * 64u];
+          base = ((base >> log2K) >> log2K) << log2K;
+          continue;
+        }
+        ord >>= 4;
+        const uint32_t node = base + nib - 1u;
+        stk[(l - 1u) * 64u] = ord; // l >= 1 here
+        base = node << log2K;
+        ord = children(l, node);
+        --l;
+      }
+    }
+    // merge: the kCand smallest entries over the lanes, then the next one
+    float cut_all = t_cut;
+    for (int o2 = 32; o2 > 0; o2 >>= 1) cut_all = fminf(cut_all, __shfl_xor(cut_all, o2, 64));
+    uint32_t taken = 0, nres = 0;
+    float t_next = AA_INF_F;
+    for (uint32_t r = 0; r <= kCand; ++r) {
+      float best = AA_INF_F;
+      uint32_t bk = 0;
+      for (uint32_t k = 0; k < n; ++k) {
+        const float v = gtin[k * 64];
+        if (!((taken >> k) & 1u) && v < best) {
+          best = v;
+          bk = k;
+        }
+      }
+      float v = best;
+      uint32_t who = lane;
+      for (int o2 = 32; o2 > 0; o2 >>= 1) {
+        const float v2 = __shfl_xor(v, o2, 64);
+        const uint32_t w2 = __shfl_xor(who, o2, 64);
+        if (v2 < v || (v2 == v && w2 < who)) {
+          v = v2;
+          who = w2;
+        }
+      }
+      if (!(v < AA_INF_F)) break; // every lane's list is used up
+      if (r == kCand) {
+        t_next = v;
+        break;
+      }
+      if (lane == who) {
+        taken |= 1u << bk;
+        s_ridx[r] = gi[bk * 64];
+        s_rtin[r] = gtin[bk * 64];
+      }
+      ++nres;
+    }
+    t_cut = fminf(t_next, cut_all);
+    __syncthreads();
+    if (lane == 0) {
+      float seg_end;
+      n = nres;
+      if (t_cut <= t0 + 2.0f * A.unit) { // crowded: point queries for a short stretch
+        flags |= F_DENSE;
+        seg_end = t0 + 4.0f * A.unit;
+        n = 0;
+      } else {
+        flags &= ~F_DENSE;
+        uint32_t *oidx = head ? S.hidx : S.cidx;
+        float *otin = head ? S.htin : S.ctin, *otout = head ? S.htout : S.ctout;
+        uint8_t *oeo = head ? S.heo : S.ceo;
+        const uint64_t row = head ? pix : slot, stride = head ? S.npix : S.nslots;
+        seg_end = t_cut;
+        // ascending particle order (insertion sort, n <= kCand)
+        for (uint32_t a = 1; a < n; ++a) {
+          const uint32_t pi = s_ridx[a];
+          const float ti = s_rtin[a];
+          uint32_t b = a;
+          while (b > 0 && s_ridx[b - 1] > pi) {
+            s_ridx[b] = s_ridx[b - 1];
+            s_rtin[b] = s_rtin[b - 1];
+            --b;
+          }
+          s_ridx[b] = pi;
+          s_rtin[b] = ti;
+        }
+        // entry order: position of rank r in tin order (ties by position)
+        for (uint32_t k = 0; k < n; ++k) {
+          const float tk = s_rtin[k];
+          uint32_t r = 0;
+          for (uint32_t j = 0; j < n; ++j) {
+            const float tj = s_rtin[j];
+            r += (tj < tk) || (tj == tk && j < k);
+          }
+          const uint32_t pk = s_ridx[k];
+          float tin, tout;
+          chord(ld_pr(A, pk), tin, tout);
+          oeo[r * stride + row] = (uint8_t)k;
+          oidx[k * stride + row] = pk;
+          otin[k * stride + row] = tk;
+          otout[k * stride + row] = tout;
+        }
+      }
+      if (head) {
+        flags = (flags & ~F_HEADGATHER) | F_HEAD;
+        S.head[pix] = make_float4(t0, seg_end, __uint_as_float(n),
+                                  __uint_as_float(1u | ((flags & F_DENSE) ? 2u : 0u)));
+      } else {
+        flags &= ~F_HEAD;
+      }
+      S.u[slot] = make_uint4(u.x, u.y, (u.z & 0xffffu) | (flags << 16), n);
+      const float4 g = S.seg[slot];
+      S.seg[slot] = make_float4(t0, seg_end, g.z, g.w);
+    }
+    __syncthreads(); // s_ridx, s_bound and the frontier are reused by the next ray
+  }
+  flush_stats<STATS>(A, c);
 }
 
 // ---- step ------------------------------------------------------------------------
@@ -1518,19 +1850,35 @@ __global__ void wf_lat_start(unsigned long long *lat, uint32_t nv) {
 // kernels read their counts on the device and loop over them).
 template <bool B>
 static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &S, uint32_t *const lists[4],
-                        uint32_t *listD, uint32_t *cnt, int cur, uint32_t gb, uint32_t sb, uint32_t db,
-                        uint32_t *nD_out) {
+                        uint32_t *listD, uint32_t *cnt, int cur, uint32_t nG, uint32_t gb, uint32_t sb,
+                        uint32_t db, uint32_t *nD_out) {
   const bool st = ctx->stats;
   uint32_t *lA = lists[cur], *lG = lists[cur + 1], *nA2 = lists[2 - cur], *nG2 = lists[3 - cur];
   uint32_t *cA = cnt + (cur ? 2 : 0), *cN = cnt + (cur ? 0 : 2);
   const bool tm = ctx->timing && db == 0;
   if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[0], ctx->stream));
+  // gathers: one lane per ray for long queues, one wave per ray for short
+  // ones (each kernel returns at once on the other's queues).  Host-driven
+  // rounds pass the queue length (nG) and launch the one that applies;
+  // captured rounds (nG == ~0u) launch both with fixed grids.
   if (gb > 0) {
-    const size_t stack_bytes = sizeof(uint32_t) * kGBlock * std::max<uint32_t>(A.bvh.top, 2u) - sizeof(uint32_t) * kGBlock;
-    if (st)
-      wf_gather<true><<<gb, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1);
-    else
-      wf_gather<false><<<gb, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1);
+    const bool fixed = nG == ~0u;
+    if (fixed || nG > kWaveGather) {
+      const size_t stack_bytes = sizeof(uint32_t) * kGBlock * std::max<uint32_t>(A.bvh.top, 2u) - sizeof(uint32_t) * kGBlock;
+      if (st)
+        wf_gather<true><<<gb, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1);
+      else
+        wf_gather<false><<<gb, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1);
+      AARBF_CHECK_LAUNCH(ctx, "wf_gather");
+    }
+    if (fixed || nG <= kWaveGather) {
+      const uint32_t wb = fixed ? (uint32_t)std::min<uint64_t>(kWaveGather, 16ull * (uint64_t)std::max(ctx->cus, 1)) : nG;
+      if (st)
+        wf_gather_wave<true><<<wb, 64, 0, ctx->stream>>>(A, S, lG, cA + 1);
+      else
+        wf_gather_wave<false><<<wb, 64, 0, ctx->stream>>>(A, S, lG, cA + 1);
+      AARBF_CHECK_LAUNCH(ctx, "wf_gather_wave");
+    }
   }
   wf_zero<<<1, 64, 0, ctx->stream>>>(cN, 2);
   wf_zero<<<1, 64, 0, ctx->stream>>>(cnt + 4, 1);
@@ -1631,7 +1979,7 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
       AARBF_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       int rc = AARBF_OK;
       for (int r = 0; r < R && rc == AARBF_OK; ++r)
-        rc = launch_round<B>(ctx, A, S, lists, listD, cnt, (r & 1) ? 2 : 0, gb, sb, db, nullptr);
+        rc = launch_round<B>(ctx, A, S, lists, listD, cnt, (r & 1) ? 2 : 0, ~0u, gb, sb, db, nullptr);
       hipGraph_t graph = nullptr;
       const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
       if (rc != AARBF_OK) {
@@ -1661,7 +2009,7 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
   while (nA > 0) {
     uint32_t nD = 0;
     const uint32_t gb = (nG + kGBlock - 1) / kGBlock, sb = (nA + kSBlock - 1) / kSBlock;
-    int rc = launch_round<B>(ctx, A, S, lists, listD, cnt, cur, gb, sb, 0, &nD);
+    int rc = launch_round<B>(ctx, A, S, lists, listD, cnt, cur, nG, gb, sb, 0, &nD);
     if (rc != AARBF_OK) return rc;
     uint32_t *cN = cnt + (cur ? 0 : 2);
     AARBF_HIP(ctx, hipMemcpyAsync(h, cN, 8, hipMemcpyDeviceToHost, ctx->stream));
