@@ -64,7 +64,11 @@ constexpr uint32_t kDmLds = 64;       // density-map texels cached in LDS
 #define AARBF_WAVE_GATHER 16384
 #endif
 constexpr uint32_t kWaveGather = AARBF_WAVE_GATHER;
-constexpr uint32_t kWaveFront = 256; // breadth-first frontier entries
+#ifndef AARBF_WAVE_SPLIT
+#define AARBF_WAVE_SPLIT 32
+#endif
+constexpr uint32_t kWaveSplit = AARBF_WAVE_SPLIT;   // subtrees the wave splits a walk into (at least)
+constexpr uint32_t kWaveFront = 8u * kWaveSplit; // breadth-first frontier entries (K <= 8)
 // Empty-space steps a lane may take per iteration of the step loop: a wave
 // iterates as long as its slowest lane, and most steps are empty-space
 // steps, so taking several per iteration cuts the iterations (and the
@@ -720,7 +724,7 @@ __device__ inline uint32_t round_cap(uint32_t base_cap, uint32_t nslots, uint32_
 // ---- gather, one wave per ray ---------------------------------------------------
 // The same candidate list and segment end as wf_gather, for short queues.
 // The wave first expands the tree breadth first from the root (64 child
-// boxes per pass) until at least 32 subtrees meet the segment, then every
+// boxes per pass) until at least kWaveSplit subtrees meet the segment, then every
 // lane walks its share of them front to back as wf_gather does, keeping its
 // own kCand nearest entries.  The lanes share a bound on the result's
 // segment end (the smallest of their own cuts, an LDS atomic): a lane's cut
@@ -794,8 +798,8 @@ __global__ __launch_bounds__(64) void wf_gather_wave(RenderArgs A, WFState S, co
       nf = 1;
     }
     __syncthreads();
-    while (nf > 0 && nf < 32 && lvl > 1) {
-      const uint32_t total = nf << log2K; // <= 31 * 8 < kWaveFront
+    while (nf > 0 && nf < kWaveSplit && lvl > 1) {
+      const uint32_t total = nf << log2K; // < kWaveSplit * 8 = kWaveFront
       uint32_t nn = 0;
       for (uint32_t base = 0; base < total; base += 64u) {
         const uint32_t j = base + lane;
