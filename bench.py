@@ -105,6 +105,20 @@ def main() -> int:
     prep_s = time.time() - tp
     log(f"pre-process + bounds + BVH: {prep_s * 1e3:.1f} ms (includes host->device copy)")
 
+    # a timestep change (leaf bounds + refit, hostCode.cpp:790-794) and a
+    # radius / transfer-function change (leaf bounds + a new tree, :781-789)
+    def upd_ms(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        t = time.time()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.time() - t) * 1e3 / reps
+
+    refit_ms, rebuild_ms = upd_ms(sc.refit), upd_ms(sc.rebuild)
+    log(f"bounds + refit {refit_ms:.2f} ms, bounds + rebuild {rebuild_ms:.2f} ms")
+
     image = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
     accum = torch.zeros_like(image)
     taa = torch.zeros_like(image)
@@ -239,6 +253,7 @@ def main() -> int:
             "bvh_branching": 8, "unit": wl.unit, "render_mode": a.mode,
             "payload": ("fp32", "fp16 leaf-local quantised")[a.payload], "max_depth": aarbf.MAX_DEPTH,
             "exhausted_fraction": st["exhausted"] / samples, "preprocess_ms": round(prep_s * 1e3, 1),
+            "refit_ms": round(refit_ms, 2), "rebuild_ms": round(rebuild_ms, 2),
             "parallelism": "single GPU",
         },
         "roofline": {
