@@ -306,6 +306,14 @@ def frame_batch_leg(ctx, stream, a, sc, wl, cams, plain_msps):
         out[f"{name}_batched_ms"] = round(ms, 2)
         out[f"{name}_batched_msamples_per_s"] = round(W * H * S * K / (ms * 1e-3) / 1e6, 3)
         log(f"frame batch ({name} payload): {K} frames in {ms:.1f} ms")
+        if payload == 1:  # the same frames one call each
+            e0.record(stream)
+            for rq in reqs:
+                ctx.render(*rq)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms1 = e0.elapsed_time(e1)
+            out["fp16_one_call_msamples_per_s"] = round(W * H * S * K / (ms1 * 1e-3) / 1e6, 3)
     ctx.set_payload(0)
     if plain_msps:
         out["plain_fp32_msamples_per_s"] = round(plain_msps, 3)
