@@ -148,6 +148,7 @@ class Context:
     TUNE_STEP_WAVES = 1
     TUNE_STEP_CAP = 2
     TUNE_GRAPH_ROUNDS = 3
+    TUNE_WALK_BUDGET = 4
 
     def set_tuning(self, param: int, value: int):
         """Schedule parameters of the default renderer (images do not change)."""

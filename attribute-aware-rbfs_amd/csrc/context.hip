@@ -177,6 +177,10 @@ int aarbf_set_tuning(aarbf_context *ctx, int param, int value) {
                   "aarbf_set_tuning: graph rounds must be 0 or an even number up to 64");
     ctx->tune_graph_rounds = value;
     return AARBF_OK;
+  case AARBF_TUNE_WALK_BUDGET:
+    AARBF_REQUIRE(ctx, value >= 1 && value <= 65536, "aarbf_set_tuning: walk budget must be in [1, 65536]");
+    ctx->tune_walk_budget = value;
+    return AARBF_OK;
   default:
     return set_error(ctx, AARBF_ERR_INVALID_ARGUMENT, "aarbf_set_tuning: unknown parameter");
   }

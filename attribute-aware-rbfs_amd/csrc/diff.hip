@@ -28,12 +28,18 @@
 // where a sphere boundary crosses a march point are not differentiated.
 #include "render_common.h"
 
+#include <algorithm>
+
 namespace aarbf {
 namespace {
 
 constexpr int kDBlock = 64;
 constexpr uint32_t kTexMax = 256; // texels per table with LDS gradient sums (larger tables: global atomics)
-constexpr uint32_t kCache = 4;    // per-lane particle adjoint cache entries
+#ifndef AARBF_DIFF_CACHE
+#define AARBF_DIFF_CACHE 1
+#endif
+constexpr uint32_t kCache = AARBF_DIFF_CACHE; // per-lane particle adjoint cache entries (a power of 2; 1, 2, 4
+                                              // measured: 1 fastest, the LDS it frees keeps more waves resident)
 constexpr uint32_t kSlot = 9;     // particle index + 8 adjoint sums
 #ifndef AARBF_DIFF_CAND
 #define AARBF_DIFF_CAND 12
@@ -409,7 +415,7 @@ __device__ inline void colour(const DPay &pl, float c[3]) {
 #define AARBF_DSEG_LDS                                                                                      \
   __shared__ uint32_t s_cidx[kDC * kDBlock];                                                                  \
   __shared__ float s_ctin[kDC * kDBlock], s_ctout[kDC * kDBlock];                                             \
-  __shared__ uint32_t s_stk[kStackLevels * kDBlock];                                                          \
+  extern __shared__ uint32_t s_stk[]; /* walk stacks, max(top, 2) words per lane (sized at launch) */         \
   const DSeg W{s_cidx + threadIdx.x, s_ctin + threadIdx.x, s_ctout + threadIdx.x, s_stk + threadIdx.x};
 
 __global__ __launch_bounds__(kDBlock) void diff_forward(DiffArgs D) {
@@ -752,6 +758,14 @@ static int diff_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const aar
   return AARBF_OK;
 }
 
+// The march's walk stacks in LDS: levels 0 .. top-1 of the tree per lane
+// (a tree that has no quantised groups is never walked: two words).  Sized
+// to the tree, which keeps more blocks resident than the 16-level maximum.
+static size_t diff_stack_bytes(const DiffArgs &D) {
+  const uint32_t levels = D.A.bvh.groups ? std::max<uint32_t>(D.A.bvh.top, 2u) : 2u;
+  return sizeof(uint32_t) * kDBlock * levels;
+}
+
 extern "C" {
 
 int aarbf_set_diff_estimator(aarbf_context *ctx, int estimator) {
@@ -772,7 +786,7 @@ int aarbf_diff_forward(aarbf_context *ctx, const aarbf_raygen_data *rg, const aa
   D.out = (float4 *)image;
   D.path = (float4 *)path;
   dim3 grid((W + 7) / 8, (H + 7) / 8);
-  diff_forward<<<grid, kDBlock, 0, ctx->stream>>>(D);
+  diff_forward<<<grid, kDBlock, diff_stack_bytes(D), ctx->stream>>>(D);
   AARBF_CHECK_LAUNCH(ctx, "diff_forward");
   return AARBF_OK;
 }
@@ -797,7 +811,7 @@ int aarbf_diff_backward(aarbf_context *ctx, const aarbf_raygen_data *rg, const a
   dim3 grid((W + 7) / 8, (H + 7) / 8);
   const uint64_t n5 = 5 * (uint64_t)D.A.numParticles;
   if (n5) diff_zero<<<(uint32_t)((n5 + 255) / 256), 256, 0, ctx->stream>>>(D.acc5, n5);
-  diff_adjoint<<<grid, kDBlock, 0, ctx->stream>>>(D);
+  diff_adjoint<<<grid, kDBlock, diff_stack_bytes(D), ctx->stream>>>(D);
   AARBF_CHECK_LAUNCH(ctx, "diff_adjoint");
   if (D.A.numParticles)
     diff_chain<<<(D.A.numParticles + kCBlock - 1) / kCBlock, kCBlock, 0, ctx->stream>>>(D);

@@ -443,7 +443,9 @@ __global__ __launch_bounds__(256) void wf_fold(RenderArgs A, WFState S) {
 // ---- gather --------------------------------------------------------------------
 template <bool STATS>
 __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderArgs A, WFState S, const uint32_t *__restrict__ listG,
-                                                     const uint32_t *__restrict__ countG) {
+                                                     const uint32_t *__restrict__ countG,
+                                                     uint32_t *__restrict__ listL, uint32_t *__restrict__ countL,
+                                                     uint32_t budget) {
   __shared__ uint32_t s_idx[kCand * kGBlock];
   __shared__ float s_tin[kCand * kGBlock];
   // the walk's per-level visiting words for levels 1 .. top-1, kGBlock
@@ -621,11 +623,15 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
         }
       }
     };
+    // A walk that opens more than `budget` nodes hands its ray to the
+    // long-walk list, which wf_gather_wave takes in the same round: the
+    // launch then lasts as long as its typical walks, not its longest.
+    bool handed = false;
     {
       const float4 *root = A.bvh.nodes + g_lvl_ofs[top];
       if (STATS) c.nodes++;
       if (seg_box(q, root[0], root[1])) {
-        uint32_t lvl = top - 1, base = 0;
+        uint32_t lvl = top - 1, base = 0, opened = 0;
         uint32_t ord = ordered(top, 0);
         for (;;) {
           if (lvl == 0) { // a whole leaf group, in entry order
@@ -640,6 +646,10 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
             base = ((base >> log2K) >> log2K) << log2K;
             continue;
           }
+          if (listL && ++opened > budget) {
+            handed = true;
+            break;
+          }
           ord >>= 4;
           uint32_t node = base + nib - 1u;
           stk[(lvl - 1u) * kGBlock] = ord; // lvl >= 1 here
@@ -649,6 +659,9 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
         }
       }
     }
+    if (handed) {
+      push(listL, countL, slot);
+    } else {
 
     if (STATS) {
       uint32_t cost = c.nodes - nodes0;
@@ -707,6 +720,7 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
     S.u[slot] = make_uint4(u.x, u.y, (u.z & 0xffffu) | (flags << 16), n);
     float4 g = S.seg[slot];
     S.seg[slot] = make_float4(t0, seg_end, g.z, g.w);
+    } // not handed
   }
   }
   flush_stats<STATS>(A, c);
@@ -745,7 +759,7 @@ __device__ inline float ord_val(int i) { return __int_as_float(i >= 0 ? i : i ^ 
 
 template <bool STATS>
 __global__ __launch_bounds__(64) void wf_gather_wave(RenderArgs A, WFState S, const uint32_t *__restrict__ listG,
-                                                     const uint32_t *__restrict__ countG) {
+                                                     const uint32_t *__restrict__ countG, uint32_t limit) {
   __shared__ uint32_t s_idx[kCand * 64];
   __shared__ float s_tin[kCand * 64];
   __shared__ uint32_t s_stack[kMaxLevels * 64];
@@ -754,7 +768,7 @@ __global__ __launch_bounds__(64) void wf_gather_wave(RenderArgs A, WFState S, co
   __shared__ float s_rtin[kCand];
   __shared__ int s_bound;
   const uint32_t n_g = *countG;
-  if (n_g > kWaveGather || blockIdx.x >= n_g) return; // the whole block
+  if (n_g > limit || blockIdx.x >= n_g) return; // the whole block
   load_levels(A);
   const uint32_t lane = threadIdx.x;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -1854,8 +1868,8 @@ __global__ void wf_lat_start(unsigned long long *lat, uint32_t nv) {
 // kernels read their counts on the device and loop over them).
 template <bool B>
 static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &S, uint32_t *const lists[4],
-                        uint32_t *listD, uint32_t *cnt, int cur, uint32_t nG, uint32_t gb, uint32_t sb,
-                        uint32_t db, uint32_t *nD_out) {
+                        uint32_t *listD, uint32_t *listL, uint32_t *cnt, int cur, uint32_t nG, uint32_t gb,
+                        uint32_t sb, uint32_t db, uint32_t *nD_out) {
   const bool st = ctx->stats;
   uint32_t *lA = lists[cur], *lG = lists[cur + 1], *nA2 = lists[2 - cur], *nG2 = lists[3 - cur];
   uint32_t *cA = cnt + (cur ? 2 : 0), *cN = cnt + (cur ? 0 : 2);
@@ -1867,20 +1881,31 @@ static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &
   // captured rounds (nG == ~0u) launch both with fixed grids.
   if (gb > 0) {
     const bool fixed = nG == ~0u;
+    const uint32_t wfix = (uint32_t)std::min<uint64_t>(kWaveGather, 16ull * (uint64_t)std::max(ctx->cus, 1));
     if (fixed || nG > kWaveGather) {
+      // long queue: one lane per ray, then one wave per ray for the walks
+      // that ran over the budget (a fixed grid looping over their count)
       const size_t stack_bytes = sizeof(uint32_t) * kGBlock * std::max<uint32_t>(A.bvh.top, 2u) - sizeof(uint32_t) * kGBlock;
+      const uint32_t budget = (uint32_t)ctx->tune_walk_budget;
+      uint32_t *cL = cnt + 6;
+      wf_zero<<<1, 64, 0, ctx->stream>>>(cL, 1);
       if (st)
-        wf_gather<true><<<gb, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1);
+        wf_gather<true><<<gb, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1, listL, cL, budget);
       else
-        wf_gather<false><<<gb, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1);
+        wf_gather<false><<<gb, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1, listL, cL, budget);
       AARBF_CHECK_LAUNCH(ctx, "wf_gather");
+      if (st)
+        wf_gather_wave<true><<<wfix, 64, 0, ctx->stream>>>(A, S, listL, cL, ~0u);
+      else
+        wf_gather_wave<false><<<wfix, 64, 0, ctx->stream>>>(A, S, listL, cL, ~0u);
+      AARBF_CHECK_LAUNCH(ctx, "wf_gather_wave");
     }
     if (fixed || nG <= kWaveGather) {
-      const uint32_t wb = fixed ? (uint32_t)std::min<uint64_t>(kWaveGather, 16ull * (uint64_t)std::max(ctx->cus, 1)) : nG;
+      const uint32_t wb = fixed ? wfix : nG;
       if (st)
-        wf_gather_wave<true><<<wb, 64, 0, ctx->stream>>>(A, S, lG, cA + 1);
+        wf_gather_wave<true><<<wb, 64, 0, ctx->stream>>>(A, S, lG, cA + 1, kWaveGather);
       else
-        wf_gather_wave<false><<<wb, 64, 0, ctx->stream>>>(A, S, lG, cA + 1);
+        wf_gather_wave<false><<<wb, 64, 0, ctx->stream>>>(A, S, lG, cA + 1, kWaveGather);
       AARBF_CHECK_LAUNCH(ctx, "wf_gather_wave");
     }
   }
@@ -1944,7 +1969,7 @@ static void graph_grids(const aarbf_context *ctx, uint64_t nslots, uint32_t &gb,
 // argument the kernels see) and replayed, with one read-back per replay.
 template <bool B>
 static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S, uint32_t *const lists[4],
-                      uint32_t *listD, uint32_t *cnt) {
+                      uint32_t *listD, uint32_t *listL, uint32_t *cnt) {
   uint32_t h[2];
   AARBF_HIP(ctx, hipMemcpyAsync(h, cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
   AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1956,9 +1981,9 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
     struct Key {
       RenderArgs A;
       WFState S;
-      uint32_t *lists[4], *listD, *cnt;
+      uint32_t *lists[4], *listD, *listL, *cnt;
       uint32_t gb, sb, db;
-      int R, batch, stats, mode, waves, cap;
+      int R, batch, stats, mode, waves, cap, budget;
     } key;
     memset(&key, 0, sizeof(key));
     key.A = A;
@@ -1975,6 +2000,8 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
     key.mode = ctx->render_mode;
     key.waves = ctx->tune_step_waves;
     key.cap = ctx->tune_step_cap;
+    key.budget = ctx->tune_walk_budget;
+    key.listL = listL;
     const std::string kb((const char *)&key, sizeof(key));
     hipGraphExec_t exec = nullptr;
     for (auto &g : ctx->graphs)
@@ -1983,7 +2010,7 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
       AARBF_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       int rc = AARBF_OK;
       for (int r = 0; r < R && rc == AARBF_OK; ++r)
-        rc = launch_round<B>(ctx, A, S, lists, listD, cnt, (r & 1) ? 2 : 0, ~0u, gb, sb, db, nullptr);
+        rc = launch_round<B>(ctx, A, S, lists, listD, listL, cnt, (r & 1) ? 2 : 0, ~0u, gb, sb, db, nullptr);
       hipGraph_t graph = nullptr;
       const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
       if (rc != AARBF_OK) {
@@ -2013,7 +2040,7 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
   while (nA > 0) {
     uint32_t nD = 0;
     const uint32_t gb = (nG + kGBlock - 1) / kGBlock, sb = (nA + kSBlock - 1) / kSBlock;
-    int rc = launch_round<B>(ctx, A, S, lists, listD, cnt, cur, nG, gb, sb, 0, &nD);
+    int rc = launch_round<B>(ctx, A, S, lists, listD, listL, cnt, cur, nG, gb, sb, 0, &nD);
     if (rc != AARBF_OK) return rc;
     uint32_t *cN = cnt + (cur ? 0 : 2);
     AARBF_HIP(ctx, hipMemcpyAsync(h, cN, 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -2073,7 +2100,7 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A0, const ReqArgs *vi
          o_ceo = take(kCand * nslots), o_hidx = take(4 * kCand * npix), o_htin = take(4 * kCand * npix),
          o_htout = take(4 * kCand * npix), o_heo = take(kCand * npix), o_head = take(16 * npix),
          o_col = take(16 * npix * chunk), o_next = take(4 * npix), o_l0 = take(4 * nslots), o_l1 = take(4 * nslots), o_l2 = take(4 * nslots),
-         o_l3 = take(4 * nslots), o_l4 = take(4 * nslots), o_cnt = take(64), o_view = take(sizeof(ReqArgs) * nv),
+         o_l3 = take(4 * nslots), o_l4 = take(4 * nslots), o_l5 = take(4 * nslots), o_cnt = take(64), o_view = take(sizeof(ReqArgs) * nv),
          o_tview = take(batch ? 4 * ntiles : 4);
   char *base = (char *)scratch(ctx, off);
   if (!base) return set_error(ctx, AARBF_ERR_OUT_OF_MEMORY, "aarbf_render_rbf: wavefront state");
@@ -2101,7 +2128,8 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A0, const ReqArgs *vi
   uint32_t *lists[4] = {(uint32_t *)(base + o_l0), (uint32_t *)(base + o_l1), (uint32_t *)(base + o_l2),
                         (uint32_t *)(base + o_l3)};
   uint32_t *listD = (uint32_t *)(base + o_l4);
-  // [0..1] and [2..3]: (A, G) counts of the two list pairs; [4]: dense; [8]: visiting pointer
+  uint32_t *listL = (uint32_t *)(base + o_l5); // walks over the budget, for wf_gather_wave
+  // [0..1] and [2..3]: (A, G) counts of the two list pairs; [4]: dense; [6]: long walks; [8]: visiting pointer
   uint32_t *cnt = (uint32_t *)(base + o_cnt);
   S.visit = cnt + 8;
   const bool st = ctx->stats;
@@ -2184,7 +2212,8 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A0, const ReqArgs *vi
       ctx->tms[3] += e;
       ctx->tlaunch[3] += 1;
     }
-    int rc = batch ? run_rounds<true>(ctx, A, S, lists, listD, cnt) : run_rounds<false>(ctx, A, S, lists, listD, cnt);
+    int rc = batch ? run_rounds<true>(ctx, A, S, lists, listD, listL, cnt)
+                   : run_rounds<false>(ctx, A, S, lists, listD, listL, cnt);
     if (rc != AARBF_OK) return rc;
     if (ctx->timing) AARBF_HIP(ctx, hipEventRecord(ctx->tev[0], ctx->stream));
     if (batch)

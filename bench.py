@@ -48,6 +48,8 @@ def main() -> int:
     ap.add_argument("--step-waves", type=int, default=0)
     ap.add_argument("--step-cap", type=int, default=4096)
     ap.add_argument("--graph-rounds", type=int, default=0, help="rounds per captured hipGraph (0 = host-driven)")
+    ap.add_argument("--walk-budget", type=int, default=256,
+                    help="nodes a one-lane gather walk opens before its ray goes to a one-wave gather")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--diff-views", type=int, default=2,
                     help="views of the differentiable-rendering leg timed (0 = skip the leg)")
@@ -97,6 +99,7 @@ def main() -> int:
     ctx.set_tuning(ctx.TUNE_STEP_WAVES, a.step_waves)
     ctx.set_tuning(ctx.TUNE_STEP_CAP, a.step_cap)
     ctx.set_tuning(ctx.TUNE_GRAPH_ROUNDS, a.graph_rounds)
+    ctx.set_tuning(ctx.TUNE_WALK_BUDGET, a.walk_budget)
     torch.cuda.synchronize()
     tp = time.time()
     sc = aarbf.Scene.prepare(ctx, wl.xyzw, wl.colormap, wl.radiusmap, wl.densitymap, wl.rbf_radius,

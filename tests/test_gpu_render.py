@@ -178,3 +178,34 @@ def test_taa_matches_oracle(gpu_ctx):
     f = d_frame.cpu().numpy().view(np.uint32)
     a8 = (np.clip(otaa[..., 3], 0, 1) * np.float32(255) + np.float32(0.5)).astype(np.uint32)
     assert np.array_equal(f >> 24, a8)  # no GUI layer: over() keeps the pixel
+
+
+@pytest.mark.parametrize("blue_noise", [False, True])
+def test_walk_budget_handoff_is_exact(gpu_ctx, blue_noise):
+    """Gather walks that open more nodes than the budget hand their ray to
+    the one-wave-per-ray gather in the same round.  An image whose first
+    rounds' gather queues exceed the wave gather's own limit (16,384 rays)
+    renders bit for bit the same with every walk handed over (budget 1) as
+    with the default budget, and equals the oracle on a pixel sample."""
+    W, H = 176, 128
+    xyzw, radii = random_scene(12000, seed=77, extent=1.0, r=(0.02, 0.07))
+    tfs = (scenes.ramp_table(256), scenes.constant_table(n=4), scenes.ramp_table(64))
+    sc, op, orad = _setup(gpu_ctx, xyzw, radii, tfs, 0.07)
+    cam = default_camera(W, H, look_from=(0.3, 0.25, 2.3))
+    stbn = gpu_ctx.stbn(scenes.synthetic_stbn(seed=5)) if blue_noise else 0
+    kw = dict(blue_noise=blue_noise, light=(0.3, 0.2, 0.4))
+    img0, acc0 = _gpu_render(gpu_ctx, sc, cam, W, H, 0.004, spp=2, frame=3, stbn=stbn, **kw)
+    try:
+        gpu_ctx.set_tuning(gpu_ctx.TUNE_WALK_BUDGET, 1)
+        img1, acc1 = _gpu_render(gpu_ctx, sc, cam, W, H, 0.004, spp=2, frame=3, stbn=stbn, **kw)
+    finally:
+        gpu_ctx.set_tuning(gpu_ctx.TUNE_WALK_BUDGET, 256)
+    assert (acc0[..., :3] > 0).mean() > 0.2
+    _compare(acc1, acc0, "accum, budget 1 vs default")
+    _compare(img1, img0, "image, budget 1 vs default")
+    px = np.array([(x, y) for y in range(3, H, 17) for x in range(5, W, 19)], np.int32)
+    r = oracle.Render(op, orad, sc.aabb, *tfs, cam, W, H, 0.004, 0.07, frame_id=3, accum_id=1,
+                      blue_noise=blue_noise, stbn=scenes.synthetic_stbn(seed=5) if blue_noise else None,
+                      light=(0.3, 0.2, 0.4))
+    oimg, oacc, _ = r.run(pixels=px, spp=2, threads=8)
+    _compare(acc0[px[:, 1], px[:, 0]], oacc[px[:, 1], px[:, 0]], "accum vs oracle")
