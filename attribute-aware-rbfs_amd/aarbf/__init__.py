@@ -340,7 +340,8 @@ class Context:
         self.check(self.lib.aarbf_autotune(self.h, self._requests(requests), len(requests), reps, C.byref(res)),
                    "aarbf_autotune")
         return {"step_cap": res.step_cap, "step_waves": res.step_waves, "graph_rounds": res.graph_rounds,
-                "candidates": res.candidates, "baseline_ms": res.baseline_ms, "best_ms": res.best_ms}
+                "walk_budget": res.walk_budget, "candidates": res.candidates, "baseline_ms": res.baseline_ms,
+                "best_ms": res.best_ms}
 
     def batch_latency(self, n: int) -> np.ndarray:
         """Per view of the last batch: ms from its start to the view's last sample."""

@@ -75,7 +75,8 @@ class Request(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("step_cap", C.c_int32), ("step_waves", C.c_int32), ("graph_rounds", C.c_int32),
-                ("candidates", u32), ("baseline_ms", C.c_double), ("best_ms", C.c_double)]
+                ("candidates", u32), ("baseline_ms", C.c_double), ("best_ms", C.c_double),
+                ("walk_budget", C.c_int32)]
 
 
 class RenderStats(C.Structure):
@@ -156,6 +157,7 @@ def _expected_layout():
         C.sizeof(RayGenData), RayGenData.fbSize.offset, RayGenData.globalAABBMax.offset,
         C.sizeof(RenderStats),
         C.sizeof(Request), Request.pc.offset, Request.spp.offset,
+        C.sizeof(TuneResult), TuneResult.walk_budget.offset,
     ]
 
 

@@ -364,6 +364,8 @@ uint32_t aarbf_abi_layout(uint32_t *out, uint32_t n) {
       (uint32_t)sizeof(aarbf_request),
       (uint32_t)offsetof(aarbf_request, pc),
       (uint32_t)offsetof(aarbf_request, spp),
+      (uint32_t)sizeof(aarbf_tune_result),
+      (uint32_t)offsetof(aarbf_tune_result, walk_budget),
   };
   uint32_t m = (uint32_t)(sizeof(v) / sizeof(v[0]));
   for (uint32_t i = 0; i < n && i < m; ++i) out[i] = v[i];

@@ -456,9 +456,9 @@ def batch_leg(ctx, stream, a):
     torch.cuda.synchronize()
     seq = np.array([ev[0].elapsed_time(e) for e in ev[1:]])
     ctx.set_payload(0)
-    for k in (ctx.TUNE_STEP_CAP, ctx.TUNE_STEP_WAVES, ctx.TUNE_GRAPH_ROUNDS):
+    for k in (ctx.TUNE_STEP_CAP, ctx.TUNE_STEP_WAVES, ctx.TUNE_GRAPH_ROUNDS, ctx.TUNE_WALK_BUDGET):
         ctx.set_tuning(k, {ctx.TUNE_STEP_CAP: a.step_cap, ctx.TUNE_STEP_WAVES: a.step_waves,
-                           ctx.TUNE_GRAPH_ROUNDS: a.graph_rounds}[k])
+                           ctx.TUNE_GRAPH_ROUNDS: a.graph_rounds, ctx.TUNE_WALK_BUDGET: a.walk_budget}[k])
     return {"workload": f"mixture-{a.batch_particles} (64 components), {len(reqs)} requests, side log-U[128,1024], "
                         "aspect U[0.5,2], spp U{1..8}, half marching / half delta tracking",
             "payload": ("fp32", "fp16 leaf-local quantised")[a.batch_payload],

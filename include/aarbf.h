@@ -333,7 +333,8 @@ int aarbf_batch_latency(aarbf_context *ctx, double *ms, uint32_t n);
 /* Schedule autotuner: renders the views (into its own buffers; the
  * callers' images are untouched) under candidate schedule parameters --
  * AARBF_TUNE_STEP_CAP over {1024, 2048, 4096, 8192}, AARBF_TUNE_STEP_WAVES
- * over {4, 5, 6}, AARBF_TUNE_GRAPH_ROUNDS over {0, 8}, by coordinate descent
+ * over {4, 5, 6}, AARBF_TUNE_GRAPH_ROUNDS over {0, 8},
+ * AARBF_TUNE_WALK_BUDGET over {256, 1024, 65536}, by coordinate descent
  * from the current setting, each timed as the best of `reps` runs after a
  * warm-up -- and keeps the fastest in the context.  The BVH leaf size is a
  * scene property, tuned by rebuilding the scene per leaf size around this
@@ -343,6 +344,7 @@ typedef struct aarbf_tune_result {
   uint32_t candidates;  /* settings timed */
   double baseline_ms;   /* the setting at entry */
   double best_ms;       /* the setting kept */
+  int32_t walk_budget;
 } aarbf_tune_result;
 int aarbf_autotune(aarbf_context *ctx, const aarbf_request *reqs, uint32_t n, uint32_t reps,
                    aarbf_tune_result *out);
