@@ -764,7 +764,7 @@ __global__ __launch_bounds__(64) void wf_gather_wave(RenderArgs A, WFState S, co
   __shared__ float s_tin[kCand * 64];
   __shared__ uint32_t s_stack[kMaxLevels * 64];
   __shared__ uint32_t s_front[2][kWaveFront];
-  __shared__ uint32_t s_ridx[kCand];
+  __shared__ uint32_t s_ridx[kCand], s_rk[kCand];
   __shared__ float s_rtin[kCand];
   __shared__ int s_bound;
   const uint32_t n_g = *countG;
@@ -958,22 +958,26 @@ __global__ __launch_bounds__(64) void wf_gather_wave(RenderArgs A, WFState S, co
         --l;
       }
     }
-    // merge: the kCand smallest entries over the lanes, then the next one
+    // merge: the kCand smallest entries over the lanes, then the next one.
+    // Each lane sorts its own list by entry first, so a round compares the
+    // lanes' heads.
+    for (uint32_t a = 1; a < n; ++a) {
+      const uint32_t pa = gi[a * 64];
+      const float ta = gtin[a * 64];
+      uint32_t b = a;
+      for (; b > 0 && gtin[(b - 1) * 64] > ta; --b) {
+        gi[b * 64] = gi[(b - 1) * 64];
+        gtin[b * 64] = gtin[(b - 1) * 64];
+      }
+      gi[b * 64] = pa;
+      gtin[b * 64] = ta;
+    }
     float cut_all = t_cut;
     for (int o2 = 32; o2 > 0; o2 >>= 1) cut_all = fminf(cut_all, __shfl_xor(cut_all, o2, 64));
-    uint32_t taken = 0, nres = 0;
+    uint32_t head_k = 0, nres = 0;
     float t_next = AA_INF_F;
     for (uint32_t r = 0; r <= kCand; ++r) {
-      float best = AA_INF_F;
-      uint32_t bk = 0;
-      for (uint32_t k = 0; k < n; ++k) {
-        const float v = gtin[k * 64];
-        if (!((taken >> k) & 1u) && v < best) {
-          best = v;
-          bk = k;
-        }
-      }
-      float v = best;
+      float v = head_k < n ? gtin[head_k * 64] : AA_INF_F;
       uint32_t who = lane;
       for (int o2 = 32; o2 > 0; o2 >>= 1) {
         const float v2 = __shfl_xor(v, o2, 64);
@@ -989,66 +993,62 @@ __global__ __launch_bounds__(64) void wf_gather_wave(RenderArgs A, WFState S, co
         break;
       }
       if (lane == who) {
-        taken |= 1u << bk;
-        s_ridx[r] = gi[bk * 64];
-        s_rtin[r] = gtin[bk * 64];
+        s_ridx[r] = gi[head_k * 64];
+        s_rtin[r] = gtin[head_k * 64];
+        ++head_k;
       }
       ++nres;
     }
     t_cut = fminf(t_next, cut_all);
     __syncthreads();
+    // the list out, one lane per entry: entry r (the r-th smallest entry)
+    // goes to its position k in ascending particle order, and entry order
+    // rank r' (ties by that position) maps to k
+    const bool crowded = t_cut <= t0 + 2.0f * A.unit; // point queries for a short stretch
+    const uint32_t nout = crowded ? 0u : nres;
+    uint32_t *oidx = head ? S.hidx : S.cidx;
+    float *otin = head ? S.htin : S.ctin, *otout = head ? S.htout : S.ctout;
+    uint8_t *oeo = head ? S.heo : S.ceo;
+    const uint64_t row = head ? pix : slot, stride = head ? S.npix : S.nslots;
+    if (lane < nout) {
+      const uint32_t pi = s_ridx[lane];
+      uint32_t k = 0;
+      for (uint32_t j = 0; j < nout; ++j) k += s_ridx[j] < pi;
+      s_rk[lane] = k;
+    }
+    __syncthreads();
+    if (lane < nout) {
+      const uint32_t pi = s_ridx[lane], k = s_rk[lane];
+      const float ti = s_rtin[lane];
+      uint32_t r = 0;
+      for (uint32_t j = 0; j < nout; ++j) {
+        const float tj = s_rtin[j];
+        r += (tj < ti) || (tj == ti && s_rk[j] < k);
+      }
+      float tin, tout;
+      chord(ld_pr(A, pi), tin, tout); // the exit parameter take() computed
+      oeo[r * stride + row] = (uint8_t)k;
+      oidx[k * stride + row] = pi;
+      otin[k * stride + row] = ti;
+      otout[k * stride + row] = tout;
+    }
     if (lane == 0) {
       float seg_end;
-      n = nres;
-      if (t_cut <= t0 + 2.0f * A.unit) { // crowded: point queries for a short stretch
+      if (crowded) {
         flags |= F_DENSE;
         seg_end = t0 + 4.0f * A.unit;
-        n = 0;
       } else {
         flags &= ~F_DENSE;
-        uint32_t *oidx = head ? S.hidx : S.cidx;
-        float *otin = head ? S.htin : S.ctin, *otout = head ? S.htout : S.ctout;
-        uint8_t *oeo = head ? S.heo : S.ceo;
-        const uint64_t row = head ? pix : slot, stride = head ? S.npix : S.nslots;
         seg_end = t_cut;
-        // ascending particle order (insertion sort, n <= kCand)
-        for (uint32_t a = 1; a < n; ++a) {
-          const uint32_t pi = s_ridx[a];
-          const float ti = s_rtin[a];
-          uint32_t b = a;
-          while (b > 0 && s_ridx[b - 1] > pi) {
-            s_ridx[b] = s_ridx[b - 1];
-            s_rtin[b] = s_rtin[b - 1];
-            --b;
-          }
-          s_ridx[b] = pi;
-          s_rtin[b] = ti;
-        }
-        // entry order: position of rank r in tin order (ties by position)
-        for (uint32_t k = 0; k < n; ++k) {
-          const float tk = s_rtin[k];
-          uint32_t r = 0;
-          for (uint32_t j = 0; j < n; ++j) {
-            const float tj = s_rtin[j];
-            r += (tj < tk) || (tj == tk && j < k);
-          }
-          const uint32_t pk = s_ridx[k];
-          float tin, tout;
-          chord(ld_pr(A, pk), tin, tout);
-          oeo[r * stride + row] = (uint8_t)k;
-          oidx[k * stride + row] = pk;
-          otin[k * stride + row] = tk;
-          otout[k * stride + row] = tout;
-        }
       }
       if (head) {
         flags = (flags & ~F_HEADGATHER) | F_HEAD;
-        S.head[pix] = make_float4(t0, seg_end, __uint_as_float(n),
+        S.head[pix] = make_float4(t0, seg_end, __uint_as_float(nout),
                                   __uint_as_float(1u | ((flags & F_DENSE) ? 2u : 0u)));
       } else {
         flags &= ~F_HEAD;
       }
-      S.u[slot] = make_uint4(u.x, u.y, (u.z & 0xffffu) | (flags << 16), n);
+      S.u[slot] = make_uint4(u.x, u.y, (u.z & 0xffffu) | (flags << 16), nout);
       const float4 g = S.seg[slot];
       S.seg[slot] = make_float4(t0, seg_end, g.z, g.w);
     }
