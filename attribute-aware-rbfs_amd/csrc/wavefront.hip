@@ -69,6 +69,10 @@ constexpr uint32_t kWaveGather = AARBF_WAVE_GATHER;
 #endif
 constexpr uint32_t kWaveSplit = AARBF_WAVE_SPLIT;   // subtrees the wave splits a walk into (at least)
 constexpr uint32_t kWaveFront = 8u * kWaveSplit; // breadth-first frontier entries (K <= 8)
+#ifndef AARBF_WAVE_SLICE
+#define AARBF_WAVE_SLICE 32
+#endif
+constexpr uint32_t kWaveSlice = AARBF_WAVE_SLICE; // nodes a lane opens between shared-bound updates
 // Empty-space steps a lane may take per iteration of the step loop: a wave
 // iterates as long as its slowest lane, and most steps are empty-space
 // steps, so taking several per iteration cuts the iterations (and the
@@ -799,8 +803,8 @@ __global__ __launch_bounds__(64) void wf_gather_wave(RenderArgs A, WFState S, co
     const float m = 2e-5f * (1.0f + fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) + fmaxf(fabsf(t0), fabsf(tmax)));
     q.m = m;
     q.len = tmax - t0 + 4.0f * m;
-    uint32_t n = 0, maxpos = 0;
-    float maxtin = -AA_INF_F, t_cut = AA_INF_F;
+    uint32_t n = 0;
+    float t_cut = AA_INF_F;
     if (STATS && lane == 0) c.queries++;
     if (lane == 0) s_bound = ord_key(AA_INF_F);
     // breadth first from the root
@@ -875,31 +879,25 @@ __global__ __launch_bounds__(64) void wf_gather_wave(RenderArgs A, WFState S, co
         if (tout >= t0 && tin <= tmax) lower_cut(tin);
         return;
       }
-      if (n < kCand) {
-        gi[n * 64] = p;
-        gtin[n * 64] = tin;
-        if (tin > maxtin) {
-          maxtin = tin;
-          maxpos = n;
+      // the lane's list stays sorted by entry; a full list drops its last
+      uint32_t b = n;
+      if (n == kCand) {
+        const float last = gtin[(kCand - 1u) * 64];
+        if (!(tin < last)) {
+          lower_cut(tin);
+          return;
         }
-        ++n;
-        return;
-      }
-      if (tin < maxtin) { // evict the latest-entering candidate
-        lower_cut(maxtin);
-        gi[maxpos * 64] = p;
-        gtin[maxpos * 64] = tin;
-        maxtin = -AA_INF_F;
-        for (uint32_t k = 0; k < kCand; ++k) {
-          float v = gtin[k * 64];
-          if (v > maxtin) {
-            maxtin = v;
-            maxpos = k;
-          }
-        }
+        lower_cut(last);
+        b = kCand - 1u;
       } else {
-        lower_cut(tin);
+        ++n;
       }
+      for (; b > 0 && gtin[(b - 1u) * 64] > tin; --b) {
+        gi[b * 64] = gi[(b - 1u) * 64];
+        gtin[b * 64] = gtin[(b - 1u) * 64];
+      }
+      gi[b * 64] = p;
+      gtin[b * 64] = tin;
     };
     auto leaf_batch = [&](uint32_t lb, uint32_t ord) {
       if (A.ppl == 1) {
@@ -933,72 +931,91 @@ __global__ __launch_bounds__(64) void wf_gather_wave(RenderArgs A, WFState S, co
       if (STATS) c.nodes += K;
       return ftb_children(A, q, l, node);
     };
-    for (uint32_t fi = lane; fi < nf; fi += 64u) {
-      const uint32_t R = s_front[cur][fi], L = lvl - 1u; // R at level lvl >= 1
-      uint32_t l = L, base = R << log2K;
-      uint32_t ord = children(lvl, R);
+    // The (kCand + 1)-th smallest entry over the lanes' sorted lists (+inf
+    // while they hold fewer), by rounds that compare the lanes' heads; with
+    // `out`, the kCand smallest are written to s_ridx / s_rtin as well.
+    auto union_cut = [&](bool out, uint32_t &taken) -> float {
+      uint32_t hk = 0;
+      taken = 0;
+      for (uint32_t r = 0; r <= kCand; ++r) {
+        float v = hk < n ? gtin[hk * 64] : AA_INF_F;
+        uint32_t who = lane;
+        for (int o2 = 32; o2 > 0; o2 >>= 1) {
+          const float v2 = __shfl_xor(v, o2, 64);
+          const uint32_t w2 = __shfl_xor(who, o2, 64);
+          if (v2 < v || (v2 == v && w2 < who)) {
+            v = v2;
+            who = w2;
+          }
+        }
+        if (!(v < AA_INF_F)) return AA_INF_F; // fewer than kCand + 1 entries in all
+        if (r == kCand) return v;
+        if (lane == who) {
+          if (out) {
+            s_ridx[r] = gi[hk * 64];
+            s_rtin[r] = gtin[hk * 64];
+          }
+          ++hk;
+        }
+        ++taken;
+      }
+      return AA_INF_F;
+    };
+    // Every lane walks its subtrees (frontier entries lane, lane + 64, ...)
+    // front to back in slices of kWaveSlice opened nodes; after each slice
+    // the (kCand + 1)-th smallest entry over all lanes becomes the shared
+    // bound, so a lane whose own subtrees hold few particles still prunes.
+    {
+      uint32_t fi = lane, L = lvl - 1u, l = 0, base = 0, ord = 0;
+      bool have = false, done = fi >= nf;
       for (;;) {
-        if (l == 0) { // a whole leaf group, in entry order
-          leaf_batch(base, ord);
-          ord = 0;
+        uint32_t left = kWaveSlice;
+        while (!done && left) {
+          if (!have) { // the next subtree, its root at level lvl >= 1
+            const uint32_t R = s_front[cur][fi];
+            l = L;
+            base = R << log2K;
+            ord = children(lvl, R);
+            have = true;
+            --left;
+            continue;
+          }
+          if (l == 0) { // a whole leaf group, in entry order
+            leaf_batch(base, ord);
+            ord = 0;
+          }
+          const uint32_t nib = ord & 15u;
+          if (nib == 0) {
+            if (l == L) {
+              have = false;
+              fi += 64u;
+              done = fi >= nf;
+              continue;
+            }
+            ++l;
+            ord = stk[(l - 1u) * 64u];
+            base = ((base >> log2K) >> log2K) << log2K;
+            continue;
+          }
+          ord >>= 4;
+          const uint32_t node = base + nib - 1u;
+          stk[(l - 1u) * 64u] = ord; // l >= 1 here
+          base = node << log2K;
+          ord = children(l, node);
+          --l;
+          --left;
         }
-        const uint32_t nib = ord & 15u;
-        if (nib == 0) {
-          if (l == L) break;
-          ++l;
-          ord = stk[(l - 1u) * 64u];
-          base = ((base >> log2K) >> log2K) << log2K;
-          continue;
-        }
-        ord >>= 4;
-        const uint32_t node = base + nib - 1u;
-        stk[(l - 1u) * 64u] = ord; // l >= 1 here
-        base = node << log2K;
-        ord = children(l, node);
-        --l;
+        if (__all(done)) break;
+        uint32_t tk;
+        const float ub = union_cut(false, tk);
+        if (lane == 0 && ub < ord_val(s_bound)) s_bound = ord_key(ub);
+        __syncthreads();
       }
-    }
-    // merge: the kCand smallest entries over the lanes, then the next one.
-    // Each lane sorts its own list by entry first, so a round compares the
-    // lanes' heads.
-    for (uint32_t a = 1; a < n; ++a) {
-      const uint32_t pa = gi[a * 64];
-      const float ta = gtin[a * 64];
-      uint32_t b = a;
-      for (; b > 0 && gtin[(b - 1) * 64] > ta; --b) {
-        gi[b * 64] = gi[(b - 1) * 64];
-        gtin[b * 64] = gtin[(b - 1) * 64];
-      }
-      gi[b * 64] = pa;
-      gtin[b * 64] = ta;
     }
     float cut_all = t_cut;
     for (int o2 = 32; o2 > 0; o2 >>= 1) cut_all = fminf(cut_all, __shfl_xor(cut_all, o2, 64));
-    uint32_t head_k = 0, nres = 0;
-    float t_next = AA_INF_F;
-    for (uint32_t r = 0; r <= kCand; ++r) {
-      float v = head_k < n ? gtin[head_k * 64] : AA_INF_F;
-      uint32_t who = lane;
-      for (int o2 = 32; o2 > 0; o2 >>= 1) {
-        const float v2 = __shfl_xor(v, o2, 64);
-        const uint32_t w2 = __shfl_xor(who, o2, 64);
-        if (v2 < v || (v2 == v && w2 < who)) {
-          v = v2;
-          who = w2;
-        }
-      }
-      if (!(v < AA_INF_F)) break; // every lane's list is used up
-      if (r == kCand) {
-        t_next = v;
-        break;
-      }
-      if (lane == who) {
-        s_ridx[r] = gi[head_k * 64];
-        s_rtin[r] = gtin[head_k * 64];
-        ++head_k;
-      }
-      ++nres;
-    }
+    uint32_t nres = 0;
+    const float t_next = union_cut(true, nres);
     t_cut = fminf(t_next, cut_all);
     __syncthreads();
     // the list out, one lane per entry: entry r (the r-th smallest entry)
