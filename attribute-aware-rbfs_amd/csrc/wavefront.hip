@@ -766,7 +766,7 @@ __global__ __launch_bounds__(64) void wf_gather_wave(RenderArgs A, WFState S, co
                                                      const uint32_t *__restrict__ countG, uint32_t limit) {
   __shared__ uint32_t s_idx[kCand * 64];
   __shared__ float s_tin[kCand * 64];
-  __shared__ uint32_t s_stack[kMaxLevels * 64];
+  extern __shared__ uint32_t s_stack[]; // levels 0 .. top-1, 64 words each (sized at launch)
   __shared__ uint32_t s_front[2][kWaveFront];
   __shared__ uint32_t s_ridx[kCand], s_rk[kCand];
   __shared__ float s_rtin[kCand];
@@ -825,18 +825,23 @@ __global__ __launch_bounds__(64) void wf_gather_wave(RenderArgs A, WFState S, co
         uint32_t child = 0;
         if (j < total) {
           const uint32_t node = s_front[cur][j >> log2K], k = j & (K - 1u);
-          const QGroup g = Qg[g_lvl_qofs[lvl] + node];
+          // this lane's child only: its bytes read from memory (k varies
+          // across the lanes, so a copy of the group would be indexed
+          // dynamically and land in scratch)
+          const QGroup *gp = Qg + g_lvl_qofs[lvl] + node;
+          const float4 ga = gp->a, gb = gp->b;
+          auto qb = [&](uint32_t field) { return (gp->q[2u * field + (k >> 2)] >> ((k & 3u) * 8u)) & 255u; };
           const uint32_t first = node << log2K, mcnt = min(K, g_lvl_cnt[lvl - 1] - first);
-          const uint32_t lx = qbyte(g, 0, k), hx = qbyte(g, 3, k);
+          const uint32_t lx = qb(0), hx = qb(3);
           child = first + k;
           if (k < mcnt && lx <= hx) {
             float4 lo, hi;
-            lo.x = g.a.x + (float)lx * g.a.w;
-            lo.y = g.a.y + (float)qbyte(g, 1, k) * g.b.x;
-            lo.z = g.a.z + (float)qbyte(g, 2, k) * g.b.y;
-            hi.x = g.a.x + (float)hx * g.a.w;
-            hi.y = g.a.y + (float)qbyte(g, 4, k) * g.b.x;
-            hi.z = g.a.z + (float)qbyte(g, 5, k) * g.b.y;
+            lo.x = ga.x + (float)lx * ga.w;
+            lo.y = ga.y + (float)qb(1) * gb.x;
+            lo.z = ga.z + (float)qb(2) * gb.y;
+            hi.x = ga.x + (float)hx * ga.w;
+            hi.y = ga.y + (float)qb(4) * gb.x;
+            hi.z = ga.z + (float)qb(5) * gb.y;
             float tn;
             hit = seg_box_t(q, lo, hi, tn);
           }
@@ -1899,6 +1904,7 @@ static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &
   if (gb > 0) {
     const bool fixed = nG == ~0u;
     const uint32_t wfix = (uint32_t)std::min<uint64_t>(kWaveGather, 16ull * (uint64_t)std::max(ctx->cus, 1));
+    const size_t wstack = sizeof(uint32_t) * 64 * std::max<uint32_t>(A.bvh.top, 2u);
     if (fixed || nG > kWaveGather) {
       // long queue: one lane per ray, then one wave per ray for the walks
       // that ran over the budget (a fixed grid looping over their count)
@@ -1912,17 +1918,17 @@ static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &
         wf_gather<false><<<gb, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1, listL, cL, budget);
       AARBF_CHECK_LAUNCH(ctx, "wf_gather");
       if (st)
-        wf_gather_wave<true><<<wfix, 64, 0, ctx->stream>>>(A, S, listL, cL, ~0u);
+        wf_gather_wave<true><<<wfix, 64, wstack, ctx->stream>>>(A, S, listL, cL, ~0u);
       else
-        wf_gather_wave<false><<<wfix, 64, 0, ctx->stream>>>(A, S, listL, cL, ~0u);
+        wf_gather_wave<false><<<wfix, 64, wstack, ctx->stream>>>(A, S, listL, cL, ~0u);
       AARBF_CHECK_LAUNCH(ctx, "wf_gather_wave");
     }
     if (fixed || nG <= kWaveGather) {
       const uint32_t wb = fixed ? wfix : nG;
       if (st)
-        wf_gather_wave<true><<<wb, 64, 0, ctx->stream>>>(A, S, lG, cA + 1, kWaveGather);
+        wf_gather_wave<true><<<wb, 64, wstack, ctx->stream>>>(A, S, lG, cA + 1, kWaveGather);
       else
-        wf_gather_wave<false><<<wb, 64, 0, ctx->stream>>>(A, S, lG, cA + 1, kWaveGather);
+        wf_gather_wave<false><<<wb, 64, wstack, ctx->stream>>>(A, S, lG, cA + 1, kWaveGather);
       AARBF_CHECK_LAUNCH(ctx, "wf_gather_wave");
     }
   }
