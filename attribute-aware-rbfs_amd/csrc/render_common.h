@@ -407,6 +407,31 @@ __device__ inline void walk_q(const RenderArgs &A, StkT *stk, BoxTest test, Leav
   }
 }
 
+// Child ordering keys: the entry distance (>= 0, so its bits order as
+// unsigned integers; the sign bit is cleared for -0) with its low three bits
+// replaced by the child number, so one unsigned min / max orders
+// (distance, child) pairs; distances within 8 ulps order by child.  A miss
+// sorts after every hit.
+constexpr uint32_t kMissKey = 0xfffffff8u;
+__device__ inline uint32_t entry_key(float t, uint32_t k) { return ((__float_as_uint(t) & 0x7ffffff8u) | k); }
+
+// Batcher's odd-even merge sort network for 8 keys (fixed indices: the keys
+// stay in registers)
+__device__ inline void sort8(uint32_t key[8]) {
+#define AARBF_KSWAP(i, j)                                                                                   \
+  {                                                                                                          \
+    const uint32_t a_ = key[i], b_ = key[j];                                                                 \
+    key[i] = min(a_, b_);                                                                                    \
+    key[j] = max(a_, b_);                                                                                    \
+  }
+  AARBF_KSWAP(0, 1) AARBF_KSWAP(2, 3) AARBF_KSWAP(4, 5) AARBF_KSWAP(6, 7)
+  AARBF_KSWAP(0, 2) AARBF_KSWAP(1, 3) AARBF_KSWAP(4, 6) AARBF_KSWAP(5, 7)
+  AARBF_KSWAP(1, 2) AARBF_KSWAP(5, 6) AARBF_KSWAP(0, 4) AARBF_KSWAP(3, 7)
+  AARBF_KSWAP(1, 5) AARBF_KSWAP(2, 6) AARBF_KSWAP(1, 4) AARBF_KSWAP(3, 6)
+  AARBF_KSWAP(2, 4) AARBF_KSWAP(3, 5) AARBF_KSWAP(3, 4)
+#undef AARBF_KSWAP
+}
+
 // Children of quantised node `node` at level `lvl` that the segment q
 // enters, in order of where it enters their boxes: nibbles (child + 1),
 // first child in the lowest nibble (K <= 8).  The front-to-back walks use
@@ -416,12 +441,10 @@ __device__ inline uint32_t ftb_children(const RenderArgs &A, const SegQuery &q, 
   const uint32_t log2K = A.bvh.log2K, K = 1u << log2K;
   const uint32_t first = node << log2K;
   const uint32_t mcnt = min(K, g_lvl_cnt[lvl - 1] - first);
-  float key[8];
-  uint32_t id[8];
+  uint32_t key[8];
 #pragma unroll
   for (uint32_t k = 0; k < 8; ++k) {
-    key[k] = AA_INF_F;
-    id[k] = k;
+    key[k] = kMissKey | k;
     const uint32_t lx = qbyte(g, 0, k), hx = qbyte(g, 3, k);
     if (k < mcnt && lx <= hx) {
       float4 lo, hi;
@@ -432,33 +455,14 @@ __device__ inline uint32_t ftb_children(const RenderArgs &A, const SegQuery &q, 
       hi.y = g.a.y + (float)qbyte(g, 4, k) * g.b.x;
       hi.z = g.a.z + (float)qbyte(g, 5, k) * g.b.y;
       float tn;
-      if (seg_box_t(q, lo, hi, tn)) key[k] = tn;
+      if (seg_box_t(q, lo, hi, tn)) key[k] = entry_key(tn, k);
     }
   }
-  // Batcher's odd-even merge sort network for 8 keys (fixed indices: the
-  // keys stay in registers)
-#define AARBF_FTB_CSWAP(i, j)                                                                                \
-  if (key[j] < key[i]) {                                                                                     \
-    const float tk = key[i];                                                                                 \
-    key[i] = key[j];                                                                                         \
-    key[j] = tk;                                                                                             \
-    const uint32_t ti = id[i];                                                                               \
-    id[i] = id[j];                                                                                           \
-    id[j] = ti;                                                                                              \
-  }
-  AARBF_FTB_CSWAP(0, 1) AARBF_FTB_CSWAP(2, 3) AARBF_FTB_CSWAP(4, 5) AARBF_FTB_CSWAP(6, 7)
-  AARBF_FTB_CSWAP(0, 2) AARBF_FTB_CSWAP(1, 3) AARBF_FTB_CSWAP(4, 6) AARBF_FTB_CSWAP(5, 7)
-  AARBF_FTB_CSWAP(1, 2) AARBF_FTB_CSWAP(5, 6) AARBF_FTB_CSWAP(0, 4) AARBF_FTB_CSWAP(3, 7)
-  AARBF_FTB_CSWAP(1, 5) AARBF_FTB_CSWAP(2, 6) AARBF_FTB_CSWAP(1, 4) AARBF_FTB_CSWAP(3, 6)
-  AARBF_FTB_CSWAP(2, 4) AARBF_FTB_CSWAP(3, 5) AARBF_FTB_CSWAP(3, 4)
-#undef AARBF_FTB_CSWAP
-  uint32_t ord = 0, cnt = 0;
+  sort8(key);
+  uint32_t ord = 0;
 #pragma unroll
   for (uint32_t r = 0; r < 8; ++r)
-    if (key[r] < AA_INF_F) {
-      ord |= (id[r] + 1u) << (4u * cnt);
-      ++cnt;
-    }
+    if (key[r] < kMissKey) ord |= ((key[r] & 7u) + 1u) << (4u * r);
   return ord;
 }
 

@@ -556,12 +556,13 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
       uint32_t first = node << log2K;
       uint32_t mcnt = min(K, g_lvl_cnt[lvl - 1] - first);
       if (STATS) c.nodes += mcnt;
-      float key[8];
-      uint32_t id[8];
+      // entry distance (>= 0, so its bits order as integers; the low three
+      // bits make room for the child number) and child in one key; a miss
+      // sorts last
+      uint32_t key[8];
 #pragma unroll
       for (uint32_t k = 0; k < 8; ++k) {
-        key[k] = AA_INF_F;
-        id[k] = k;
+        key[k] = kMissKey | k;
         uint32_t lx = qbyte(g, 0, k), hx = qbyte(g, 3, k);
         if (k < mcnt && lx <= hx) {
           float4 lo, hi;
@@ -572,32 +573,14 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
           hi.y = g.a.y + (float)qbyte(g, 4, k) * g.b.x;
           hi.z = g.a.z + (float)qbyte(g, 5, k) * g.b.y;
           float tn;
-          if (seg_box_t(q, lo, hi, tn)) key[k] = tn;
+          if (seg_box_t(q, lo, hi, tn)) key[k] = entry_key(tn, k);
         }
       }
-      // Batcher's odd-even merge sort network for 8 keys
-#define AARBF_CSWAP(i, j)                                                                                    \
-  if (key[j] < key[i]) {                                                                                     \
-    float tk = key[i];                                                                                       \
-    key[i] = key[j];                                                                                         \
-    key[j] = tk;                                                                                             \
-    uint32_t ti = id[i];                                                                                     \
-    id[i] = id[j];                                                                                           \
-    id[j] = ti;                                                                                              \
-  }
-      AARBF_CSWAP(0, 1) AARBF_CSWAP(2, 3) AARBF_CSWAP(4, 5) AARBF_CSWAP(6, 7)
-      AARBF_CSWAP(0, 2) AARBF_CSWAP(1, 3) AARBF_CSWAP(4, 6) AARBF_CSWAP(5, 7)
-      AARBF_CSWAP(1, 2) AARBF_CSWAP(5, 6) AARBF_CSWAP(0, 4) AARBF_CSWAP(3, 7)
-      AARBF_CSWAP(1, 5) AARBF_CSWAP(2, 6) AARBF_CSWAP(1, 4) AARBF_CSWAP(3, 6)
-      AARBF_CSWAP(2, 4) AARBF_CSWAP(3, 5) AARBF_CSWAP(3, 4)
-#undef AARBF_CSWAP
-      uint32_t ord = 0, cnt = 0;
+      sort8(key);
+      uint32_t ord = 0;
 #pragma unroll
       for (uint32_t r = 0; r < 8; ++r)
-        if (key[r] < AA_INF_F) {
-          ord |= (id[r] + 1u) << (4u * cnt);
-          ++cnt;
-        }
+        if (key[r] < kMissKey) ord |= ((key[r] & 7u) + 1u) << (4u * r);
       return ord;
     };
     auto leaf_batch = [&](uint32_t lb, uint32_t ord) {
