@@ -441,21 +441,33 @@ __device__ inline uint32_t ftb_children(const RenderArgs &A, const SegQuery &q, 
   const uint32_t log2K = A.bvh.log2K, K = 1u << log2K;
   const uint32_t first = node << log2K;
   const uint32_t mcnt = min(K, g_lvl_cnt[lvl - 1] - first);
+  // The slab parameters of a child bound b = origin + code * step are
+  // (b -+ m - start) / d = code * (step / d) + (origin -+ m - start) / d:
+  // per group a scale and two offsets per axis, then one fused
+  // multiply-add per bound.  Rounding moves a parameter by ulps, far
+  // inside the margin m, so the test stays conservative.
+  const float sx = g.a.w * q.ix, sy = g.b.x * q.iy, sz = g.b.y * q.iz;
+  const float lx0 = (g.a.x - q.m - q.ax) * q.ix, hx0 = (g.a.x + q.m - q.ax) * q.ix;
+  const float ly0 = (g.a.y - q.m - q.ay) * q.iy, hy0 = (g.a.y + q.m - q.ay) * q.iy;
+  const float lz0 = (g.a.z - q.m - q.az) * q.iz, hz0 = (g.a.z + q.m - q.az) * q.iz;
   uint32_t key[8];
 #pragma unroll
   for (uint32_t k = 0; k < 8; ++k) {
     key[k] = kMissKey | k;
     const uint32_t lx = qbyte(g, 0, k), hx = qbyte(g, 3, k);
     if (k < mcnt && lx <= hx) {
-      float4 lo, hi;
-      lo.x = g.a.x + (float)lx * g.a.w;
-      lo.y = g.a.y + (float)qbyte(g, 1, k) * g.b.x;
-      lo.z = g.a.z + (float)qbyte(g, 2, k) * g.b.y;
-      hi.x = g.a.x + (float)hx * g.a.w;
-      hi.y = g.a.y + (float)qbyte(g, 4, k) * g.b.x;
-      hi.z = g.a.z + (float)qbyte(g, 5, k) * g.b.y;
-      float tn;
-      if (seg_box_t(q, lo, hi, tn)) key[k] = entry_key(tn, k);
+      float t1 = __builtin_fmaf((float)lx, sx, lx0), t2 = __builtin_fmaf((float)hx, sx, hx0);
+      float nr = fminf(t1, t2), fr = fmaxf(t1, t2);
+      t1 = __builtin_fmaf((float)qbyte(g, 1, k), sy, ly0);
+      t2 = __builtin_fmaf((float)qbyte(g, 4, k), sy, hy0);
+      nr = fmaxf(nr, fminf(t1, t2));
+      fr = fminf(fr, fmaxf(t1, t2));
+      t1 = __builtin_fmaf((float)qbyte(g, 2, k), sz, lz0);
+      t2 = __builtin_fmaf((float)qbyte(g, 5, k), sz, hz0);
+      nr = fmaxf(nr, fminf(t1, t2));
+      fr = fminf(fr, fmaxf(t1, t2));
+      const float tn = fmaxf(nr, 0.0f);
+      if (tn <= fminf(fr, q.len)) key[k] = entry_key(tn, k);
     }
   }
   sort8(key);

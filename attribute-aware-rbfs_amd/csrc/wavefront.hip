@@ -552,36 +552,8 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
     const QGroup *__restrict__ Qg = A.bvh.groups;
     const uint32_t log2K = A.bvh.log2K, K = 1u << log2K, top = A.bvh.top;
     auto ordered = [&](uint32_t lvl, uint32_t node) -> uint32_t {
-      const QGroup g = Qg[g_lvl_qofs[lvl] + node];
-      uint32_t first = node << log2K;
-      uint32_t mcnt = min(K, g_lvl_cnt[lvl - 1] - first);
-      if (STATS) c.nodes += mcnt;
-      // entry distance (>= 0, so its bits order as integers; the low three
-      // bits make room for the child number) and child in one key; a miss
-      // sorts last
-      uint32_t key[8];
-#pragma unroll
-      for (uint32_t k = 0; k < 8; ++k) {
-        key[k] = kMissKey | k;
-        uint32_t lx = qbyte(g, 0, k), hx = qbyte(g, 3, k);
-        if (k < mcnt && lx <= hx) {
-          float4 lo, hi;
-          lo.x = g.a.x + (float)lx * g.a.w;
-          lo.y = g.a.y + (float)qbyte(g, 1, k) * g.b.x;
-          lo.z = g.a.z + (float)qbyte(g, 2, k) * g.b.y;
-          hi.x = g.a.x + (float)hx * g.a.w;
-          hi.y = g.a.y + (float)qbyte(g, 4, k) * g.b.x;
-          hi.z = g.a.z + (float)qbyte(g, 5, k) * g.b.y;
-          float tn;
-          if (seg_box_t(q, lo, hi, tn)) key[k] = entry_key(tn, k);
-        }
-      }
-      sort8(key);
-      uint32_t ord = 0;
-#pragma unroll
-      for (uint32_t r = 0; r < 8; ++r)
-        if (key[r] < kMissKey) ord |= ((key[r] & 7u) + 1u) << (4u * r);
-      return ord;
+      if (STATS) c.nodes += min(K, g_lvl_cnt[lvl - 1] - (node << log2K));
+      return ftb_children(A, q, lvl, node);
     };
     auto leaf_batch = [&](uint32_t lb, uint32_t ord) {
       if (A.ppl == 1) {
