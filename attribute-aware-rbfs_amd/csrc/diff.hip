@@ -591,8 +591,9 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
         add_dm(j1, aD * fD);
         float a_rho = aD * sD, a_S[3];
         for (int k = 0; k < 3; ++k) {
+          // d(m^(1/2.2))/dm = (1/2.2) m^(1/2.2) / m, with m^(1/2.2) = c[k]
           const float gc = g[k] * pj, m = pl.S[k] / pl.rho;
-          const float dcdm = m > 0.0f ? (1.0f / 2.2f) * aa_powf(m, 1.0f / 2.2f - 1.0f) : 0.0f;
+          const float dcdm = m > 0.0f ? (1.0f / 2.2f) * (c[k] / m) : 0.0f;
           a_S[k] = gc * dcdm / pl.rho;
           a_rho -= gc * dcdm * m / pl.rho;
         }
