@@ -58,7 +58,7 @@ def main() -> int:
                     help="camera requests of the batching leg (0 = skip the leg; the batching workload has 256)")
     ap.add_argument("--batch-particles", type=int, default=16777216)
     ap.add_argument("--batch-payload", type=int, default=1)
-    ap.add_argument("--frame-batch", type=int, default=4,
+    ap.add_argument("--frame-batch", type=int, default=8,
                     help="orbit frames of the main workload per aarbf_render_batch call in the frame-batch leg "
                          "(FP16 and FP32 payload; 0 = skip)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "render_traffic.json"),
