@@ -72,6 +72,17 @@ struct DiffArgs {
 // Ratio tracking ends a sample after the first point with T < 2^-20.
 constexpr float kRatioTmin = 9.5367431640625e-07f;
 
+// The expected image and its adjoint are compared with the FP64 oracle to a
+// tolerance, not bit for bit, so the march's transcendentals use the
+// hardware's exp2 / log2 (about 1 ulp) instead of the renderer's
+// deterministic polynomials.
+__device__ inline float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+__device__ inline float fpow(float m, float y) { return m > 0.0f ? __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(m)) : 0.0f; }
+__device__ inline float frbf(float d, float r) {
+  const float q = (d * 3.0f) / r;
+  return fexp(-0.5f * (q * q));
+}
+
 __global__ __launch_bounds__(256) void diff_bake(const float4 *__restrict__ particles, const float *__restrict__ radii,
                                                  uint32_t n, const float4 *__restrict__ rmap, uint32_t rm_n,
                                                  const float4 *__restrict__ cmap, uint32_t cm_n,
@@ -95,7 +106,7 @@ __device__ inline float tf_grad(const float4 *t, uint32_t n, int c, float u, flo
 }
 
 __device__ inline float Fcdf(float L, float &dF) {
-  float e = aa_expf(-L);
+  float e = fexp(-L);
   float v = (1.0f - e) / 0.99f;
   if (v >= 1.0f) {
     dF = 0.0f;
@@ -399,7 +410,7 @@ __device__ inline void march(const DiffArgs &D, const DSeg &W, int px, int py, u
     pl.rho = pl.S[0] = pl.S[1] = pl.S[2] = 0.0f;
     cover([&](uint32_t p, const float4 &q, float dd) {
       const float4 k = A.pc[p];
-      const float w = aa_rbf(dd, q.w) * k.x;
+      const float w = frbf(dd, q.w) * k.x;
       pl.count += 1;
       pl.rho += w;
       pl.S[0] += k.y * w;
@@ -412,7 +423,7 @@ __device__ inline void march(const DiffArgs &D, const DSeg &W, int px, int py, u
 }
 
 __device__ inline void colour(const DPay &pl, float c[3]) {
-  for (int k = 0; k < 3; ++k) c[k] = aa_powf(pl.S[k] / pl.rho, 1.0f / 2.2f);
+  for (int k = 0; k < 3; ++k) c[k] = fpow(pl.S[k] / pl.rho, 1.0f / 2.2f);
 }
 
 #define AARBF_DSEG_LDS                                                                                      \
@@ -639,7 +650,7 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
         auto scatter = [&](uint32_t p, const float4 &q, float d) {
           const float4 k = A.pc[p];
           const float qq = (d * 3.0f) / q.w;
-          const float rbf = aa_expf(-0.5f * (qq * qq)), w = rbf * k.x;
+          const float rbf = fexp(-0.5f * (qq * qq)), w = rbf * k.x;
           const float a_w = a_rho + a_S[0] * k.y + a_S[1] * k.z + a_S[2] * k.w;
           const float a_rbf = a_w * k.x;
           const float kk = d > 0.0f ? a_rbf * (-9.0f / (q.w * q.w) * rbf) : 0.0f; // d(rbf)/dd / d
