@@ -1846,7 +1846,7 @@ __global__ void wf_lat_start(unsigned long long *lat, uint32_t nv) {
 template <bool B>
 static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &S, uint32_t *const lists[4],
                         uint32_t *listD, uint32_t *listL, uint32_t *cnt, int cur, uint32_t nG, uint32_t gb,
-                        uint32_t sb, uint32_t db, uint32_t *nD_out) {
+                        uint32_t sb, uint32_t db, uint32_t budget, uint32_t *nD_out) {
   const bool st = ctx->stats;
   uint32_t *lA = lists[cur], *lG = lists[cur + 1], *nA2 = lists[2 - cur], *nG2 = lists[3 - cur];
   uint32_t *cA = cnt + (cur ? 2 : 0), *cN = cnt + (cur ? 0 : 2);
@@ -1864,7 +1864,6 @@ static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &
       // long queue: one lane per ray, then one wave per ray for the walks
       // that ran over the budget (a fixed grid looping over their count)
       const size_t stack_bytes = sizeof(uint32_t) * kGBlock * std::max<uint32_t>(A.bvh.top, 2u) - sizeof(uint32_t) * kGBlock;
-      const uint32_t budget = (uint32_t)ctx->tune_walk_budget;
       uint32_t *cL = cnt + 6;
       wf_zero<<<1, 64, 0, ctx->stream>>>(cL, 1);
       if (st)
@@ -1988,7 +1987,8 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
       AARBF_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       int rc = AARBF_OK;
       for (int r = 0; r < R && rc == AARBF_OK; ++r)
-        rc = launch_round<B>(ctx, A, S, lists, listD, listL, cnt, (r & 1) ? 2 : 0, ~0u, gb, sb, db, nullptr);
+        rc = launch_round<B>(ctx, A, S, lists, listD, listL, cnt, (r & 1) ? 2 : 0, ~0u, gb, sb, db,
+                             (uint32_t)ctx->tune_walk_budget, nullptr);
       hipGraph_t graph = nullptr;
       const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
       if (rc != AARBF_OK) {
@@ -2015,14 +2015,23 @@ static int run_rounds(aarbf_context *ctx, const RenderArgs &A, const WFState &S,
     return AARBF_OK;
   }
   int cur = 0; // lists[cur] = A, lists[cur+1] = G; the other pair receives
+  // The walk budget starts at the tuned value and grows after a round in
+  // which more than 1/32 of a long gather queue ran over it (x2; x8 above
+  // 1/8): in scenes whose walks are long (dense clusters) the one-wave
+  // gather would otherwise take too many rays.
+  uint32_t budget = (uint32_t)ctx->tune_walk_budget;
   while (nA > 0) {
     uint32_t nD = 0;
     const uint32_t gb = (nG + kGBlock - 1) / kGBlock, sb = (nA + kSBlock - 1) / kSBlock;
-    int rc = launch_round<B>(ctx, A, S, lists, listD, listL, cnt, cur, nG, gb, sb, 0, &nD);
+    int rc = launch_round<B>(ctx, A, S, lists, listD, listL, cnt, cur, nG, gb, sb, 0, budget, &nD);
     if (rc != AARBF_OK) return rc;
-    uint32_t *cN = cnt + (cur ? 0 : 2);
-    AARBF_HIP(ctx, hipMemcpyAsync(h, cN, 8, hipMemcpyDeviceToHost, ctx->stream));
+    uint32_t c8[8];
+    AARBF_HIP(ctx, hipMemcpyAsync(c8, cnt, sizeof(c8), hipMemcpyDeviceToHost, ctx->stream));
     AARBF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    h[0] = c8[cur ? 0 : 2];
+    h[1] = c8[(cur ? 0 : 2) + 1];
+    if (nG > kWaveGather && 32ull * c8[6] > (uint64_t)nG)
+      budget = std::min<uint32_t>((8ull * c8[6] > (uint64_t)nG ? 8u : 2u) * budget, 65536u);
     if (ctx->timing) {
       float e[3];
       for (int k = 0; k < 3; ++k) AARBF_HIP(ctx, hipEventElapsedTime(&e[k], ctx->tev[k], ctx->tev[k + 1]));
