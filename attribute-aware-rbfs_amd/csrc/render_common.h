@@ -182,7 +182,7 @@ __device__ inline void point_query(const RenderArgs &A, float x, float y, float 
           const float e = aa_dist2(q.x, q.y, q.z, x, y, z) * A.pm[p];
           if (e > -4.5f) {
             float4 k = ld_pc(A, p);
-            float wa = aa_expf(e) * k.x;
+            float wa = aa_expf_mid(e) * k.x;
             if (STATS) c.covering++;
             pl.count += 1;
             pl.dens += wa;
@@ -492,7 +492,7 @@ __device__ inline void point_query_q(const RenderArgs &A, uint16_t *stk, float x
     const float e = aa_dist2(q.x, q.y, q.z, x, y, z) * A.pm[p];
     if (e > -4.5f) {
       float4 k = ld_pc(A, p);
-      float wa = aa_expf(e) * k.x;
+      float wa = aa_expf_mid(e) * k.x;
       if (STATS) c.covering++;
       pl.count += 1;
       pl.dens += wa;
@@ -544,7 +544,7 @@ __device__ inline void point_query_b(const RenderArgs &A, uint16_t *stk, float x
     const float e = aa_dist2(q.x, q.y, q.z, x, y, z) * A.pm[p];
     if (e > -4.5f) {
       float4 k = ld_pc(A, p);
-      float wa = aa_expf(e) * k.x;
+      float wa = aa_expf_mid(e) * k.x;
       if (STATS) c.covering++;
       pl.count += 1;
       pl.dens += wa;

@@ -1235,7 +1235,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
           if (STATS) c.particles++;
           const float e = aa_dist2(qq.x, qq.y, qq.z, x, y, z) * A.pm[p];
           if (e > -4.5f) {
-            float wa = aa_expf(e) * kk.x;
+            float wa = aa_expf_mid(e) * kk.x;
             if (STATS) c.covering++;
             pl.count += 1;
             pl.dens += wa;
@@ -1496,7 +1496,7 @@ __global__ __launch_bounds__(kSBlock, 4) void wf_step_m(RenderArgs A, WFState S,
             if (STATS) c.particles++;
             const float e = aa_dist2(qq.x, qq.y, qq.z, x, y, z) * A.pm[p];
             if (e > -4.5f) {
-              float wa = aa_expf(e) * kk.x;
+              float wa = aa_expf_mid(e) * kk.x;
               if (STATS) c.covering++;
               pl.count += 1;
               pl.dens += wa;
@@ -1663,7 +1663,7 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
           const float e = aa_dist2(q.x, q.y, q.z, x, y, z) * A.pm[p];
           if (e > -4.5f) {
             const float4 k = ld_pc(A, p);
-            wa = aa_expf(e) * k.x;
+            wa = aa_expf_mid(e) * k.x;
             wr = k.y * wa;
             wg = k.z * wa;
             wb = k.w * wa;

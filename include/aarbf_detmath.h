@@ -200,6 +200,23 @@ AA_FN float aa_rbf(float d, float r) {
   return aa_expf(-0.5f * (q * q));
 }
 
+/* aa_expf for -87 <= x <= 87 (no NaN, overflow or underflow handling):
+ * the same operations on that range, so the same bits.  The renderer's
+ * RBF weights take it (their argument lies in (-4.5, 0]). */
+AA_FN float aa_expf_mid(float x) {
+  float k = AA_RINTF(x * 1.4426950216293335f);
+  float r = AA_FMAF(k, -0.693145751953125f, x);
+  r = AA_FMAF(k, -1.4286068203094172e-06f, r);
+  float p = 0.00019899285689461976f;
+  p = AA_FMAF(p, r, 0.0013933650916442275f);
+  p = AA_FMAF(p, r, 0.0083332983776927f);
+  p = AA_FMAF(p, r, 0.04166646674275398f);
+  p = AA_FMAF(p, r, 0.1666666716337204f);
+  p = AA_FMAF(p, r, 0.5f);
+  float y = AA_FMAF(r * r, p, r) + 1.0f;
+  return y * aa_u2f((uint32_t)((int)k + 127) << 23);
+}
+
 /* over (sharedCode.h:142-149): a composited over b. */
 AA_FN void aa_over(const float a[4], const float b[4], float out[4]) {
   float ra = a[3] + b[3] * (1.0f - a[3]);

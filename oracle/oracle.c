@@ -1472,6 +1472,18 @@ ORC_EXPORT void orc_math(int which, const float *x, const float *y, float *out, 
 
 /* Number of arguments of the tracker-step logarithm's domain (0 and every
  * binary32 in [2^-24, 1]) where aa_log_step and aa_logf differ bitwise. */
+/* Number of binary32 x in [-87, 87] where aa_expf_mid and aa_expf differ
+ * bitwise (every 13th binary32 in [-4.5, 4.5], every 1009th beyond). */
+ORC_EXPORT uint64_t orc_expf_mid_mismatches(void) {
+  uint64_t bad = 0;
+  for (uint32_t u = 0u; u <= 0x42ae0000u; u += (u < 0x40900000u ? 13u : 1009u)) { /* [0, 87] */
+    float x = aa_u2f(u), y = -x;
+    if (aa_f2u(aa_expf_mid(x)) != aa_f2u(aa_expf(x))) ++bad;
+    if (aa_f2u(aa_expf_mid(y)) != aa_f2u(aa_expf(y))) ++bad;
+  }
+  return bad;
+}
+
 ORC_EXPORT uint64_t orc_log_step_mismatches(void) {
   uint64_t bad = 0;
   if (aa_f2u(aa_log_step(0.0f)) != aa_f2u(aa_logf(0.0f))) ++bad;

@@ -193,6 +193,12 @@ def test_logf_within_one_ulp():
     assert np.isnan(_math(0, np.array([-1.0], np.float32))[0])
 
 
+def test_expf_mid_equals_expf_on_its_domain():
+    L = oracle.lib()
+    L.orc_expf_mid_mismatches.restype = C.c_uint64
+    assert L.orc_expf_mid_mismatches() == 0
+
+
 def test_log_step_equals_logf_on_its_domain():
     # exhaustive: 0 and all 2e8 binary32 values in [2^-24, 1]
     L = oracle.lib()
