@@ -86,7 +86,8 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
   };
   // every particle of a leaf met by [t0, t1] whose sample lies in [t0, t1),
   // in ascending order; returns how many there are (only kSplatCap kept)
-  auto collect = [&](float t0, float t1) -> uint32_t {
+  // and sets tcut to the smallest sample t among the hits not kept
+  auto collect = [&](float t0, float t1, float &tcut) -> uint32_t {
     uint32_t cnt = 0, level = top, idx = 0;
     for (;;) {
       const float4 *lv = N + g_lvl_ofs[level];
@@ -109,6 +110,8 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
               if (cnt < kSplatCap) {
                 hid[cnt * kSplatBlock] = p;
                 ht[cnt * kSplatBlock] = t;
+              } else {
+                tcut = fminf(tcut, t);
               }
               ++cnt;
             }
@@ -148,19 +151,24 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
         c1 = fminf(texit, tl + sp);
         tl = tl + sp;
       }
-      const uint32_t cnt = collect(c0, c1);
-      if (cnt > kSplatCap && S > 1) {
-        S = max(1u, S / 2u);
-        continue;
-      }
+      float tcut = INFINITY;
+      const uint32_t cnt = collect(c0, c1, tcut);
+      // An overflowing list holds every hit with t < tcut (the hits left
+      // out all lie at or after it), so each slab ending at or before tcut
+      // finds its whole set in the list, in the same order; the chunk's
+      // remaining slabs go to the next walk.  A one-slab chunk keeps the
+      // first kSplatCap hits, which hold its first 15.
+      const bool partial = cnt > kSplatCap && S > 1;
       const uint32_t m = min(cnt, kSplatCap);
       uint32_t first = 0; // list entries before the current slab
-      for (uint32_t k = 0; k < ns; ++k) {
+      uint32_t k = 0;
+      for (; k < ns; ++k) {
         if (res.w > 0.99f) {
           done = true;
           break;
         }
         const float tmin = fmaxf(tenter, tslab), tmax = fminf(texit, tslab + sp);
+        if (partial && !(tmax <= tcut)) break;
         // payload: the first 15 hits of this slab in traversal order
         uint32_t tail = 0;
         for (uint32_t e = first; e < m && tail < kPayload; ++e) {
@@ -213,7 +221,10 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
         // order, not t order, so only a prefix can be skipped)
         while (first < m && ht[first * kSplatBlock] < tmax) ++first;
       }
-      if (cnt <= kSplatCap / 2 && S < kChunkMax) S *= 2;
+      if (partial)
+        S = k ? k : max(1u, S / 2u);
+      else if (cnt <= kSplatCap / 2)
+        S = min(2u * S, kChunkMax);
     }
   }
   over4(res, make_float4(0.0f, 0.0f, 0.0f, 1.0f));

@@ -53,6 +53,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--diff-views", type=int, default=2,
                     help="views of the differentiable-rendering leg timed (0 = skip the leg)")
+    ap.add_argument("--splat-frames", type=int, default=2, help="full-size splat frames timed (0 = no splat leg)")
     ap.add_argument("--diff-particles", type=int, default=1048576)
     ap.add_argument("--batch-views", type=int, default=0,
                     help="camera requests of the batching leg (0 = skip the leg; the batching workload has 256)")
@@ -233,6 +234,7 @@ def main() -> int:
     ctx.set_payload(0)
     fbatch = frame_batch_leg(ctx, stream, a, sc, wl, cams, msps if a.payload == 0 and a.mode == 0 else None) \
         if a.frame_batch > 0 else None
+    splat = splat_leg(ctx, stream, a, sc, wl, cams) if a.splat_frames > 0 else None
     diff = diff_leg(ctx, stream, a) if a.diff_views > 0 else None
     batch = batch_leg(ctx, stream, a) if a.batch_views > 0 else None
 
@@ -271,6 +273,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "frame_batch": fbatch,
+        "splat": splat,
         "diff": diff,
         "batch": batch,
         "wall_s": round(wall, 3),
@@ -323,6 +326,49 @@ def frame_batch_leg(ctx, stream, a, sc, wl, cams, plain_msps):
         out["fp16_batched_over_plain_fp32"] = round(out["fp16_batched_msamples_per_s"] / plain_msps, 3)
     del bufs
     torch.cuda.synchronize()
+    return out
+
+
+def splat_leg(ctx, stream, a, sc, wl, cams):
+    """The splat path (ParticleSplatRayGen, deviceCodeSplat.hlsl:29-194,
+    one ray per pixel) on the main workload's scene and orbit cameras.
+    One frame at a quarter of each side is timed first; the full-size
+    frames follow only when that frame predicts at most 20 s per full
+    frame, so the leg stays bounded whatever the scene."""
+    import torch
+    import aarbf
+    from aarbf import scenes
+    W, H = wl.width, wl.height
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(w, h, cam):
+        img = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
+        rg = sc.raygen_data(img, img, w, h)
+        e0.record(stream)
+        ctx.render_splat(rg, sc.push_constants(cam, wl.unit), w, h)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), float((img[..., 3] > 0).float().mean())
+
+    qw, qh = (W + 3) // 4, (H + 3) // 4
+    lf, la, lu = scenes.orbit_camera(1, len(cams), 1.6)
+    qcam = aarbf.camera_basis(lf, la, lu, wl.cos_fovy, qw, qh)
+    q_ms, _ = timed(qw, qh, qcam)
+    log(f"splat {qw}x{qh}: {q_ms:.1f} ms")
+    out = {"workload": f"splat, one ray per pixel, scene of the main workload, orbit frames 1..{a.splat_frames}",
+           "quarter_size": [qw, qh], "quarter_ms": round(q_ms, 2)}
+    if q_ms * (W * H) / (qw * qh) > 20e3:
+        out["skipped_full_size"] = "predicted over 20 s per frame"
+        return out
+    ms = []
+    lit = 0.0
+    for k in range(a.splat_frames):
+        t, lit = timed(W, H, cams[(1 + k) % len(cams)])
+        ms.append(t)
+        log(f"splat {W}x{H} frame {k + 1}/{a.splat_frames}: {t:.1f} ms")
+    m = float(np.mean(ms))
+    out.update({"width": W, "height": H, "frames": a.splat_frames, "ms_per_frame": round(m, 2),
+                "mrays_per_s": round(W * H / (m * 1e-3) / 1e6, 3), "lit_fraction": round(lit, 3)})
     return out
 
 
