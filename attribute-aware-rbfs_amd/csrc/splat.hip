@@ -10,9 +10,13 @@
 // chunk's segment meets and whose splat sample lies in the chunk.  Each
 // slab then takes from that list the hits whose t falls in the slab and
 // whose leaf box its own segment meets -- the same set, in the same order,
-// that a walk of the slab alone reports.  A chunk whose list overflows the
-// per-lane buffer is halved (down to a single slab, where only the first 15
-// hits matter).
+// that a walk of the slab alone reports.  A list that overflows the
+// per-lane buffer still holds every hit before the smallest t left out, so
+// the slabs ending there are composited and the rest of the chunk is walked
+// again; a chunk none of whose slabs fit is halved (down to a single slab,
+// where only the first 15 hits matter).  The walk opens a node by testing
+// all its children's boxes, loaded together, and keeps the children still
+// to visit as one bit mask per level in LDS.
 //
 // Box tests (aa_seg_box) use the leaf boxes the BVH stores (GenRBFBounds
 // boxes widened by 2^-20 relative); hits are taken in ascending particle
@@ -51,8 +55,8 @@ __device__ inline void over4(float4 &res, const float4 &c) {
 __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
   __shared__ uint32_t s_id[kSplatCap * kSplatBlock];
   __shared__ float s_t[kSplatCap * kSplatBlock];
-  __shared__ uint32_t s_pid[kPayload * kSplatBlock];
-  __shared__ float s_pt[kPayload * kSplatBlock];
+  __shared__ uint8_t s_pe[kPayload * kSplatBlock];    // a slab's payload: positions in the hit list
+  __shared__ uint16_t s_mask[kMaxLevels * kSplatBlock]; // per level: siblings still to visit
   if (threadIdx.x < kMaxLevels) {
     g_lvl_ofs[threadIdx.x] = A.bvh.level_ofs[threadIdx.x];
     g_lvl_cnt[threadIdx.x] = A.bvh.counts[threadIdx.x];
@@ -60,8 +64,10 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
   __syncthreads();
   const int px = (int)(blockIdx.x * 8u + (threadIdx.x & 7u)), py = (int)(blockIdx.y * 8u + (threadIdx.x >> 3));
   if (px >= A.W || py >= A.H) return;
-  uint32_t *hid = s_id + threadIdx.x, *pid = s_pid + threadIdx.x;
-  float *ht = s_t + threadIdx.x, *pt = s_pt + threadIdx.x;
+  uint32_t *hid = s_id + threadIdx.x;
+  float *ht = s_t + threadIdx.x;
+  uint8_t *pe = s_pe + threadIdx.x;
+  uint16_t *smask = s_mask + threadIdx.x;
 
   const float sx = ((float)px + 0.5f) / (float)A.W, sy = ((float)py + 0.5f) / (float)A.H;
   float3 d;
@@ -84,23 +90,87 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
     const float4 lo = lb[2 * leaf], hi = lb[2 * leaf + 1];
     return aa_seg_box(o.x, o.y, o.z, iv.x, iv.y, iv.z, t0, t1, lo.x, lo.y, lo.z, hi.x, hi.y, hi.z) != 0;
   };
+  // Nodes first .. first + K - 1 of `level` (clipped to the level's count)
+  // against [t0, t1]: bit j set when node first + j meets the segment.
+  // The boxes are loaded eight at a time before any is tested, so a group
+  // costs one memory round trip instead of one per node (lanes past the
+  // group's end reload its last box and are masked off).
+  auto group_mask = [&](uint32_t level, uint32_t first, float t0, float t1) -> uint32_t {
+    const float4 *lv = N + g_lvl_ofs[level] + 2u * first;
+    const uint32_t n = min(kmask + 1u, g_lvl_cnt[level] - first);
+    uint32_t m = 0;
+    for (uint32_t h = 0; h < n; h += 8u) {
+      float4 lo[8], hi[8];
+#pragma unroll
+      for (uint32_t j = 0; j < 8u; ++j) {
+        const uint32_t c = min(h + j, n - 1u);
+        lo[j] = lv[2u * c];
+        hi[j] = lv[2u * c + 1u];
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < 8u; ++j)
+        if (h + j < n && aa_seg_box(o.x, o.y, o.z, iv.x, iv.y, iv.z, t0, t1, lo[j].x, lo[j].y, lo[j].z, hi[j].x,
+                                    hi[j].y, hi[j].z))
+          m |= 1u << (h + j);
+    }
+    return m;
+  };
   // every particle of a leaf met by [t0, t1] whose sample lies in [t0, t1),
   // in ascending order; returns how many there are (only kSplatCap kept)
-  // and sets tcut to the smallest sample t among the hits not kept
+  // and sets tcut to the smallest sample t among the hits not kept.  The
+  // walk is depth first in ascending node order: `m` holds the current
+  // group's children still to visit, smask[level] those of the groups above.
   auto collect = [&](float t0, float t1, float &tcut) -> uint32_t {
-    uint32_t cnt = 0, level = top, idx = 0;
+    uint32_t cnt = 0;
+    if (!group_mask(top, 0u, t0, t1)) return 0; // the root
+    uint32_t level = top - 1u, base = 0;
+    uint32_t m = group_mask(level, 0u, t0, t1);
     for (;;) {
-      const float4 *lv = N + g_lvl_ofs[level];
-      const float4 lo = lv[2 * idx], hi = lv[2 * idx + 1];
-      if (aa_seg_box(o.x, o.y, o.z, iv.x, iv.y, iv.z, t0, t1, lo.x, lo.y, lo.z, hi.x, hi.y, hi.z)) {
-        if (level != 1) {
-          --level;
-          idx <<= log2K;
-          continue;
+      if (level != 0 && m) { // open the next child
+        const uint32_t node = base + (uint32_t)__builtin_ctz(m);
+        smask[level * kSplatBlock] = (uint16_t)(m & (m - 1u));
+        --level;
+        base = node << log2K;
+        m = group_mask(level, base, t0, t1);
+        continue;
+      }
+      if (level == 0 && A.ppl == 1u && A.n != 0) {
+        // one particle per leaf: the particles of up to eight met leaves
+        // are loaded together, then tested in ascending order
+        while (m) {
+          uint32_t id[8];
+          float4 pc[8];
+          float pr[8];
+#pragma unroll
+          for (uint32_t j = 0; j < 8u; ++j) {
+            id[j] = m ? base + (uint32_t)__builtin_ctz(m) : ~0u;
+            m &= m - 1u;
+          }
+#pragma unroll
+          for (uint32_t j = 0; j < 8u; ++j) {
+            const uint32_t q = min(id[j], A.n - 1u); // unused slots (~0u) and leaves past n load particle n - 1
+            pc[j] = A.particles[q];
+            pr[j] = A.radii ? A.radii[q] : A.rbf;
+          }
+#pragma unroll
+          for (uint32_t j = 0; j < 8u; ++j) {
+            float t;
+            if (id[j] < A.n &&
+                aa_splat_hit(pc[j].x, pc[j].y, pc[j].z, pr[j], o.x, o.y, o.z, d.x, d.y, d.z, t0, t1, &t)) {
+              if (cnt < kSplatCap) {
+                hid[cnt * kSplatBlock] = id[j];
+                ht[cnt * kSplatBlock] = t;
+              } else {
+                tcut = fminf(tcut, t);
+              }
+              ++cnt;
+            }
+          }
         }
-        const uint32_t l0 = idx << log2K, l1 = min(l0 + kmask + 1u, g_lvl_cnt[0]);
-        for (uint32_t leaf = l0; leaf < l1; ++leaf) {
-          if (!leaf_meets(leaf, t0, t1)) continue;
+      }
+      for (; level == 0 && m; m &= m - 1u) {
+        const uint32_t leaf = base + (uint32_t)__builtin_ctz(m);
+        {
           const uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.n);
           for (uint32_t p = p0; p < p1; ++p) {
             const float4 c = A.particles[p];
@@ -118,15 +188,11 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
           }
         }
       }
-      for (;;) { // next node in depth-first order
-        if (level == top) return cnt;
-        if ((idx & kmask) != kmask && idx + 1 < g_lvl_cnt[level]) {
-          ++idx;
-          break;
-        }
-        ++level;
-        idx >>= log2K;
-      }
+      // this group is done: back to the parent's group
+      if (level == top - 1u) return cnt;
+      ++level;
+      base = (base >> log2K) & ~kmask;
+      m = smask[level * kSplatBlock];
     }
   };
 
@@ -176,27 +242,25 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
           if (!(tmin <= t && t < tmax)) continue;
           const uint32_t p = hid[e * kSplatBlock];
           if (!leaf_meets(p / A.ppl, tmin, tmax)) continue;
-          pid[tail * kSplatBlock] = p;
-          pt[tail * kSplatBlock] = t;
+          pe[tail * kSplatBlock] = (uint8_t)e;
           ++tail;
         }
         // SplatPayload::sort (':44-55'): stable by t
         for (uint32_t a = 1; a < tail; ++a) {
-          const uint32_t hp = pid[a * kSplatBlock];
-          const float ht_ = pt[a * kSplatBlock];
+          const uint32_t ea = pe[a * kSplatBlock];
+          const float ta = ht[ea * kSplatBlock];
           uint32_t b = a;
-          while (b > 0 && pt[(b - 1) * kSplatBlock] > ht_) {
-            pid[b * kSplatBlock] = pid[(b - 1) * kSplatBlock];
-            pt[b * kSplatBlock] = pt[(b - 1) * kSplatBlock];
+          while (b > 0 && ht[pe[(b - 1) * kSplatBlock] * kSplatBlock] > ta) {
+            pe[b * kSplatBlock] = pe[(b - 1) * kSplatBlock];
             --b;
           }
-          pid[b * kSplatBlock] = hp;
-          pt[b * kSplatBlock] = ht_;
+          pe[b * kSplatBlock] = (uint8_t)ea;
         }
         // front-to-back integration (':117-140')
         for (uint32_t e = 0; e < tail; ++e) {
-          const uint32_t p = pid[e * kSplatBlock];
-          const float t = pt[e * kSplatBlock];
+          const uint32_t le = pe[e * kSplatBlock];
+          const uint32_t p = hid[le * kSplatBlock];
+          const float t = ht[le * kSplatBlock];
           const float4 c = A.particles[p];
           const float r = A.radii ? A.radii[p] : A.rbf;
           const float x = o.x + d.x * t, y = o.y + d.y * t, z = o.z + d.z * t;

@@ -45,6 +45,7 @@ CASES = [
     ("ppl3-K4", 3000, 3, 4, True, True),
     ("global-radius-K2", 2000, 1, 2, False, True),
     ("density-colour-ppl2-K16", 2500, 2, 16, True, False),
+    ("ppl1-K16", 2800, 1, 16, True, True),
 ]
 
 
