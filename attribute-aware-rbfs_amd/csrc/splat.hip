@@ -31,6 +31,7 @@ constexpr int kSplatBlock = 64;       // one wave, an 8x8 pixel tile
 constexpr uint32_t kSplatCap = 32;    // hits one chunk may collect per lane
 constexpr uint32_t kPayload = 15;     // PARTICLE_BUFFER_SIZE (deviceCodeSplat.hlsl:37)
 constexpr uint32_t kChunkMax = 512;   // slabs per chunk
+constexpr uint32_t kWalkLimit = 4 * kSplatCap; // hits after which a walk of several slabs gives up
 
 struct SplatArgs {
   int W, H;
@@ -117,10 +118,11 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
   };
   // every particle of a leaf met by [t0, t1] whose sample lies in [t0, t1),
   // in ascending order; returns how many there are (only kSplatCap kept)
-  // and sets tcut to the smallest sample t among the hits not kept.  The
+  // and sets tcut to the smallest sample t among the hits not kept; it
+  // stops at `limit` hits (the list and tcut are then incomplete).  The
   // walk is depth first in ascending node order: `m` holds the current
   // group's children still to visit, smask[level] those of the groups above.
-  auto collect = [&](float t0, float t1, float &tcut) -> uint32_t {
+  auto collect = [&](float t0, float t1, float &tcut, uint32_t limit) -> uint32_t {
     uint32_t cnt = 0;
     if (!group_mask(top, 0u, t0, t1)) return 0; // the root
     uint32_t level = top - 1u, base = 0;
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
               } else {
                 tcut = fminf(tcut, t);
               }
-              ++cnt;
+              if (++cnt >= limit) return cnt;
             }
           }
         }
@@ -183,7 +185,7 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
               } else {
                 tcut = fminf(tcut, t);
               }
-              ++cnt;
+              if (++cnt >= limit) return cnt;
             }
           }
         }
@@ -218,7 +220,16 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
         tl = tl + sp;
       }
       float tcut = INFINITY;
-      const uint32_t cnt = collect(c0, c1, tcut);
+      // A chunk of several slabs that runs into a dense region would visit
+      // every leaf its whole segment meets just to find tcut: its walk
+      // gives up at kWalkLimit hits, composites nothing, and the chunk
+      // shrinks to a quarter (at least kWalkLimit hits in S slabs means
+      // about S / 4 slabs hold kSplatCap).  A one-slab chunk walks to the end.
+      const uint32_t cnt = collect(c0, c1, tcut, S > 1 ? kWalkLimit : ~0u);
+      if (S > 1 && cnt >= kWalkLimit) {
+        S = max(1u, S / 4u);
+        continue;
+      }
       // An overflowing list holds every hit with t < tcut (the hits left
       // out all lie at or after it), so each slab ending at or before tcut
       // finds its whole set in the list, in the same order; the chunk's
