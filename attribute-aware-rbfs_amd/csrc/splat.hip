@@ -14,7 +14,8 @@
 // per-lane buffer still holds every hit before the smallest t left out, so
 // the slabs ending there are composited and the rest of the chunk is walked
 // again; a chunk none of whose slabs fit is halved (down to a single slab,
-// where only the first 15 hits matter).  The walk opens a node by testing
+// where only the first 15 hits matter), and a walk of several slabs that
+// finds kWalkLimit hits gives up and leaves a chunk a quarter the size.  The walk opens a node by testing
 // all its children's boxes, loaded together, and keeps the children still
 // to visit as one bit mask per level in LDS.
 //
