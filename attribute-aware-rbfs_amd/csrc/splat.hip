@@ -10,14 +10,12 @@
 // chunk's segment meets and whose splat sample lies in the chunk.  Each
 // slab then takes from that list the hits whose t falls in the slab and
 // whose leaf box its own segment meets -- the same set, in the same order,
-// that a walk of the slab alone reports.  A list that overflows the
-// per-lane buffer still holds every hit before the smallest t left out, so
-// the slabs ending there are composited and the rest of the chunk is walked
-// again; a chunk none of whose slabs fit is halved (down to a single slab,
-// where only the first 15 hits matter), and a walk of several slabs that
-// finds kWalkLimit hits gives up and leaves a chunk a quarter the size.  The walk opens a node by testing
-// all its children's boxes, loaded together, and keeps the children still
-// to visit as one bit mask per level in LDS.
+// that a walk of the slab alone reports.  A walk of several slabs stops at
+// the first hit its per-lane list cannot hold, and the chunk shrinks to a
+// quarter (down to a single slab, which is walked to the end and where only
+// the first 15 hits matter).  The walk opens a node by testing all its
+// children's boxes, loaded together, and keeps the children still to visit
+// as one bit mask per level in LDS.
 //
 // Box tests (aa_seg_box) use the leaf boxes the BVH stores (GenRBFBounds
 // boxes widened by 2^-20 relative); hits are taken in ascending particle
@@ -32,7 +30,6 @@ constexpr int kSplatBlock = 64;       // one wave, an 8x8 pixel tile
 constexpr uint32_t kSplatCap = 32;    // hits one chunk may collect per lane
 constexpr uint32_t kPayload = 15;     // PARTICLE_BUFFER_SIZE (deviceCodeSplat.hlsl:37)
 constexpr uint32_t kChunkMax = 512;   // slabs per chunk
-constexpr uint32_t kWalkLimit = 4 * kSplatCap; // hits after which a walk of several slabs gives up
 
 struct SplatArgs {
   int W, H;
@@ -119,11 +116,10 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
   };
   // every particle of a leaf met by [t0, t1] whose sample lies in [t0, t1),
   // in ascending order; returns how many there are (only kSplatCap kept)
-  // and sets tcut to the smallest sample t among the hits not kept; it
-  // stops at `limit` hits (the list and tcut are then incomplete).  The
+  // and stops at `limit` hits (the list is then incomplete).  The
   // walk is depth first in ascending node order: `m` holds the current
   // group's children still to visit, smask[level] those of the groups above.
-  auto collect = [&](float t0, float t1, float &tcut, uint32_t limit) -> uint32_t {
+  auto collect = [&](float t0, float t1, uint32_t limit) -> uint32_t {
     uint32_t cnt = 0;
     if (!group_mask(top, 0u, t0, t1)) return 0; // the root
     uint32_t level = top - 1u, base = 0;
@@ -163,8 +159,6 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
               if (cnt < kSplatCap) {
                 hid[cnt * kSplatBlock] = id[j];
                 ht[cnt * kSplatBlock] = t;
-              } else {
-                tcut = fminf(tcut, t);
               }
               if (++cnt >= limit) return cnt;
             }
@@ -183,8 +177,6 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
               if (cnt < kSplatCap) {
                 hid[cnt * kSplatBlock] = p;
                 ht[cnt * kSplatBlock] = t;
-              } else {
-                tcut = fminf(tcut, t);
               }
               if (++cnt >= limit) return cnt;
             }
@@ -220,33 +212,25 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
         c1 = fminf(texit, tl + sp);
         tl = tl + sp;
       }
-      float tcut = INFINITY;
-      // A chunk of several slabs that runs into a dense region would visit
-      // every leaf its whole segment meets just to find tcut: its walk
-      // gives up at kWalkLimit hits, composites nothing, and the chunk
-      // shrinks to a quarter (at least kWalkLimit hits in S slabs means
-      // about S / 4 slabs hold kSplatCap).  A one-slab chunk walks to the end.
-      const uint32_t cnt = collect(c0, c1, tcut, S > 1 ? kWalkLimit : ~0u);
-      if (S > 1 && cnt >= kWalkLimit) {
+      // A chunk of several slabs whose hits overflow the list is walked
+      // no further than its first hit past the list (a chunk that has grown
+      // long in a sparse stretch and runs into a dense region would
+      // otherwise visit every leaf its whole segment meets); it composites
+      // nothing and shrinks to a quarter.  A one-slab chunk walks to the
+      // end and keeps the first kSplatCap hits, which hold its first 15.
+      const uint32_t cnt = collect(c0, c1, S > 1 ? kSplatCap + 1u : ~0u);
+      if (S > 1 && cnt > kSplatCap) {
         S = max(1u, S / 4u);
         continue;
       }
-      // An overflowing list holds every hit with t < tcut (the hits left
-      // out all lie at or after it), so each slab ending at or before tcut
-      // finds its whole set in the list, in the same order; the chunk's
-      // remaining slabs go to the next walk.  A one-slab chunk keeps the
-      // first kSplatCap hits, which hold its first 15.
-      const bool partial = cnt > kSplatCap && S > 1;
       const uint32_t m = min(cnt, kSplatCap);
       uint32_t first = 0; // list entries before the current slab
-      uint32_t k = 0;
-      for (; k < ns; ++k) {
+      for (uint32_t k = 0; k < ns; ++k) {
         if (res.w > 0.99f) {
           done = true;
           break;
         }
         const float tmin = fmaxf(tenter, tslab), tmax = fminf(texit, tslab + sp);
-        if (partial && !(tmax <= tcut)) break;
         // payload: the first 15 hits of this slab in traversal order
         uint32_t tail = 0;
         for (uint32_t e = first; e < m && tail < kPayload; ++e) {
@@ -297,10 +281,7 @@ __global__ __launch_bounds__(kSplatBlock) void splat_kernel(SplatArgs A) {
         // order, not t order, so only a prefix can be skipped)
         while (first < m && ht[first * kSplatBlock] < tmax) ++first;
       }
-      if (partial)
-        S = k ? k : max(1u, S / 2u);
-      else if (cnt <= kSplatCap / 2)
-        S = min(2u * S, kChunkMax);
+      if (cnt <= kSplatCap / 2) S = min(2u * S, kChunkMax);
     }
   }
   over4(res, make_float4(0.0f, 0.0f, 0.0f, 1.0f));
