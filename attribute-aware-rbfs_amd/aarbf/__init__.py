@@ -22,8 +22,8 @@ from ._lib import (BoundsConstants, Float3, Int2, PushConstants, RayGenData, Ren
 MAX_DEPTH = 10000  # sharedCode.h:27
 
 __all__ = [
-    "AarbfError", "Context", "Scene", "Camera", "camera_basis", "PushConstants", "RayGenData",
-    "BoundsConstants", "TAAConstants", "RenderStats", "MAX_DEPTH", "load_library",
+    "AarbfError", "Context", "Scene", "Camera", "camera_basis", "light_direction", "CameraController",
+    "PushConstants", "RayGenData", "BoundsConstants", "TAAConstants", "RenderStats", "MAX_DEPTH", "load_library",
 ]
 
 
@@ -59,27 +59,70 @@ class Camera:
     dir_dv: np.ndarray
 
 
+def _camera_of(pc: PushConstants) -> Camera:
+    c = pc.camera
+    v = lambda f: np.array([f.x, f.y, f.z], dtype=np.float32)
+    return Camera(v(c.pos), v(c.dir_00), v(c.dir_du), v(c.dir_dv))
+
+
 def camera_basis(look_from, look_at, look_up, cos_fovy: float, width: int, height: int) -> Camera:
-    """Camera basis of hostCode.cpp:669-682 in float32 arithmetic.
+    """Camera basis of hostCode.cpp:669-682 (aarbf_camera_basis, binary32).
 
     cos_fovy is the reference's image-plane scale (its 10th --camera value);
     for a true vertical field of view f it is 2*tan(f/2).
     """
-    f = np.float32
-    lf = np.asarray(look_from, dtype=f)
-    la = np.asarray(look_at, dtype=f)
-    lu = np.asarray(look_up, dtype=f)
+    lib = _lib.load()
+    f = lambda v: np.ascontiguousarray(np.asarray(v, dtype=np.float32).reshape(3))
+    a, b, c = f(look_from), f(look_at), f(look_up)
+    pc = PushConstants()
+    st = lib.aarbf_camera_basis(_fptr(a), _fptr(b), _fptr(c), float(cos_fovy), int(width), int(height),
+                                C.byref(pc))
+    if st != 0:
+        raise AarbfError(st, "aarbf_camera_basis: invalid arguments")
+    return _camera_of(pc)
 
-    def normalize(v):
-        return (v / np.sqrt(np.sum(v * v, dtype=f), dtype=f)).astype(f)
 
-    d00 = normalize(la - lf)
-    aspect = f(width) / f(height)
-    ddu = (f(cos_fovy) * aspect * normalize(np.cross(d00, lu).astype(f))).astype(f)
-    ddv = (f(cos_fovy) * normalize(np.cross(ddu, d00).astype(f))).astype(f)
-    d00 = (d00 - f(0.5) * ddu).astype(f)
-    d00 = (d00 - f(0.5) * ddv).astype(f)
-    return Camera(lf.copy(), d00, ddu, ddv)
+def light_direction(azimuth: float, elevation: float) -> np.ndarray:
+    """getLightDirection (sharedCode.h:127-132), as the renderer evaluates it."""
+    out = np.empty(3, dtype=np.float32)
+    st = _lib.load().aarbf_light_direction(float(azimuth), float(elevation), _fptr(out))
+    if st != 0:
+        raise AarbfError(st, "aarbf_light_direction")
+    return out
+
+
+class CameraController:
+    """The reference's interactive camera (hostCode.cpp:593-731) over
+    aarbf_camera_update: call update() once per frame with the cursor
+    movement and the buttons / keys held."""
+
+    def __init__(self, look_from, look_at, look_up, cos_fovy: float, width: int, height: int,
+                 diagonal: float = 1.0):
+        self.state = _lib.CameraState(_f3(look_from), _f3(look_at), _f3(look_up), float(cos_fovy), 1)
+        self.width, self.height, self.diagonal = int(width), int(height), float(diagonal)
+        self.pc = PushConstants()
+
+    def update(self, dx: float = 0.0, dy: float = 0.0, left=False, right=False, middle=False, key_x=False,
+               key_y=False, key_z=False, left_shift=False) -> bool:
+        """One frame of input; returns True when the camera moved (accumID reset to 1)."""
+        inp = _lib.CameraInput(float(dx), float(dy), int(bool(left)), int(bool(right)), int(bool(middle)),
+                               int(bool(key_x)), int(bool(key_y)), int(bool(key_z)), int(bool(left_shift)))
+        changed = C.c_int()
+        st = _lib.load().aarbf_camera_update(C.byref(self.state), C.byref(inp), self.width, self.height,
+                                             self.diagonal, C.byref(self.pc), C.byref(changed))
+        if st != 0:
+            raise AarbfError(st, "aarbf_camera_update: invalid arguments")
+        return bool(changed.value)
+
+    @property
+    def camera(self) -> Camera:
+        return _camera_of(self.pc)
+
+    @property
+    def look(self):
+        s = self.state
+        v = lambda f: np.array([f.x, f.y, f.z], dtype=np.float32)
+        return v(s.lookFrom), v(s.lookAt), v(s.lookUp)
 
 
 def _ordered(fn):
@@ -140,6 +183,12 @@ class Context:
 
     def sync(self):
         self.check(self.lib.aarbf_sync(self.h), "aarbf_sync")
+
+    def handle_counts(self) -> dict:
+        """Live transfer-function, blue-noise and BVH handles on the context."""
+        c = (C.c_uint32 * 3)()
+        self.check(self.lib.aarbf_handle_counts(self.h, c), "aarbf_handle_counts")
+        return {"tf": int(c[0]), "stbn": int(c[1]), "bvh": int(c[2])}
 
     def set_render_mode(self, mode: int):
         """0: persistent segment-gather kernel (default); 1: per-pixel point queries."""

@@ -79,6 +79,18 @@ class TuneResult(C.Structure):
                 ("walk_budget", C.c_int32)]
 
 
+class CameraState(C.Structure):
+    """The reference's interactive camera (hostCode.cpp:253-271)."""
+    _fields_ = [("lookFrom", Float3), ("lookAt", Float3), ("lookUp", Float3), ("cosFovy", f32),
+                ("firstFrame", i32)]
+
+
+class CameraInput(C.Structure):
+    """Mouse and key input of one frame of the camera controller."""
+    _fields_ = [("dx", f32), ("dy", f32), ("left", i32), ("right", i32), ("middle", i32),
+                ("key_x", i32), ("key_y", i32), ("key_z", i32), ("left_shift", i32)]
+
+
 class RenderStats(C.Structure):
     _fields_ = [(n, u64) for n in ("samples", "steps", "queries", "nodes", "leaves", "particles",
                                    "covering", "exhausted")]
@@ -143,6 +155,11 @@ SIGNATURES = {
     "aarbf_set_diff_estimator": (C.c_int, [P, C.c_int]),
     "aarbf_taa_resolve": (C.c_int, [P, C.POINTER(TAAConstants)]),
     "aarbf_abi_layout": (u32, [C.POINTER(u32), u32]),
+    "aarbf_handle_counts": (C.c_int, [P, C.POINTER(u32)]),
+    "aarbf_light_direction": (C.c_int, [f32, f32, FP]),
+    "aarbf_camera_basis": (C.c_int, [FP, FP, FP, f32, C.c_int, C.c_int, C.POINTER(PushConstants)]),
+    "aarbf_camera_update": (C.c_int, [C.POINTER(CameraState), C.POINTER(CameraInput), C.c_int, C.c_int, f32,
+                                      C.POINTER(PushConstants), C.POINTER(C.c_int)]),
 }
 
 
@@ -158,6 +175,8 @@ def _expected_layout():
         C.sizeof(RenderStats),
         C.sizeof(Request), Request.pc.offset, Request.spp.offset,
         C.sizeof(TuneResult), TuneResult.walk_budget.offset,
+        C.sizeof(CameraState), CameraState.firstFrame.offset,
+        C.sizeof(CameraInput), CameraInput.left_shift.offset,
     ]
 
 

@@ -15,6 +15,8 @@ over its threshold (primary ray, ambient 1), see include/aarbf.h.
 """
 from __future__ import annotations
 
+import weakref
+
 import numpy as np
 
 
@@ -68,6 +70,10 @@ class _Render:
         rg.fbSize = Int2(W, H)
         rg.globalAABBMin, rg.globalAABBMax = _f3(self.aabb[:3]), _f3(self.aabb[3:])
         self.rg = rg
+        # the BVH and the tables live in the context until closed: a forward
+        # that never reaches backward (no_grad, validation, an output left
+        # out of the loss) releases them when this object is collected
+        self._finalizer = weakref.finalize(self, _Render._release, ctx, self.bvh, list(self.tfs))
 
     def forward(self):
         self.ctx.set_diff_estimator(self.estimator)
@@ -99,10 +105,17 @@ class _Render:
         grm4[:, 0] = grm
         return inv(gp.view(n, 3)), inv(ga), inv(gr), gcm.view(-1, 4), gdm4, grm4
 
+    @staticmethod
+    def _release(ctx, bvh, tfs):
+        if getattr(ctx, "h", None) is None:  # the context is gone, and its handles with it
+            return
+        ctx.bvh_destroy(bvh)
+        for h in tfs:
+            ctx.tf_destroy(h)
+
     def close(self):
-        self.ctx.bvh_destroy(self.bvh)
-        for h in self.tfs:
-            self.ctx.tf_destroy(h)
+        """Releases the BVH and tables now; later calls do nothing."""
+        self._finalizer()
 
 
 class _Fn:
@@ -113,19 +126,20 @@ class _Fn:
         class ParticleRender(torch.autograd.Function):
             @staticmethod
             def forward(fctx, settings, positions, attributes, radii, colormap, densitymap, radiusmap):
-                r = _Render(*settings[:-1], positions, attributes, radii, colormap, densitymap, radiusmap,
-                            estimator=settings[-1])
+                *args, estimator, keep = settings
+                r = _Render(*args, positions, attributes, radii, colormap, densitymap, radiusmap,
+                            estimator=estimator)
                 img = r.forward()
-                fctx.r = r
+                if keep:
+                    fctx.r = r  # released with the graph (several backwards with retain_graph are fine)
+                else:
+                    r.close()  # no backward can follow
                 return img
 
             @staticmethod
             def backward(fctx, grad):
                 r = fctx.r
-                try:
-                    gp, ga, gr, gcm, gdm, grm = r.backward(grad.float().contiguous())
-                finally:
-                    r.close()
+                gp, ga, gr, gcm, gdm, grm = r.backward(grad.float().contiguous())
                 return None, gp, ga, gr, gcm, gdm, grm
 
         return ParticleRender
@@ -140,4 +154,7 @@ class DiffRenderer:
         self.fn = _Fn.build()
 
     def __call__(self, positions, attributes, radii, colormap, densitymap, radiusmap):
-        return self.fn.apply(self.settings, positions, attributes, radii, colormap, densitymap, radiusmap)
+        torch = _torch()
+        inputs = (positions, attributes, radii, colormap, densitymap, radiusmap)
+        keep = torch.is_grad_enabled() and any(t.requires_grad for t in inputs)
+        return self.fn.apply(self.settings + (keep,), *inputs)

@@ -2,8 +2,9 @@
 
 The leaf size (particles per leaf) is a property of the scene: each
 candidate rebuilds the bounds and the BVH (Scene.prepare) and runs the
-schedule autotuner (aarbf_autotune: step budget, occupancy, graph rounds)
-on the views; the fastest (leaf size, schedule) is returned with its scene,
+schedule autotuner (aarbf_autotune: step budget, occupancy, graph rounds,
+gather walk budget) on the views, each leaf size from the same starting
+schedule; the fastest (leaf size, schedule) is returned with its scene,
 and the context keeps that schedule.
 """
 from __future__ import annotations
@@ -27,6 +28,7 @@ def tune_leaf_size(ctx, xyzw, colormap, radiusmap, densitymap, rbf_radius: float
         ctx.set_tuning(ctx.TUNE_STEP_CAP, 4096)
         ctx.set_tuning(ctx.TUNE_STEP_WAVES, 0)
         ctx.set_tuning(ctx.TUNE_GRAPH_ROUNDS, 0)
+        ctx.set_tuning(ctx.TUNE_WALK_BUDGET, 256)
         r = ctx.autotune(reqs, reps=reps)
         per[ppl] = r["best_ms"]
         settings[ppl] = r
@@ -37,5 +39,7 @@ def tune_leaf_size(ctx, xyzw, colormap, radiusmap, densitymap, rbf_radius: float
     ctx.set_tuning(ctx.TUNE_STEP_CAP, r["step_cap"])
     ctx.set_tuning(ctx.TUNE_STEP_WAVES, r["step_waves"])
     ctx.set_tuning(ctx.TUNE_GRAPH_ROUNDS, r["graph_rounds"])
+    ctx.set_tuning(ctx.TUNE_WALK_BUDGET, r["walk_budget"])
     return sc, {"particles_per_leaf": ppl, "step_cap": r["step_cap"], "step_waves": r["step_waves"],
-                "graph_rounds": r["graph_rounds"], "best_ms": r["best_ms"], "per_leaf_size": per}
+                "graph_rounds": r["graph_rounds"], "walk_budget": r["walk_budget"], "best_ms": r["best_ms"],
+                "per_leaf_size": per}

@@ -342,6 +342,14 @@ int aarbf_stbn_destroy(aarbf_context *ctx, uint64_t handle) {
 }
 
 // Layout probe used by the Python bindings to check their struct mirrors.
+int aarbf_handle_counts(const aarbf_context *ctx, uint32_t counts[3]) {
+  if (!ctx || !counts) return AARBF_ERR_INVALID_ARGUMENT;
+  counts[0] = (uint32_t)ctx->tfs.size();
+  counts[1] = (uint32_t)ctx->stbns.size();
+  counts[2] = (uint32_t)ctx->bvhs.size();
+  return AARBF_OK;
+}
+
 uint32_t aarbf_abi_layout(uint32_t *out, uint32_t n) {
   uint32_t v[] = {
       (uint32_t)sizeof(aarbf_push_constants),
@@ -366,6 +374,10 @@ uint32_t aarbf_abi_layout(uint32_t *out, uint32_t n) {
       (uint32_t)offsetof(aarbf_request, spp),
       (uint32_t)sizeof(aarbf_tune_result),
       (uint32_t)offsetof(aarbf_tune_result, walk_budget),
+      (uint32_t)sizeof(aarbf_camera_state),
+      (uint32_t)offsetof(aarbf_camera_state, firstFrame),
+      (uint32_t)sizeof(aarbf_camera_input),
+      (uint32_t)offsetof(aarbf_camera_input, left_shift),
   };
   uint32_t m = (uint32_t)(sizeof(v) / sizeof(v[0]));
   for (uint32_t i = 0; i < n && i < m; ++i) out[i] = v[i];

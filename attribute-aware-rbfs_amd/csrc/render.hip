@@ -312,11 +312,12 @@ __global__ __launch_bounds__(256) void render_kernel(RenderArgs A) {
 
 using namespace aarbf;
 
-// getLightDirection (sharedCode.h:127-132), evaluated once on the host.
+// getLightDirection (sharedCode.h:127-132), evaluated once on the host
+// (camera.cpp).
 static float3 light_direction(float azimuth, float elevation) {
-  float a = elevation * 3.14f - 0.5f * 3.14f;
-  float b = azimuth * 2.0f * 3.14f;
-  return make_float3(sinf(a) * cosf(b), sinf(a) * sinf(b), cosf(a));
+  float l[3];
+  aarbf_light_direction(azimuth, elevation, l);
+  return make_float3(l[0], l[1], l[2]);
 }
 
 // Per-view fields of a render call (camera, counters, light, buffers).

@@ -6,6 +6,7 @@
 // exact texel centres, so the reference's linear sampler returns texels
 // unchanged; borders clamp to the edge.  Each pixel reads only its own
 // history texel, so the history copy is done in the same pass.
+#include "aarbf_detmath.h"
 #include "internal.h"
 
 namespace aarbf {
@@ -30,9 +31,12 @@ __device__ inline float3 ycocg2rgb(float3 c) {
 __device__ inline float3 fmin3(float3 a, float3 b) { return make_float3(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)); }
 __device__ inline float3 fmax3(float3 a, float3 b) { return make_float3(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); }
 
+// sRGB transfer function of the GUI layer's R8G8B8A8_SRGB texels, with the
+// renderer's deterministic power function (host restatements get the same
+// bits).
 __device__ inline float srgb_decode(uint32_t v) {
   float c = (float)v / 255.0f;
-  return c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f);
+  return c <= 0.04045f ? c / 12.92f : aa_powf((c + 0.055f) / 1.055f, 2.4f);
 }
 
 __device__ inline uint32_t to8(float v) {

@@ -192,6 +192,10 @@ int aarbf_render_histogram(aarbf_context *ctx, uint64_t *out, int reset);
  * set-up / fold kernels (3).  No counterpart in the reference. */
 int aarbf_set_timing(aarbf_context *ctx, int enable);
 int aarbf_read_timing(aarbf_context *ctx, double *ms, uint64_t *launches, int reset);
+/* Live handles on the context: counts[0] transfer functions, counts[1]
+ * blue-noise atlases, counts[2] BVHs (for leak checks).  No counterpart in
+ * the reference. */
+int aarbf_handle_counts(const aarbf_context *ctx, uint32_t counts[3]);
 /* Writes sizes and offsets of the structs above into out[0..n) and returns
  * how many there are; bindings in other languages use it to check their
  * struct mirrors.  No counterpart in the reference. */
@@ -385,8 +389,53 @@ int aarbf_diff_backward(aarbf_context *ctx, const aarbf_raygen_data *rg, const a
  * sample's colour.  No counterpart in the reference. */
 int aarbf_set_diff_estimator(aarbf_context *ctx, int estimator);
 /* CompositeGui's TAA resolve and BGRA8 packing (deviceCodeCommon.hlsl:67-115)
- * plus the taa -> taaPrev history copy (hostCode.cpp:819-830). */
+ * plus the taa -> taaPrev history copy (hostCode.cpp:819-830).  The GUI
+ * layer (guiTexture, R8G8B8A8_SRGB texels, hostCode.cpp:328-330) is decoded
+ * with the sRGB transfer function and composited with over() on top of the
+ * pixel; frameBuffer receives B | G << 8 | R << 16 | A << 24, each channel
+ * clamped to [0, 1] and rounded as (uint)(v * 255 + 0.5). */
 int aarbf_taa_resolve(aarbf_context *ctx, const aarbf_taa_constants *tc);
+
+/* ---- host helpers of the reference's render loop (no device work) ------- */
+
+/* getLightDirection (sharedCode.h:127-132) evaluated on the host in
+ * binary32, as the renderer uses it: out = (sin(a) cos(b), sin(a) sin(b),
+ * cos(a)), a = elevation * 3.14 - 0.5 * 3.14, b = azimuth * 2 * 3.14. */
+int aarbf_light_direction(float azimuth, float elevation, float out[3]);
+/* The camera basis the reference writes into rtConstants.camera
+ * (hostCode.cpp:669-682, 712-719): pos = look_from,
+ * d00 = normalize(look_at - look_from), du = cos_fovy * (width / height) *
+ * normalize(cross(d00, look_up)), dv = cos_fovy * normalize(cross(du, d00)),
+ * then d00 -= du / 2 and d00 -= dv / 2.  cos_fovy is the reference's
+ * image-plane scale (2 tan(fovy / 2) for a vertical field of view fovy).
+ * Writes pc->camera only. */
+int aarbf_camera_basis(const float look_from[3], const float look_at[3], const float look_up[3],
+                       float cos_fovy, int width, int height, aarbf_push_constants *pc);
+/* The reference's interactive camera (hostCode.cpp:253-271 state,
+ * 593-731 per frame). */
+typedef struct aarbf_camera_state {
+  aarbf_float3 lookFrom, lookAt, lookUp;
+  float cosFovy;
+  int32_t firstFrame; /* set to 1 before the first frame */
+} aarbf_camera_state;
+typedef struct aarbf_camera_input {
+  float dx, dy;                 /* cursor movement since the last frame, pixels */
+  int32_t left, right, middle;  /* mouse buttons held and not captured by a GUI */
+  int32_t key_x, key_y, key_z;  /* up-axis keys */
+  int32_t left_shift;           /* negates the up axis a key selects */
+} aarbf_camera_input;
+/* One frame of the controller: an X / Y / Z key sets the up axis (negated
+ * with left shift); the left button (or an up-axis key, or the first frame)
+ * orbits look_from about look_at by -dx * 2 pi / width around the up axis
+ * and -dy * pi / height around cross(up, normalize(look_at - position));
+ * the right button zooms (look_from - look_at scaled by 0.95 for dy > 0,
+ * 1.05 for dy < 0; only pc->camera.pos changes); the middle button pans
+ * look_from and look_at by (right * dx - up * dy) * 0.001 * diagonal.  The
+ * camera basis in pc is recomputed as the reference recomputes it, and
+ * pc->accumID is reset to 1 when the camera moved; *changed (may be NULL)
+ * tells whether it did. */
+int aarbf_camera_update(aarbf_camera_state *st, const aarbf_camera_input *in, int width, int height,
+                        float diagonal, aarbf_push_constants *pc, int *changed);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,7 @@ benchmark's CPU-baseline leg, never by the product package.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -53,42 +54,63 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        L.orc_hilbert_c2i.restype = C.c_uint64
-        L.orc_hilbert_c2i.argtypes = [u64p]
-        L.orc_hilbert_i2c.argtypes = [C.c_uint64, u64p]
-        L.orc_hilbert_codes.argtypes = [f32p, C.c_uint64, f32p, u64p]
-        L.orc_spacetime_aabb.argtypes = [f32p, C.c_uint64, f32p]
-        L.orc_sort_particles.argtypes = [f32p, u64p, C.c_uint64, u32p]
-        L.orc_attribute_range.argtypes = [f32p, C.c_uint64, f32p]
-        L.orc_normalize_attributes.argtypes = [f32p, C.c_uint64, f32p]
-        L.orc_tf_sample.argtypes = [f32p, C.c_uint32, C.c_float, f32p]
-        L.orc_gen_bounds.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_float, f32p, C.c_uint32, f32p, f32p,
-                                     C.c_uint32]
-        L.orc_bvh_num_levels.restype = C.c_uint32
-        L.orc_bvh_num_levels.argtypes = [C.c_uint32, C.c_uint32]
-        L.orc_bvh_build_level.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_uint32, f32p]
-        L.orc_render.restype = C.c_int
-        L.orc_render.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int32), C.c_uint64, C.c_uint32, f32p, f32p,
-                                 C.c_int, u64p]
-        L.orc_scene_new.restype = C.c_void_p
-        L.orc_scene_new.argtypes = [C.POINTER(OrcParams)]
-        L.orc_scene_render.restype = C.c_int
-        L.orc_scene_render.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_uint64, C.c_uint32, f32p, f32p,
-                                       C.c_int, u64p]
-        L.orc_scene_free.argtypes = [C.c_void_p]
-        L.orc_point_queries.restype = C.c_int
-        L.orc_point_queries.argtypes = [C.POINTER(OrcParams), f32p, C.c_uint64, f32p]
-        L.orc_light_direction.argtypes = [C.c_float, C.c_float, f32p]
-        L.orc_taa_resolve.argtypes = [f32p, f32p, f32p, C.c_int, C.c_int, C.c_int]
-        L.orc_expected.restype = C.c_int
-        L.orc_expected.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int32), C.c_uint64,
-                                   C.POINTER(C.c_double), C.c_int]
-        L.orc_splat.restype = C.c_int
-        L.orc_splat.argtypes = [C.POINTER(OrcParams), f32p, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32),
-                                C.c_uint64, f32p]
-        _lib = L
+        _lib = _bind(C.CDLL(LIB))
     return _lib
+
+
+@contextlib.contextmanager
+def using(path: str):
+    """Routes every wrapper of this module through another build of
+    oracle.c (the tests build copies with a planted mistake to show that a
+    check catches it), then restores the normal build."""
+    global _lib
+    saved = _lib
+    _lib = _bind(C.CDLL(path))
+    try:
+        yield _lib
+    finally:
+        _lib = saved
+
+
+def _bind(L: C.CDLL) -> C.CDLL:
+    L.orc_hilbert_c2i.restype = C.c_uint64
+    L.orc_hilbert_c2i.argtypes = [u64p]
+    L.orc_hilbert_i2c.argtypes = [C.c_uint64, u64p]
+    L.orc_hilbert_codes.argtypes = [f32p, C.c_uint64, f32p, u64p]
+    L.orc_spacetime_aabb.argtypes = [f32p, C.c_uint64, f32p]
+    L.orc_sort_particles.argtypes = [f32p, u64p, C.c_uint64, u32p]
+    L.orc_attribute_range.argtypes = [f32p, C.c_uint64, f32p]
+    L.orc_normalize_attributes.argtypes = [f32p, C.c_uint64, f32p]
+    L.orc_tf_sample.argtypes = [f32p, C.c_uint32, C.c_float, f32p]
+    L.orc_gen_bounds.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_float, f32p, C.c_uint32, f32p, f32p,
+                                 C.c_uint32]
+    L.orc_bvh_num_levels.restype = C.c_uint32
+    L.orc_bvh_num_levels.argtypes = [C.c_uint32, C.c_uint32]
+    L.orc_bvh_build_level.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_uint32, f32p]
+    L.orc_render.restype = C.c_int
+    L.orc_render.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int32), C.c_uint64, C.c_uint32, f32p, f32p,
+                             C.c_int, u64p]
+    L.orc_scene_new.restype = C.c_void_p
+    L.orc_scene_new.argtypes = [C.POINTER(OrcParams)]
+    L.orc_scene_render.restype = C.c_int
+    L.orc_scene_render.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_uint64, C.c_uint32, f32p, f32p,
+                                   C.c_int, u64p]
+    L.orc_scene_free.argtypes = [C.c_void_p]
+    L.orc_point_queries.restype = C.c_int
+    L.orc_point_queries.argtypes = [C.POINTER(OrcParams), f32p, C.c_uint64, f32p]
+    L.orc_light_direction.argtypes = [C.c_float, C.c_float, f32p]
+    L.orc_taa_resolve.argtypes = [f32p, f32p, f32p, C.c_int, C.c_int, C.c_int, u32p, u32p]
+    L.orc_shadow_march.restype = C.c_int
+    L.orc_shadow_march.argtypes = [C.POINTER(OrcParams), f32p, C.c_float, f32p]
+    L.orc_expected.restype = C.c_int
+    L.orc_expected.argtypes = [C.POINTER(OrcParams), C.POINTER(C.c_int32), C.c_uint64,
+                               C.POINTER(C.c_double), C.c_int]
+    L.orc_splat.restype = C.c_int
+    L.orc_splat.argtypes = [C.POINTER(OrcParams), f32p, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32),
+                            C.c_uint64, f32p]
+    L.orc_rng_stream.restype = C.c_uint32
+    L.orc_rng_stream.argtypes = [C.c_uint32] * 4 + [f32p, C.c_uint32]
+    return L
 
 
 def ref_lib():
@@ -205,6 +227,13 @@ def bvh_levels(aabbs: np.ndarray, K: int):
         levels.append(nxt)
         n = m
     return levels
+
+
+def rng_stream(frame: int, px: int, py: int, width: int, k: int):
+    """get_rng(frameID, pixel, width) and its first k lcg_randomf draws."""
+    d = np.empty(max(1, k), np.float32)
+    s = lib().orc_rng_stream(frame, px, py, width, _f(d), k)
+    return int(s), d[:k]
 
 
 def light_direction(azimuth: float, elevation: float) -> np.ndarray:
@@ -333,6 +362,15 @@ class Render:
             raise RuntimeError("orc_splat failed")
         return image
 
+    def shadow_visibility(self, origin, t0: float) -> float:
+        """Visibility of the marching tracker's NEE shadow ray from `origin`
+        toward the light, started at t = t0."""
+        o = np.ascontiguousarray(np.asarray(origin, np.float32).reshape(3))
+        v = C.c_float()
+        if lib().orc_shadow_march(C.byref(self.p), _f(o), C.c_float(t0), C.byref(v)) != 0:
+            raise RuntimeError("orc_shadow_march failed")
+        return float(v.value)
+
     def point_queries(self, pts: np.ndarray) -> np.ndarray:
         q = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
         out = np.empty((q.shape[0], 5), dtype=np.float32)
@@ -341,10 +379,21 @@ class Render:
         return out
 
 
-def taa_resolve(image: np.ndarray, taa: np.ndarray, taa_prev: np.ndarray, disable: bool = False):
+def taa_resolve(image: np.ndarray, taa: np.ndarray, taa_prev: np.ndarray, disable: bool = False, gui=None,
+                frame=None):
+    """CompositeGui + the history copy: updates taa and taa_prev in place;
+    `gui` (uint32 RGBA8 words [H, W]) and `frame` (uint32 [H, W], receives
+    the BGRA8 words) are optional."""
     H, W = image.shape[:2]
     im = np.ascontiguousarray(image, np.float32)
-    lib().orc_taa_resolve(_f(im), _f(taa), _f(taa_prev), W, H, int(disable))
+    for a in (taa, taa_prev):
+        assert a.dtype == np.float32 and a.flags.c_contiguous and a.shape == im.shape
+    g = None if gui is None else np.ascontiguousarray(gui, np.uint32).reshape(H, W)
+    if frame is not None:
+        assert frame.dtype == np.uint32 and frame.flags.c_contiguous and frame.size == W * H
+    lib().orc_taa_resolve(_f(im), _f(taa), _f(taa_prev), W, H, int(disable),
+                          g.ctypes.data_as(u32p) if g is not None else None,
+                          frame.ctypes.data_as(u32p) if frame is not None else None)
 
 
 # ---- differentiable renderer (adjoint.c) -----------------------------------------------

@@ -1414,8 +1414,26 @@ static void ycocg2rgb(const float c[3], float o[3]) {
   o[2] = b;
 }
 
+/* sRGB electro-optical transfer function of an R8G8B8A8_SRGB texel
+ * channel (the GUI layer, hostCode.cpp:328-330). */
+static float srgb_decode8(uint32_t v) {
+  float c = (float)v / 255.0f;
+  return c <= 0.04045f ? c / 12.92f : aa_powf((c + 0.055f) / 1.055f, 2.4f);
+}
+
+static uint32_t to_unorm8(float v) {
+  v = fminf(fmaxf(v, 0.0f), 1.0f); /* NaN -> 0 */
+  return (uint32_t)(v * 255.0f + 0.5f);
+}
+
+/* CompositeGui (deviceCodeCommon.hlsl:67-115) for every pixel: the TAA
+ * resolve into taa[] (unless disabled), the GUI layer composited on top
+ * with over(), and the BGRA8 frame-buffer word (B | G<<8 | R<<16 | A<<24,
+ * channels clamped to [0, 1] and rounded half up); then the host's history
+ * copy taa -> taa_prev (hostCode.cpp:829-830).  gui (RGBA8 words, sRGB
+ * RGB) and frame may be NULL. */
 ORC_EXPORT void orc_taa_resolve(const float *image, float *taa, float *taa_prev, int W, int H,
-                                int disable) {
+                                int disable, const uint32_t *gui, uint32_t *frame) {
   for (int py = 0; py < H; ++py)
     for (int px = 0; px < W; ++px) {
       uint64_t ofs = (uint64_t)px + (uint64_t)W * py;
@@ -1446,11 +1464,54 @@ ORC_EXPORT void orc_taa_resolve(const float *image, float *taa, float *taa_prev,
         rgb2ycocg(old, y);
         for (int j = 0; j < 3; ++j) y[j] = fminf(y[j], mx[j]);
         ycocg2rgb(y, old);
+        /* lerp(old, center, .2f) */
         for (int j = 0; j < 4; ++j) out[j] = old[j] + 0.2f * (center[j] - old[j]);
         memcpy(taa + 4 * ofs, out, 16);
-        memcpy(taa_prev + 4 * ofs, out, 16);
+      } else {
+        memcpy(out, center, 16);
+      }
+      if (frame) {
+        float g[4] = {0.0f, 0.0f, 0.0f, 0.0f}, c[4];
+        if (gui) {
+          uint32_t v = gui[ofs];
+          g[0] = srgb_decode8(v & 255u);
+          g[1] = srgb_decode8((v >> 8) & 255u);
+          g[2] = srgb_decode8((v >> 16) & 255u);
+          g[3] = (float)(v >> 24) / 255.0f;
+        }
+        over(g, out, c);
+        frame[ofs] = to_unorm8(c[2]) | (to_unorm8(c[1]) << 8) | (to_unorm8(c[0]) << 16) |
+                     (to_unorm8(c[3]) << 24);
       }
     }
+  /* the host's history copy, after the pass */
+  memcpy(taa_prev, taa, 16 * (size_t)W * (size_t)H);
+}
+
+/* The shadow ray of ParticleRBFRayGen's next-event estimation under
+ * stochastic marching (deviceCodeRBF.hlsl:287-312 with the tracker's
+ * shadow branch, :131-137), from `origin` toward the light, starting at
+ * t = t0 (the raygen passes -unit * random.z) with a fresh step budget.
+ * Writes the visibility 1 - albedo.w; returns 0, or -1 when out of memory. */
+ORC_EXPORT int orc_shadow_march(const orc_params *p, const float origin[3], float t0, float *visibility) {
+  orc_scene s;
+  if (scene_init(&s, p) != 0) return -1;
+  float lb[3], rt[3], ste, stx;
+  for (int d = 0; d < 3; ++d) {
+    rt[d] = p->aabb_max[d] + 2.0f * p->rbfRadius;
+    lb[d] = p->aabb_min[d] - 2.0f * p->rbfRadius;
+  }
+  aabb_intersection(origin, s.light, 0.0f, 10000.0f, lb, rt, &ste, &stx);
+  orc_tracker tr;
+  memset(&tr, 0, sizeof(tr));
+  tr.max_depth = p->max_depth ? p->max_depth : ORC_MAX_DEPTH;
+  tr.marching = 1;
+  tr.shadow = 1;
+  tr.t = t0;
+  track(&s, &tr, origin, s.light, stx);
+  scene_free(&s);
+  *visibility = 1.0f - tr.albedo[3];
+  return 0;
 }
 
 /* Deterministic math exposed for its accuracy tests: which = 0 log, 1 exp,

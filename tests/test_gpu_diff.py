@@ -112,3 +112,37 @@ def test_diff_backward_cost(gpu_ctx, estimator):
     img = R(*args)
     img.sum().backward()
     assert all(a.grad is not None and torch.isfinite(a.grad).all() for a in args)
+
+
+def test_diff_renderer_releases_its_handles(gpu_ctx):
+    """Each forward builds a BVH and three tables on the context: they are
+    released right away when no backward can follow (no_grad, no input
+    needing a gradient), after backward otherwise, and when an output is
+    dropped without a backward."""
+    import gc
+    import torch
+    W = H = 12
+    pos, attr, radii, cm, dm, rm = _scene(n=60)
+    cam = default_camera(W, H, look_from=(0.2, 0.1, 1.3))
+    R = DiffRenderer(gpu_ctx, cam, W, H, 0.01, spp=1, rbf_radius=0.2)
+    t = lambda a, g: torch.tensor(a, device="cuda", requires_grad=g)
+    base = gpu_ctx.handle_counts()
+    with torch.no_grad():
+        R(*(t(a, True) for a in (pos, attr, radii, cm, dm, rm)))
+    assert gpu_ctx.handle_counts() == base, "forward under no_grad"
+    R(*(t(a, False) for a in (pos, attr, radii, cm, dm, rm)))
+    assert gpu_ctx.handle_counts() == base, "forward without inputs that need gradients"
+    P = t(pos, True)
+    img = R(P, *(t(a, False) for a in (attr, radii, cm, dm, rm)))
+    assert gpu_ctx.handle_counts()["bvh"] == base["bvh"] + 1
+    img.sum().backward(retain_graph=True)
+    g1 = P.grad.clone()
+    img.sum().backward()  # a second backward through the retained graph
+    assert torch.allclose(P.grad, 2 * g1)
+    del img
+    gc.collect()
+    assert gpu_ctx.handle_counts() == base, "after backward, graph freed"
+    img = R(P, *(t(a, False) for a in (attr, radii, cm, dm, rm)))
+    del img
+    gc.collect()
+    assert gpu_ctx.handle_counts() == base, "output dropped without backward"

@@ -156,28 +156,42 @@ def test_stats_count_samples(gpu_ctx):
     assert st["steps"] > 0 and st["nodes"] >= st["queries"] > 0 and st["covering"] <= st["particles"]
 
 
-def test_taa_matches_oracle(gpu_ctx):
+@pytest.mark.parametrize("gui_layer,disable", [(False, False), (True, False), (True, True)])
+def test_taa_matches_oracle(gpu_ctx, gui_layer, disable):
+    """CompositeGui: TAA resolve, history copy, the GUI layer composited with
+    over() and the BGRA8 frame-buffer words, all compared bit for bit."""
     torch = _torch()
     rng = np.random.default_rng(0)
     H, W = 37, 53
     image = rng.uniform(0, 1, (H, W, 4)).astype(np.float32)
+    image[rng.uniform(size=(H, W)) < 0.2, 3] = 1.0
+    image[0, :3] = [[-0.25, 1.5, 0.5, 1.0], [0.2, 2.0, 0.7, 0.5], [0.1, 0.2, 0.3, 0.0]]
     prev = rng.uniform(-0.5, 1.5, (H, W, 4)).astype(np.float32)
-    taa = np.zeros_like(image)
+    taa = rng.uniform(0, 1, (H, W, 4)).astype(np.float32)
+    gui = rng.integers(0, 2 ** 32, (H, W), dtype=np.uint64).astype(np.uint32) if gui_layer else None
+    if gui is not None:
+        gui[:4] &= 0x00FFFFFF  # transparent texels
+        gui[4:8] |= 0xFF000000  # opaque texels
     otaa, oprev = taa.copy(), prev.copy()
-    oracle.taa_resolve(image, otaa, oprev)
+    oframe = np.zeros((H, W), np.uint32)
+    oracle.taa_resolve(image, otaa, oprev, disable=disable, gui=gui, frame=oframe)
     d_img = torch.from_numpy(image).cuda()
-    d_taa = torch.zeros_like(d_img)
+    d_taa = torch.from_numpy(taa).cuda()
     d_prev = torch.from_numpy(prev).cuda()
     d_frame = torch.zeros((H, W), dtype=torch.int32, device="cuda")
-    tc = aarbf.TAAConstants(0, 0, aarbf._lib.Int2(W, H), d_img.data_ptr(), 0, d_frame.data_ptr(),
+    d_gui = torch.from_numpy(gui.view(np.int32)).cuda() if gui is not None else None
+    tc = aarbf.TAAConstants(1 if disable else 0, 0, aarbf._lib.Int2(W, H), d_img.data_ptr(),
+                            d_gui.data_ptr() if d_gui is not None else 0, d_frame.data_ptr(),
                             d_taa.data_ptr(), d_prev.data_ptr())
     gpu_ctx.taa_resolve(tc)
     gpu_ctx.sync()
-    _compare(d_taa.cpu().numpy(), otaa, "taa")
-    assert np.array_equal(d_prev.cpu().numpy(), oprev)
+    got_taa = d_taa.cpu().numpy()
+    assert np.array_equal(got_taa, otaa, equal_nan=True), "taa"
+    assert np.array_equal(d_prev.cpu().numpy(), oprev, equal_nan=True), "history copy"
     f = d_frame.cpu().numpy().view(np.uint32)
-    a8 = (np.clip(otaa[..., 3], 0, 1) * np.float32(255) + np.float32(0.5)).astype(np.uint32)
-    assert np.array_equal(f >> 24, a8)  # no GUI layer: over() keeps the pixel
+    bad = np.argwhere(f != oframe)
+    assert bad.size == 0, f"frame words differ at {bad[:3].tolist()}: {f[tuple(bad[0])]:08x} vs " \
+                          f"{oframe[tuple(bad[0])]:08x}"
 
 
 @pytest.mark.parametrize("blue_noise", [False, True])
