@@ -365,6 +365,16 @@ class Context:
         self.check(self.lib.aarbf_render_rbf_spp(self.h, C.byref(rg), C.byref(pc), width, height, spp),
                    "aarbf_render_rbf_spp")
 
+    NOALLOC_MAX_PARTICLES = 1048576
+
+    @_ordered
+    def render_noalloc(self, rg: RayGenData, pc: PushConstants, width: int, height: int, spp: int = 1):
+        """aarbf_render_rbf_spp_noalloc: the same image as render(), no device
+        allocation (one-lane-per-pixel schedule, payload in a static per-device
+        array of NOALLOC_MAX_PARTICLES particles)."""
+        self.check(self.lib.aarbf_render_rbf_spp_noalloc(self.h, C.byref(rg), C.byref(pc), width, height, spp),
+                   "aarbf_render_rbf_spp_noalloc")
+
     @_ordered
     def render_batch(self, requests):
         """Renders views of one scene in one schedule (aarbf_render_batch);

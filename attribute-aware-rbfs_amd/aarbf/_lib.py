@@ -149,6 +149,8 @@ SIGNATURES = {
                                        C.c_int, u32]),
     "aarbf_render_splat": (C.c_int, [P, C.POINTER(RayGenData), C.POINTER(PushConstants), C.c_int,
                                      C.c_int]),
+    "aarbf_render_rbf_spp_noalloc": (C.c_int, [P, C.POINTER(RayGenData), C.POINTER(PushConstants), C.c_int,
+                                               C.c_int, u32]),
     "aarbf_render_batch": (C.c_int, [P, C.POINTER(Request), u32]),
     "aarbf_batch_latency": (C.c_int, [P, C.POINTER(C.c_double), u32]),
     "aarbf_autotune": (C.c_int, [P, C.POINTER(Request), u32, u32, C.POINTER(TuneResult)]),

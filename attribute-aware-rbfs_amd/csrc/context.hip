@@ -93,8 +93,11 @@ int aarbf_context_create(int device, aarbf_context **out) {
       hipMalloc(&ctx->d_stats, sizeof(uint64_t) * 32) != hipSuccess ||
       hipMalloc(&ctx->d_counter, sizeof(uint32_t) * 4) != hipSuccess ||
       hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
-      hipMemset(ctx->d_stats, 0, sizeof(uint64_t) * 32) != hipSuccess) {
+      hipMemset(ctx->d_stats, 0, sizeof(uint64_t) * 32) != hipSuccess ||
+      aarbf::noalloc_init(device) != AARBF_OK) {
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
+    if (ctx->d_stats) hipFree(ctx->d_stats);
+    if (ctx->d_counter) hipFree(ctx->d_counter);
     delete ctx;
     return AARBF_ERR_HIP;
   }

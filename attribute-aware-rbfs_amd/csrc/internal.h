@@ -114,6 +114,9 @@ TF *get_tf(aarbf_context *ctx, uint64_t h);
 STBN *get_stbn(aarbf_context *ctx, uint64_t h);
 BVH *get_bvh(aarbf_context *ctx, uint64_t h);
 BVHView make_view(const BVH &b);
+// Takes the current device's address of the allocation-free render entry
+// point's scratch array (render.hip); called once per device.
+int noalloc_init(int device);
 
 } // namespace aarbf
 

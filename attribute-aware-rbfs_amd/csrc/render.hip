@@ -16,6 +16,7 @@
 #include "render_common.h"
 
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace aarbf {
@@ -347,10 +348,25 @@ static ReqArgs view_args(const aarbf_raygen_data *rg, const aarbf_push_constants
   return R;
 }
 
+// Size of the baked payload of n particles in float4 units: FP32 32 B per
+// particle, FP16 16 B per particle then 32 B per chunk, whichever format is
+// larger, followed by the RBF coefficients (one float per particle).
+static size_t payload_float4s(uint32_t n, size_t *pm_off) {
+  size_t need = 2 * (size_t)(n ? n : 1);
+  const size_t need16 = (size_t)(n ? n : 1) + 2 * (size_t)((n + 31u) >> kQChunkLog2);
+  if (need16 > need) need = need16;
+  *pm_off = need;
+  return need + ((size_t)(n ? n : 1) + 3) / 4;
+}
+
 // Validates a render call, bakes the particle payload and fills the
 // kernel arguments.  *stbn_out: the call's blue-noise atlas (or null).
+// fixed_payload: a device buffer of at least payload_float4s(n) float4 for
+// the baked payload (checked by the caller), or null for the context's
+// grow-only buffer.
 static int render_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const aarbf_push_constants *pc, int W,
-                        int H, uint32_t spp, RenderArgs &A, BVH *&bvh_out, STBN *&stbn_out) {
+                        int H, uint32_t spp, RenderArgs &A, BVH *&bvh_out, STBN *&stbn_out,
+                        float4 *fixed_payload = nullptr) {
   AARBF_REQUIRE(ctx, rg && pc, "aarbf_render_rbf: null struct");
   AARBF_REQUIRE(ctx, W > 0 && H > 0 && spp >= 1, "aarbf_render_rbf: empty launch");
   AARBF_REQUIRE(ctx, rg->imageBuffer && rg->accumBuffer, "aarbf_render_rbf: null image/accum buffer");
@@ -371,27 +387,26 @@ static int render_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const a
   stbn_out = stbn;
   hipSetDevice(ctx->device);
   uint32_t n = pc->numParticles;
-  // FP32: 32 B per particle; FP16: 16 B per particle, then 32 B per chunk
-  size_t need = 2 * (size_t)(n ? n : 1);
-  const size_t need16 = (size_t)(n ? n : 1) + 2 * (size_t)((n + 31u) >> kQChunkLog2);
-  if (need16 > need) need = need16;
-  // then the RBF coefficients, one float per particle
-  const size_t pm_off = need;
-  need += ((size_t)(n ? n : 1) + 3) / 4;
-  if (ctx->baked_count < need) {
-    if (ctx->baked) {
-      hipStreamSynchronize(ctx->stream);
-      hipFree(ctx->baked);
-      ctx->baked = nullptr;
-      ctx->baked_count = 0;
+  size_t pm_off = 0;
+  const size_t need = payload_float4s(n, &pm_off);
+  float4 *buf = fixed_payload;
+  if (!buf) {
+    if (ctx->baked_count < need) {
+      if (ctx->baked) {
+        hipStreamSynchronize(ctx->stream);
+        hipFree(ctx->baked);
+        ctx->baked = nullptr;
+        ctx->baked_count = 0;
+      }
+      AARBF_HIP(ctx, hipMalloc(&ctx->baked, sizeof(float4) * need));
+      ctx->baked_count = need;
     }
-    AARBF_HIP(ctx, hipMalloc(&ctx->baked, sizeof(float4) * need));
-    ctx->baked_count = need;
+    buf = ctx->baked;
   }
-  float4 *pr = ctx->baked, *pcol = ctx->baked + (n ? n : 1);
-  float *pm = (float *)(ctx->baked + pm_off);
-  uint4 *qp = (uint4 *)ctx->baked;
-  float4 *qc = ctx->baked + (n ? n : 1);
+  float4 *pr = buf, *pcol = buf + (n ? n : 1);
+  float *pm = (float *)(buf + pm_off);
+  uint4 *qp = (uint4 *)buf;
+  float4 *qc = buf + (n ? n : 1);
   const bool q16 = ctx->payload == 1;
   if (n && !q16)
     bake_kernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>((const float4 *)pc->particles, (const float *)pc->radii,
@@ -401,8 +416,10 @@ static int render_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const a
     bake_q<<<(n + 255) / 256, 256, 0, ctx->stream>>>((const float4 *)pc->particles, (const float *)pc->radii, n,
                                                      pc->rbfRadius, rm->table, rm->n, cm->table, cm->n, qc, qp, pm);
   AARBF_CHECK_LAUNCH(ctx, "bake");
-  ctx->baked_fmt = q16 ? 1 : 0;
-  ctx->baked_n = n;
+  if (!fixed_payload) { // what aarbf_payload_read decodes
+    ctx->baked_fmt = q16 ? 1 : 0;
+    ctx->baked_n = n;
+  }
   A = RenderArgs{};
   A.frameID = pc->frameID;
   A.accumID = pc->accumID;
@@ -544,3 +561,65 @@ int aarbf_render_rbf(aarbf_context *ctx, const aarbf_raygen_data *rg, const aarb
 }
 
 } // extern "C"
+
+// ---- the allocation-free render entry point ----------------------------------
+//
+// aarbf_render_rbf_spp_noalloc bakes the particle payload into this
+// file-scope array instead of the context's grow-only buffer, so the call
+// allocates nothing.  Each device holds its own copy of the array (for the
+// life of the process) at its own address; host code uses that address,
+// taken once per device with hipGetSymbolAddress when a context is created
+// on the device, never the array's name, which on the host is only a
+// stand-in for the device symbol.
+static constexpr size_t kNoallocFloat4s =
+    2 * (size_t)AARBF_NOALLOC_MAX_PARTICLES + ((size_t)AARBF_NOALLOC_MAX_PARTICLES + 3) / 4;
+static __device__ __align__(256) float4 noalloc_scratch[kNoallocFloat4s];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_lock;
+float4 *g_noalloc_ptr[kMaxDevices] = {};
+} // namespace
+
+namespace aarbf {
+// Called by aarbf_context_create with `device` current.
+int noalloc_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return AARBF_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> g(g_noalloc_lock);
+  if (g_noalloc_ptr[device]) return AARBF_OK;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)) != hipSuccess || !p) return AARBF_ERR_HIP;
+  g_noalloc_ptr[device] = (float4 *)p;
+  return AARBF_OK;
+}
+} // namespace aarbf
+
+extern "C" int aarbf_render_rbf_spp_noalloc(aarbf_context *ctx, const aarbf_raygen_data *rg,
+                                            const aarbf_push_constants *pc, int W, int H, uint32_t spp) {
+  if (!ctx) return AARBF_ERR_INVALID_ARGUMENT;
+  AARBF_REQUIRE(ctx, pc, "aarbf_render_rbf_spp_noalloc: null struct");
+  AARBF_REQUIRE(ctx, pc->numParticles <= AARBF_NOALLOC_MAX_PARTICLES,
+                "aarbf_render_rbf_spp_noalloc: numParticles exceeds AARBF_NOALLOC_MAX_PARTICLES (1,048,576); "
+                "use aarbf_render_rbf_spp");
+  float4 *buf = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_noalloc_lock);
+    buf = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_noalloc_ptr[ctx->device] : nullptr;
+  }
+  AARBF_REQUIRE(ctx, buf, "aarbf_render_rbf_spp_noalloc: the device's scratch array was not set up");
+  size_t pm_off = 0;
+  AARBF_REQUIRE(ctx, payload_float4s(pc->numParticles, &pm_off) <= kNoallocFloat4s,
+                "aarbf_render_rbf_spp_noalloc: payload larger than the scratch array");
+  RenderArgs A;
+  BVH *bvh = nullptr;
+  STBN *stbn = nullptr;
+  int rc = render_setup(ctx, rg, pc, W, H, spp, A, bvh, stbn, buf);
+  if (rc != AARBF_OK) return rc;
+  dim3 grid((W + 15) / 16, (H + 15) / 16);
+  if (ctx->stats)
+    render_kernel<true><<<grid, 256, 0, ctx->stream>>>(A);
+  else
+    render_kernel<false><<<grid, 256, 0, ctx->stream>>>(A);
+  AARBF_CHECK_LAUNCH(ctx, "render_kernel");
+  return AARBF_OK;
+}

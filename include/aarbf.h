@@ -298,6 +298,22 @@ int aarbf_render_rbf(aarbf_context *ctx, const aarbf_raygen_data *rg,
  * incremented between them (hostCode.cpp:826-827). */
 int aarbf_render_rbf_spp(aarbf_context *ctx, const aarbf_raygen_data *rg,
                          const aarbf_push_constants *pc, int width, int height, uint32_t spp);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc: aarbf_render_rbf_spp on the one-lane-per-pixel
+ * schedule (render mode 1; the same image bit for bit), allocating nothing.
+ * Its only scratch buffer, the particle payload baked from the particles
+ * and transfer functions (36 bytes per particle), lives in a static array
+ * of each device (AARBF_NOALLOC_MAX_PARTICLES particles, 36 MiB, held until
+ * the process ends), set up when a context is created on the device; calls
+ * with more particles are rejected.  All calls on one device share that
+ * array: calls that overlap in time on one device (several contexts,
+ * threads or streams) race and must be serialised by the caller.  The
+ * payload format of aarbf_set_payload applies; aarbf_payload_read does not
+ * see this call's payload. */
+#define AARBF_NOALLOC_MAX_PARTICLES 1048576
+int aarbf_render_rbf_spp_noalloc(aarbf_context *ctx, const aarbf_raygen_data *rg,
+                                 const aarbf_push_constants *pc, int width, int height, uint32_t spp);
 /* Particle payload the renderer bakes from the particles and TFs: 0 (default)
  * FP32, 32 bytes per particle; 1 FP16 leaf-local, 16 bytes per particle --
  * positions and radius as 16-bit fixed point in the frame of their chunk of
