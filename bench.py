@@ -50,13 +50,16 @@ def main() -> int:
     ap.add_argument("--graph-rounds", type=int, default=0, help="rounds per captured hipGraph (0 = host-driven)")
     ap.add_argument("--walk-budget", type=int, default=256,
                     help="nodes a one-lane gather walk opens before its ray goes to a one-wave gather")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline's all-core run (0 = the cores this process may use, at most 16)")
+    ap.add_argument("--prep-67m", type=int, default=1,
+                    help="time pre-processing and the first frame of the 67,108,864-particle precision scene (0 = skip)")
     ap.add_argument("--diff-views", type=int, default=2,
                     help="views of the differentiable-rendering leg timed (0 = skip the leg)")
     ap.add_argument("--splat-frames", type=int, default=2, help="full-size splat frames timed (0 = no splat leg)")
     ap.add_argument("--diff-particles", type=int, default=1048576)
-    ap.add_argument("--batch-views", type=int, default=0,
-                    help="camera requests of the batching leg (0 = skip the leg; the batching workload has 256)")
+    ap.add_argument("--batch-views", type=int, default=256,
+                    help="camera requests of the batching leg (0 = skip the leg)")
     ap.add_argument("--batch-particles", type=int, default=16777216)
     ap.add_argument("--batch-payload", type=int, default=1)
     ap.add_argument("--frame-batch", type=int, default=8,
@@ -184,8 +187,7 @@ def main() -> int:
 
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
-    ctx.set_timing(a.mode == 0)
-    ctx.read_timing(reset=True)
+    ctx.set_timing(False)  # the timed frames carry no per-kernel events
     torch.cuda.synchronize()
     w0 = time.time()
     t_start.record(stream)
@@ -199,13 +201,23 @@ def main() -> int:
     wall = time.time() - w0
     gpu_ms = t_start.elapsed_time(t_end)
     kern_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
-    # per-kernel times of the render rounds, from the library's HIP events
-    # on the same stream (mode 0)
-    ktime = ctx.read_timing(reset=True)
-    ctx.set_timing(False)
-    kernels = {k: {"ms_per_frame": round(ms / a.steps, 3), "launches_per_frame": round(n / a.steps, 2),
-                   "ms_per_launch": round(ms / n, 4) if n else None}
-               for k, (ms, n) in ktime.items()} if a.mode == 0 else None
+    # per-kernel split of the render rounds (mode 0): the library's HIP
+    # events on the same stream, in an untimed pass over the first timed
+    # frames (events and per-round waits make it a little slower)
+    kernels = None
+    if a.mode == 0:
+        nk = min(2, a.steps)
+        ctx.set_timing(True)
+        ctx.read_timing(reset=True)
+        for k in range(nk):
+            frame(a.warmup + k)
+        torch.cuda.synchronize()
+        ktime = ctx.read_timing(reset=True)
+        ctx.set_timing(False)
+        kernels = {k: {"ms_per_frame": round(ms / nk, 3), "launches_per_frame": round(n / nk, 2),
+                       "ms_per_launch": round(ms / n, 4) if n else None}
+                   for k, (ms, n) in ktime.items()}
+        kernels["frames"] = nk
     total_samples = W * H * S * a.steps
     msps = total_samples / (gpu_ms * 1e-3) / 1e6
     achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
@@ -225,6 +237,10 @@ def main() -> int:
             valu = {"kernel": v["kernel"], "busy": v["valu_busy"], "source": v["source"]}
         except Exception:
             valu = None
+
+    prep = preprocess_leg(ctx, stream, wl, sc, cams, prep_s)
+    if a.prep_67m:
+        prep["hierarchical_67m"] = preprocess_67m_leg(ctx, stream)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -259,7 +275,7 @@ def main() -> int:
             "payload": ("fp32", "fp16 leaf-local quantised")[a.payload], "max_depth": aarbf.MAX_DEPTH,
             "exhausted_fraction": st["exhausted"] / samples, "preprocess_ms": round(prep_s * 1e3, 1),
             "refit_ms": round(refit_ms, 2), "rebuild_ms": round(rebuild_ms, 2),
-            "parallelism": "single GPU",
+            "parallelism": "single GPU", "timed_schedule": "host-driven rounds, no per-kernel events",
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -272,6 +288,7 @@ def main() -> int:
             "valu": valu,
         },
         "cpu_baseline": cpu,
+        "preprocess": prep,
         "frame_batch": fbatch,
         "splat": splat,
         "diff": diff,
@@ -532,30 +549,194 @@ def copy_calibration(nbytes: int) -> int:
 
 
 def cpu_baseline(sc, wl, cam, threads):
-    """The oracle (scalar C restatement of the reference shaders, std::thread
-    -style pthreads over pixels) on frame 0 at 2 spp on a fixed 1/16 pixel
-    subsample (every 4th pixel in x and y)."""
+    """The oracle (scalar C restatement of the reference shaders, pthreads
+    over pixels) on frame 0 at 2 spp: on all the cores this process may use
+    (at most 16, the host share of one GPU) over every 4th pixel in x and y,
+    and on one core over every 16th pixel."""
     import oracle
     p = sc.particles.cpu().numpy()
     r = sc.radii.cpu().numpy() if sc.radii is not None else None
     W, H = wl.width, wl.height
-    threads = threads or min(16, os.cpu_count() or 1)
+    nproc = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = nproc
+    threads = threads or min(16, avail)
     R = oracle.Render(p, r, sc.aabb, wl.colormap, wl.radiusmap, wl.densitymap, cam, W, H, wl.unit, wl.rbf_radius,
                       frame_id=0, accum_id=1)
     t0 = time.time()
     prep = R.prepared()
     setup = time.time() - t0
-    ys, xs = np.mgrid[0:H:4, 0:W:4]
-    pix = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.int32)
     spp = 2
+
+    def run(step, nthreads):
+        ys, xs = np.mgrid[0:H:step, 0:W:step]
+        pix = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.int32)
+        t = time.time()
+        prep.render(pix, spp=spp, threads=nthreads)
+        dt = time.time() - t
+        n = len(pix) * spp
+        log(f"cpu baseline: {n} samples in {dt:.2f} s on {nthreads} thread(s) (scene set-up {setup:.1f} s)")
+        return n / dt / 1e6, n
+
+    v_all, n_all = run(4, threads)
+    v_one, n_one = run(16, 1)
+    return {"value": round(v_all, 6), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"frame 0, {spp} spp, every 4th pixel in x and y ({n_all} samples of the {W}x{H} frame); "
+                      f"oracle/oracle.c -O2 -ffp-contract=off, {threads} threads, grid set-up {setup:.1f} s untimed",
+            "nproc": nproc, "cores_available": avail,
+            "single_core": {"value": round(v_one, 6), "unit": "Msamples/s", "cores": 1,
+                            "sample": f"frame 0, {spp} spp, every 16th pixel in x and y ({n_one} samples)"}}
+
+
+# Bytes per particle that each pre-processing stage moves as implemented
+# (every read and write of a pass counted once): space-time AABB reads the
+# float4; the Hilbert pass reads it and writes the 8-byte code; the sort
+# copies the codes (16), writes the index (4), runs six 8-bit radix passes
+# over 48-bit keys with 32-bit values (histogram read 8 + scatter read 12 +
+# write 12 each), gathers the particles (36), checks runs of equal codes
+# (24) and copies particles, codes and permutation back (56); the radii
+# follow the permutation (12); attribute range 16, normalisation 32; leaf
+# bounds read particle + radius and write the 24-byte box; the BVH reads the
+# leaf boxes and writes 32-byte level-0 nodes plus the internal levels and
+# quantised groups (about 12 more at 8 children per node).
+PREP_STAGE_BYTES = {"aabb": 16, "hilbert": 24, "sort": 16 + 4 + 6 * 32 + 36 + 24 + 56, "radii": 12, "range": 16,
+                    "normalize": 32, "bounds": 44, "bvh": 24 + 32 + 12}
+
+
+def prep_stages(ctx, stream, p_raw, r_raw, radiusmap, rbf, ppl=1, K=8):
+    """The pre-processing stages (hostCode.cpp:183-250, then GenRBFBounds
+    and the BVH build) on device-resident input, each bracketed by HIP
+    events on the library's stream.  Returns {stage: ms}."""
+    import torch
+    p = p_raw.clone()
+    r = r_raw.clone() if r_raw is not None else None
+    torch.cuda.synchronize()
+    ev = [("start", torch.cuda.Event(enable_timing=True))]
+    ev[0][1].record(stream)
+
+    def mark(name):
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(stream)
+        ev.append((name, e))
+
+    aabb = ctx.spacetime_aabb(p)
+    mark("aabb")
+    codes = ctx.hilbert_codes(p, aabb)
+    mark("hilbert")
+    perm = ctx.sort_particles(p, codes)
+    mark("sort")
+    if r is not None:
+        r = r[perm.long()].contiguous()
+        mark("radii")
+    rng = ctx.attribute_range(p)
+    mark("range")
+    ctx.normalize_attributes(p, rng)
+    mark("normalize")
+    aabbs = ctx.gen_bounds(p, ppl, rbf, radiusmap, r)
+    mark("bounds")
+    bvh = ctx.bvh_build(aabbs, K)
+    mark("bvh")
+    torch.cuda.synchronize()
+    out = {name: ev[i][1].elapsed_time(e) for i, (name, e) in enumerate(ev[1:])}
+    ctx.bvh_destroy(bvh)
+    del p, r, codes, perm, aabbs
+    return out
+
+
+def _prep_summary(stages, n):
+    total = sum(stages.values())
+    nbytes = sum(PREP_STAGE_BYTES[k] for k in stages) * n
+    return {"stage_ms": {k: round(v, 3) for k, v in stages.items()}, "total_ms": round(total, 3),
+            "bytes": int(nbytes), "gbs": round(nbytes / (total * 1e-3) / 1e9, 1),
+            "frac": round(nbytes / (total * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def preprocess_leg(ctx, stream, wl, sc, cams, prep_s):
+    """Pre-processing of the main workload: per-stage device time on
+    device-resident input (GB/s), and the time to the first frame from host
+    arrays on a fresh context (context creation, host-to-device copy,
+    pre-processing, bounds, BVH, one 16-spp 1080p frame)."""
+    import torch
+    import aarbf
+    n = int(wl.xyzw.shape[0])
+    p_raw = torch.from_numpy(wl.xyzw).cuda()
+    r_raw = torch.from_numpy(wl.radii).cuda()
+    prep_stages(ctx, stream, p_raw, r_raw, sc.radiusmap, wl.rbf_radius)  # warm-up
+    out = _prep_summary(prep_stages(ctx, stream, p_raw, r_raw, sc.radiusmap, wl.rbf_radius), n)
+    del p_raw, r_raw
+    log(f"pre-process stages at {n} particles: {out}")
+    # time to first frame, fresh context
+    torch.cuda.synchronize()
     t0 = time.time()
-    prep.render(pix, spp=spp, threads=threads)
-    dt = time.time() - t0
-    n = len(pix) * spp
-    log(f"cpu baseline: {n} samples in {dt:.2f} s on {threads} threads (scene set-up {setup:.1f} s)")
-    return {"value": round(n / dt / 1e6, 6), "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"frame 0, {spp} spp, every 4th pixel in x and y ({n} samples of the {W}x{H} frame); "
-                      f"oracle/oracle.c -O2 -ffp-contract=off, {threads} threads, grid set-up {setup:.1f} s untimed"}
+    c2 = aarbf.Context(0)
+    s2 = aarbf.Scene.prepare(c2, wl.xyzw, wl.colormap, wl.radiusmap, wl.densitymap, wl.rbf_radius, radii=wl.radii,
+                             ppl=1, branching=8)
+    img = torch.zeros((wl.height, wl.width, 4), dtype=torch.float32, device="cuda")
+    acc = torch.zeros_like(img)
+    c2.render(s2.raygen_data(img, acc, wl.width, wl.height), s2.push_constants(cams[0], wl.unit), wl.width,
+              wl.height, wl.spp)
+    c2.sync()
+    ttff = time.time() - t0
+    c2.close()
+    del s2, img, acc
+    torch.cuda.empty_cache()
+    out.update({"workload": f"mixture-{n}", "prepare_ms_host_arrays": round(prep_s * 1e3, 1),
+                "time_to_first_frame_ms": round(ttff * 1e3, 1),
+                "time_to_first_frame": "fresh context: host arrays -> device, pre-process, bounds, BVH, first "
+                                       f"{wl.spp}-spp {wl.width}x{wl.height} frame"})
+    log(f"time to first frame (8M, fresh context): {ttff * 1e3:.0f} ms")
+    return out
+
+
+def preprocess_67m_leg(ctx, stream):
+    """The 67,108,864-particle precision scene (3-level hierarchical
+    clusters, generated on the device): per-stage pre-processing time and
+    the time from device-resident particles to the first 3840x2160 8-spp
+    frame with the FP16 payload."""
+    import torch
+    import aarbf
+    from aarbf import scenes
+    n, W, H, spp, unit = 67108864, 3840, 2160, 8, 0.0004
+    t0 = time.time()
+    xyzw, radii = scenes.hierarchical_clusters(n=n, seed=13, device="cuda")
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+    tfs = (scenes.reference_tf("colormap"), scenes.reference_tf("radiusmap"), scenes.reference_tf("densitymap"))
+    rbf = float(radii.max())
+    rm = ctx.tf(tfs[1])
+    stages = prep_stages(ctx, stream, xyzw, radii, rm, rbf)
+    ctx.tf_destroy(rm)
+    out = _prep_summary(stages, n)
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    e[0].record(stream)
+    sc = aarbf.Scene.prepare(ctx, xyzw, *tfs, rbf, radii=radii, ppl=1, branching=8)
+    e[1].record(stream)
+    lf, la, lu = scenes.orbit_camera(1, 32, 1.4)
+    cam = aarbf.camera_basis(lf, la, lu, scenes.fovy_scale(45.0), W, H)
+    img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    acc = torch.zeros_like(img)
+    ctx.set_payload(ctx.PAYLOAD_FP16)
+    try:
+        ctx.render(sc.raygen_data(img, acc, W, H), sc.push_constants(cam, unit, frame_id=8), W, H, spp)
+        e[2].record(stream)
+        torch.cuda.synchronize()
+    finally:
+        ctx.set_payload(ctx.PAYLOAD_FP32)
+    prep_ms, frame_ms = e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])
+    for h in (sc.colormap, sc.radiusmap, sc.densitymap):
+        ctx.tf_destroy(h)
+    ctx.bvh_destroy(sc.bvh)
+    del sc, xyzw, radii, img, acc
+    torch.cuda.empty_cache()
+    out.update({"workload": "hierarchical-67108864 (3 levels x 16^3 clusters, power-law radii), generated on the "
+                            "device", "generate_s": round(gen_s, 2), "prepare_ms": round(prep_ms, 1),
+                "first_frame_ms": round(frame_ms, 1), "time_to_first_frame_ms": round(prep_ms + frame_ms, 1),
+                "first_frame": f"{W}x{H}, {spp} spp delta tracking, FP16 payload"})
+    log(f"67M: {out}")
+    return out
 
 
 if __name__ == "__main__":
