@@ -260,6 +260,52 @@ def mixture_8m(n: int = 8388608, seed: int = 3, width: int = 1920, height: int =
                     reference_tf("densitymap"), frames=frames, taa=True)
 
 
+@dataclass
+class DiffWorkload:
+    """Differentiation workload: the scene, its perturbed target copy, the
+    tables and the view set."""
+    xyzw: np.ndarray         # float32 [N, 4], attributes normalised to [0, 1]
+    radii: np.ndarray        # float32 [N]
+    target_xyzw: np.ndarray  # positions + N(0, 0.002), attributes + N(0, 0.05) clipped to [0, 1]
+    target_radii: np.ndarray  # radii * U[0.9, 1.1]
+    colormap: np.ndarray     # 256-entry RGBA ramp
+    densitymap: np.ndarray   # 256-entry ramp
+    radiusmap: np.ndarray    # constant 1
+    unit: float              # 0.1 x the median radius
+    rbf_radius: float        # the largest radius of scene and target (clip-box margin)
+    width: int = 512
+    height: int = 512
+    spp: int = 4
+    views: int = 64
+
+    def view(self, v: int):
+        """(look_from, look_at, up) of view v: a seeded sphere of radius 1.6
+        about the centre of the unit cube."""
+        z = 2.0 * float(uniform(23, self.views, 0)[v]) - 1.0
+        ph = 2.0 * np.pi * float(uniform(23, self.views, 1)[v])
+        rr = np.sqrt(max(0.0, 1.0 - z * z))
+        lf = (0.5 + 1.6 * rr * np.cos(ph), 0.5 + 1.6 * z, 0.5 + 1.6 * rr * np.sin(ph))
+        return lf, (0.5, 0.5, 0.5), ((0.0, 1.0, 0.0) if abs(z) < 0.95 else (1.0, 0.0, 0.0))
+
+
+def diff_workload(n: int = 1048576, seed: int = 7, views: int = 64) -> DiffWorkload:
+    """Differentiation workload: n particles from the 32-component mixture
+    (as mixture_8m) with attributes normalised to [0, 1]; the target is the
+    same set with positions + N(0, 0.002), radii x U[0.9, 1.1] and
+    attributes + N(0, 0.05); 64 views of 512 x 512 at 4 spp on a seeded
+    sphere of cameras."""
+    xyzw, radii = gaussian_mixture(n, 32, seed)
+    attr = xyzw[:, 3].astype(np.float64)
+    xyzw[:, 3] = ((attr - attr.min()) / max(attr.max() - attr.min(), 1e-12)).astype(np.float32)
+    rng = np.random.default_rng(17)
+    tgt = xyzw.copy()
+    tgt[:, :3] += rng.normal(0.0, 0.002, (n, 3)).astype(np.float32)
+    tgt[:, 3] = np.clip(tgt[:, 3] + rng.normal(0.0, 0.05, n), 0.0, 1.0).astype(np.float32)
+    tgt_r = (radii * rng.uniform(0.9, 1.1, n)).astype(np.float32)
+    return DiffWorkload(xyzw, radii, tgt, tgt_r, ramp_table(256), ramp_table(256), constant_table(n=4),
+                        0.1 * float(np.median(radii)), float(max(radii.max(), tgt_r.max())), views=views)
+
+
 def mixture_16m(n: int = 16777216, seed: int = 5):
     """Batching workload scene: 16,777,216 particles from a 64-component
     Gaussian mixture (as mixture_8m otherwise)."""

@@ -402,28 +402,18 @@ def diff_leg(ctx, stream, a):
     from aarbf import scenes
     from aarbf.diff import _Render
     t0 = time.time()
-    xyzw, radii = scenes.gaussian_mixture(a.diff_particles, 32, 7)
-    attr = xyzw[:, 3].astype(np.float64)
-    xyzw[:, 3] = ((attr - attr.min()) / max(attr.max() - attr.min(), 1e-12)).astype(np.float32)
-    rng = np.random.default_rng(17)
-    tgt = xyzw.copy()
-    tgt[:, :3] += rng.normal(0.0, 0.002, (len(xyzw), 3)).astype(np.float32)
-    tgt[:, 3] = np.clip(tgt[:, 3] + rng.normal(0.0, 0.05, len(xyzw)), 0.0, 1.0).astype(np.float32)
-    tgt_r = (radii * rng.uniform(0.9, 1.1, len(radii))).astype(np.float32)
-    cm, dm, rm = scenes.ramp_table(256), scenes.ramp_table(256), scenes.constant_table(n=4)
+    dw = scenes.diff_workload(n=a.diff_particles)
+    xyzw, radii, tgt, tgt_r = dw.xyzw, dw.radii, dw.target_xyzw, dw.target_radii
+    cm, dm, rm = dw.colormap, dw.densitymap, dw.radiusmap
     dev = torch.device("cuda")
     T = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
-    W = H = 512
-    spp, unit = 4, 0.1 * float(np.median(radii))
-    rbf = float(max(radii.max(), tgt_r.max()))
+    W, H = dw.width, dw.height
+    spp, unit, rbf = dw.spp, dw.unit, dw.rbf_radius
     log(f"diff leg: workload generated in {time.time() - t0:.1f} s")
     fw, bw, rebuild = [], [], []
     for v in range(a.diff_views):
-        z = 2.0 * float(scenes.uniform(23, a.diff_views, 0)[v]) - 1.0
-        ph = 2.0 * np.pi * float(scenes.uniform(23, a.diff_views, 1)[v])
-        rr = np.sqrt(max(0.0, 1.0 - z * z))
-        lf = (0.5 + 1.6 * rr * np.cos(ph), 0.5 + 1.6 * z, 0.5 + 1.6 * rr * np.sin(ph))
-        cam = aarbf_camera(lf, (0.5, 0.5, 0.5), (0.0, 1.0, 0.0) if abs(z) < 0.95 else (1.0, 0.0, 0.0), W, H)
+        lf, la, up = dw.view(v)
+        cam = aarbf_camera(lf, la, up, W, H)
         torch.cuda.synchronize()
         tr = time.time()
         target = _Render(ctx, cam, W, H, unit, spp, v, rbf, 1, 8, T(tgt[:, :3]), T(tgt[:, 3]), T(tgt_r), T(cm),

@@ -447,20 +447,45 @@ class Diff:
         L.orc_diff_forward.argtypes = [C.POINTER(OrcDiff), C.POINTER(C.c_double)]
         L.orc_diff_adjoint.restype = C.c_int
         L.orc_diff_adjoint.argtypes = [C.POINTER(OrcDiff)] + [C.POINTER(C.c_double)] * 7
+        L.orc_diff_forward_pixels.restype = C.c_int
+        L.orc_diff_forward_pixels.argtypes = [C.POINTER(OrcDiff), C.POINTER(C.c_int32), C.c_uint64,
+                                              C.POINTER(C.c_double)]
+        L.orc_diff_adjoint_pixels.restype = C.c_int
+        L.orc_diff_adjoint_pixels.argtypes = [C.POINTER(OrcDiff), C.POINTER(C.c_int32), C.c_uint64] + \
+            [C.POINTER(C.c_double)] * 7
 
-    def forward(self) -> np.ndarray:
+    def forward(self, pixels=None) -> np.ndarray:
+        """[H, W, 3] image, or [n, 3] for the given (x, y) pixels only."""
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        if pixels is not None:
+            px = np.ascontiguousarray(np.asarray(pixels, np.int32).reshape(-1, 2))
+            out = np.zeros((px.shape[0], 3), np.float64)
+            if lib().orc_diff_forward_pixels(C.byref(self.p), px.ctypes.data_as(C.POINTER(C.c_int32)), px.shape[0],
+                                             dp(out)) != 0:
+                raise RuntimeError("orc_diff_forward_pixels failed")
+            return out
         img = np.zeros((self.p.height, self.p.width, 3), np.float64)
-        if lib().orc_diff_forward(C.byref(self.p), img.ctypes.data_as(C.POINTER(C.c_double))) != 0:
+        if lib().orc_diff_forward(C.byref(self.p), dp(img)) != 0:
             raise RuntimeError("orc_diff_forward failed")
         return img
 
-    def adjoint(self, gimage: np.ndarray) -> dict:
+    def adjoint(self, gimage: np.ndarray, pixels=None) -> dict:
+        """Gradients of sum(gimage * image); with `pixels`, gimage is [n, 3]
+        for those (x, y) pixels and the other pixels do not contribute."""
         n, ncm, ndm, nrm = self.sizes
         g = np.ascontiguousarray(gimage, np.float64)
         out = {"pos": np.zeros((n, 3)), "radii": np.zeros(n), "attr": np.zeros(n), "colormap": np.zeros((ncm, 4)),
                "densitymap": np.zeros(ndm), "radiusmap": np.zeros(nrm)}
         dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
-        if lib().orc_diff_adjoint(C.byref(self.p), dp(g), dp(out["pos"]), dp(out["radii"]), dp(out["attr"]),
-                                  dp(out["colormap"]), dp(out["densitymap"]), dp(out["radiusmap"])) != 0:
+        grads = (dp(out["pos"]), dp(out["radii"]), dp(out["attr"]), dp(out["colormap"]), dp(out["densitymap"]),
+                 dp(out["radiusmap"]))
+        if pixels is not None:
+            px = np.ascontiguousarray(np.asarray(pixels, np.int32).reshape(-1, 2))
+            assert g.shape == (px.shape[0], 3)
+            st = lib().orc_diff_adjoint_pixels(C.byref(self.p), px.ctypes.data_as(C.POINTER(C.c_int32)), px.shape[0],
+                                               dp(g), *grads)
+        else:
+            st = lib().orc_diff_adjoint(C.byref(self.p), dp(g), *grads)
+        if st != 0:
             raise RuntimeError("orc_diff_adjoint failed")
         return out

@@ -242,20 +242,32 @@ static dpart *parts(const orc_diff *p) {
   return pp;
 }
 
+static void pixel_forward(const orc_diff *p, const dpart *pp, int px, int py, double out[3]) {
+  double acc[3] = {0, 0, 0};
+  for (uint32_t s = 0; s < p->spp; ++s) {
+    fwd_state st = {0.0, {0, 0, 0}, 1.0};
+    march(p, pp, px, py, s, fwd_point, &st);
+    for (int k = 0; k < 3; ++k) acc[k] += st.C[k];
+  }
+  for (int k = 0; k < 3; ++k) out[k] = acc[k] / p->spp;
+}
+
 /* image: W*H*3 doubles (mean over spp samples) */
 ORC_EXPORT int orc_diff_forward(const orc_diff *p, double *image) {
   dpart *pp = parts(p);
   if (!pp) return -1;
   for (int py = 0; py < p->height; ++py)
-    for (int px = 0; px < p->width; ++px) {
-      double acc[3] = {0, 0, 0};
-      for (uint32_t s = 0; s < p->spp; ++s) {
-        fwd_state st = {0.0, {0, 0, 0}, 1.0};
-        march(p, pp, px, py, s, fwd_point, &st);
-        for (int k = 0; k < 3; ++k) acc[k] += st.C[k];
-      }
-      for (int k = 0; k < 3; ++k) image[3 * ((size_t)py * p->width + px) + k] = acc[k] / p->spp;
-    }
+    for (int px = 0; px < p->width; ++px) pixel_forward(p, pp, px, py, image + 3 * ((size_t)py * p->width + px));
+  free(pp);
+  return 0;
+}
+
+/* The same for the listed (x, y) pixels of the W x H image only: out is
+ * npix * 3 doubles. */
+ORC_EXPORT int orc_diff_forward_pixels(const orc_diff *p, const int32_t *pixels, uint64_t npix, double *out) {
+  dpart *pp = parts(p);
+  if (!pp) return -1;
+  for (uint64_t i = 0; i < npix; ++i) pixel_forward(p, pp, pixels[2 * i], pixels[2 * i + 1], out + 3 * i);
   free(pp);
   return 0;
 }
@@ -437,6 +449,41 @@ static void rt_adjoint(const orc_diff *p, const dpart *pp, const rt_state *st, c
   free(R);
 }
 
+/* adds the adjoint of one pixel (g: dLoss/dpixel, rgb) */
+static void pixel_adjoint(const orc_diff *p, const dpart *pp, int px, int py, const double *gpix, double *g_pos,
+                          double *g_rad, double *g_attr, double *g_cm, double *g_dm, double *g_rm) {
+  double g[3];
+  for (int k = 0; k < 3; ++k) g[k] = gpix[k] / p->spp;
+  for (uint32_t s = 0; s < p->spp; ++s) {
+    if (p->estimator == 1) {
+      rt_state rs = {NULL, 0, 0, 1.0};
+      march(p, pp, px, py, s, rt_keep, &rs);
+      rt_adjoint(p, pp, &rs, g, g_pos, g_rad, g_attr, g_cm, g_dm, g_rm);
+      free(rs.pts);
+      continue;
+    }
+    adj_state st;
+    memset(&st, 0, sizeof(st));
+    st.g = g;
+    march(p, pp, px, py, s, pass1, &st);
+    if (st.have_prev) { /* the last point: c_{K+1} = 0 */
+      double dF;
+      Fcdf(st.L, &dF);
+      double gd = 0;
+      for (int k = 0; k < 3; ++k) gd += g[k] * st.prev_c[k];
+      st.total += dF * gd;
+    }
+    st.have_prev = 0;
+    st.g_pos = g_pos;
+    st.g_rad = g_rad;
+    st.g_attr = g_attr;
+    st.g_cm = g_cm;
+    st.g_dm = g_dm;
+    st.g_rm = g_rm;
+    march(p, pp, px, py, s, pass2, &st);
+  }
+}
+
 /* gimage: dLoss/dimage, W*H*3.  Gradients are accumulated (callers zero
  * them): g_pos 3N, g_rad N, g_attr N, g_cm 4*cm_n, g_dm dm_n, g_rm rm_n. */
 ORC_EXPORT int orc_diff_adjoint(const orc_diff *p, const double *gimage, double *g_pos, double *g_rad, double *g_attr,
@@ -444,38 +491,21 @@ ORC_EXPORT int orc_diff_adjoint(const orc_diff *p, const double *gimage, double 
   dpart *pp = parts(p);
   if (!pp) return -1;
   for (int py = 0; py < p->height; ++py)
-    for (int px = 0; px < p->width; ++px) {
-      double g[3];
-      for (int k = 0; k < 3; ++k) g[k] = gimage[3 * ((size_t)py * p->width + px) + k] / p->spp;
-      for (uint32_t s = 0; s < p->spp; ++s) {
-        if (p->estimator == 1) {
-          rt_state rs = {NULL, 0, 0, 1.0};
-          march(p, pp, px, py, s, rt_keep, &rs);
-          rt_adjoint(p, pp, &rs, g, g_pos, g_rad, g_attr, g_cm, g_dm, g_rm);
-          free(rs.pts);
-          continue;
-        }
-        adj_state st;
-        memset(&st, 0, sizeof(st));
-        st.g = g;
-        march(p, pp, px, py, s, pass1, &st);
-        if (st.have_prev) { /* the last point: c_{K+1} = 0 */
-          double dF;
-          Fcdf(st.L, &dF);
-          double gd = 0;
-          for (int k = 0; k < 3; ++k) gd += g[k] * st.prev_c[k];
-          st.total += dF * gd;
-        }
-        st.have_prev = 0;
-        st.g_pos = g_pos;
-        st.g_rad = g_rad;
-        st.g_attr = g_attr;
-        st.g_cm = g_cm;
-        st.g_dm = g_dm;
-        st.g_rm = g_rm;
-        march(p, pp, px, py, s, pass2, &st);
-      }
-    }
+    for (int px = 0; px < p->width; ++px)
+      pixel_adjoint(p, pp, px, py, gimage + 3 * ((size_t)py * p->width + px), g_pos, g_rad, g_attr, g_cm, g_dm,
+                    g_rm);
+  free(pp);
+  return 0;
+}
+
+/* The adjoint of the listed (x, y) pixels only; gpix: npix * 3 doubles. */
+ORC_EXPORT int orc_diff_adjoint_pixels(const orc_diff *p, const int32_t *pixels, uint64_t npix, const double *gpix,
+                                       double *g_pos, double *g_rad, double *g_attr, double *g_cm, double *g_dm,
+                                       double *g_rm) {
+  dpart *pp = parts(p);
+  if (!pp) return -1;
+  for (uint64_t i = 0; i < npix; ++i)
+    pixel_adjoint(p, pp, pixels[2 * i], pixels[2 * i + 1], gpix + 3 * i, g_pos, g_rad, g_attr, g_cm, g_dm, g_rm);
   free(pp);
   return 0;
 }

@@ -110,3 +110,40 @@ def test_ratio_tracking_forward_is_the_tracking_expectation():
     # measured: max 2.2e-3, mean 5.9e-4 of a mean colour 0.20 at K = 1024
     np.testing.assert_allclose(img, e, atol=8e-3)
     assert float(np.abs(img - e).mean()) < 2e-3
+
+
+def test_per_pixel_evaluation_on_the_particles_near_the_ray():
+    """The per-pixel entry points over only the particles whose spheres can
+    meet the pixel's ray give the full-scene values bit for bit: the
+    forward, and the adjoint of one pixel (per-particle blocks on those
+    particles, zero elsewhere; transfer-function blocks)."""
+    from helpers import default_camera, near_ray, pixel_ray
+    rng = np.random.default_rng(8)
+    n, W, H = 3000, 24, 20
+    pos = rng.uniform(0.0, 1.0, (n, 3))
+    attr = rng.uniform(0.05, 0.95, n)
+    radii = rng.uniform(0.02, 0.06, n)
+    cm, dm, rm = scenes.ramp_table(64), scenes.ramp_table(32), scenes.constant_table(n=4)
+    cam = default_camera(W, H, look_from=(0.5, 0.6, 2.2), look_at=(0.5, 0.5, 0.5))
+    aabb = np.concatenate([pos.min(0), pos.max(0)]).astype(np.float32)
+    kw = dict(spp=2, frame_id=4)
+    full = oracle.Diff(pos, attr, radii, cm, dm, rm, cam, W, H, 0.004, 0.06, aabb, **kw)
+    img = full.forward()
+    for x, y in [(3, 4), (12, 10), (20, 15), (7, 18)]:
+        o, d, t0, t1 = pixel_ray(cam, W, H, x, y, aabb, 0.06)
+        idx = near_ray(pos, radii, o, d, t0, t1)
+        assert 0 < len(idx) < n // 4
+        sub = oracle.Diff(pos[idx], attr[idx], radii[idx], cm, dm, rm, cam, W, H, 0.004, 0.06, aabb, **kw)
+        px = np.array([[x, y]], np.int32)
+        assert np.array_equal(sub.forward(pixels=px)[0], img[y, x]) and img[y, x].sum() > 0
+        g = np.array([[0.3, -1.2, 0.7]])
+        gi = np.zeros((H, W, 3))
+        gi[y, x] = g[0]
+        a_full = full.adjoint(gi)
+        a_sub = sub.adjoint(g, pixels=px)
+        for k in ("pos", "radii", "attr"):
+            assert np.array_equal(a_full[k][idx], a_sub[k])
+            rest = np.delete(a_full[k], idx, axis=0)
+            assert not np.any(rest), k
+        for k in ("colormap", "densitymap", "radiusmap"):
+            assert np.array_equal(a_full[k], a_sub[k]), k
