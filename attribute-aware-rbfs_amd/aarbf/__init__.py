@@ -303,6 +303,19 @@ class Context:
                                                   1 if srgb else 0, C.byref(h)), "aarbf_tf_create_rgba8")
         return h.value
 
+    def tf_from_marks(self, marks, kind: int) -> int:
+        """Table from a gradient of (position, rgba) marks, as the reference
+        builds it after a widget edit (aarbf_tf_create_from_marks); kind 0
+        colormap, 1 radius / density map."""
+        arr = (_lib.TFMark * len(marks))()
+        for k, (pos, rgba) in enumerate(marks):
+            arr[k].position = float(pos)
+            arr[k].rgba[:] = [float(v) for v in rgba]
+        h = C.c_uint64()
+        self.check(self.lib.aarbf_tf_create_from_marks(self.h, arr, len(marks), int(kind), C.byref(h)),
+                   "aarbf_tf_create_from_marks")
+        return h.value
+
     def stbn(self, atlas) -> int:
         a = np.ascontiguousarray(np.asarray(atlas, dtype=np.uint8))
         assert a.ndim == 3 and a.shape[2] == 4

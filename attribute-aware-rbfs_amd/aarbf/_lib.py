@@ -91,6 +91,11 @@ class CameraInput(C.Structure):
                 ("key_x", i32), ("key_y", i32), ("key_z", i32), ("left_shift", i32)]
 
 
+class TFMark(C.Structure):
+    """One mark of a transfer-function gradient."""
+    _fields_ = [("position", f32), ("rgba", f32 * 4)]
+
+
 class RenderStats(C.Structure):
     _fields_ = [(n, u64) for n in ("samples", "steps", "queries", "nodes", "leaves", "particles",
                                    "covering", "exhausted")]
@@ -125,6 +130,7 @@ SIGNATURES = {
     "aarbf_timesteps_count": (u32, [P]),
     "aarbf_timesteps_get": (C.c_int, [P, u32, C.POINTER(FP), C.POINTER(u64), C.POINTER(C.c_char_p)]),
     "aarbf_timesteps_destroy": (None, [P]),
+    "aarbf_import_step_device": (C.c_int, [P, C.c_char_p, u32, u64, u64, C.POINTER(u64)]),
     "aarbf_export_particles": (C.c_int, [C.c_char_p, u32, C.POINTER(C.c_char_p), C.POINTER(FP),
                                          C.POINTER(u64)]),
     "aarbf_spacetime_aabb": (C.c_int, [P, u64, u64, FP]),
@@ -135,6 +141,8 @@ SIGNATURES = {
     "aarbf_tf_create": (C.c_int, [P, FP, u32, C.POINTER(u64)]),
     "aarbf_tf_create_rgba8": (C.c_int, [P, C.POINTER(C.c_uint8), u32, C.c_int, C.POINTER(u64)]),
     "aarbf_tf_destroy": (C.c_int, [P, u64]),
+    "aarbf_tf_texels_from_marks": (C.c_int, [C.POINTER(TFMark), u32, C.c_int, C.POINTER(C.c_uint8)]),
+    "aarbf_tf_create_from_marks": (C.c_int, [P, C.POINTER(TFMark), u32, C.c_int, C.POINTER(u64)]),
     "aarbf_stbn_create": (C.c_int, [P, C.POINTER(C.c_uint8), C.c_int, C.c_int, C.POINTER(u64)]),
     "aarbf_stbn_destroy": (C.c_int, [P, u64]),
     "aarbf_gen_bounds": (C.c_int, [P, C.POINTER(BoundsConstants)]),

@@ -56,18 +56,6 @@ def make_8bit(f: float) -> int:
     return min(255, max(0, int(np.float32(f) * np.float32(256.0))))
 
 
-def _gradient(marks, pos: float):
-    """Piecewise-linear interpolation between (position, rgba) marks."""
-    marks = sorted(marks, key=lambda m: m[0])
-    if pos <= marks[0][0]:
-        return np.asarray(marks[0][1], np.float64)
-    for (p0, c0), (p1, c1) in zip(marks, marks[1:]):
-        if pos <= p1:
-            t = 0.0 if p1 == p0 else (pos - p0) / (p1 - p0)
-            return (1 - t) * np.asarray(c0, np.float64) + t * np.asarray(c1, np.float64)
-    return np.asarray(marks[-1][1], np.float64)
-
-
 # hostCode.cpp:420-436 default marks
 REFERENCE_COLORMAP_MARKS = [(0.0, (0.0, 0.0, 1.0, 1.0)), (0.25, (0.0, 0.972549, 1.0, 1.0)),
                             (0.5, (0.0, 1.0, 0.00784314, 1.0)), (0.75, (0.996078, 1.0, 0.0, 1.0)),
@@ -75,25 +63,33 @@ REFERENCE_COLORMAP_MARKS = [(0.0, (0.0, 0.0, 1.0, 1.0)), (0.25, (0.0, 0.972549, 
 REFERENCE_RADIUSMAP_MARKS = [(0.0, (1.0, 1.0, 1.0, 1.0)), (1.0, (1.0, 1.0, 1.0, 1.0))]
 REFERENCE_DENSITYMAP_MARKS = [(0.0, (0.0, 0.0, 0.0, 0.0)), (1.0, (1.0, 1.0, 1.0, 1.0))]
 
+TF_COLORMAP, TF_SCALAR = 0, 1
+
+
+def tf_texels_from_marks(marks, kind: int) -> np.ndarray:
+    """The 64 RGBA8 texels of a gradient of (position, rgba) marks
+    (aarbf_tf_texels_from_marks, hostCode.cpp:524-572): kind TF_COLORMAP
+    gamma-encodes RGB with pow(c, 1/2.2) and stores alpha linearly;
+    TF_SCALAR (radius / density maps) writes only R."""
+    import ctypes as C
+    from . import _lib
+    arr = (_lib.TFMark * len(marks))()
+    for k, (pos, rgba) in enumerate(marks):
+        arr[k].position = float(pos)
+        arr[k].rgba[:] = [float(v) for v in rgba]
+    out = np.zeros((64, 4), dtype=np.uint8)
+    st = _lib.load().aarbf_tf_texels_from_marks(arr, len(marks), int(kind), out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if st != 0:
+        raise ValueError("aarbf_tf_texels_from_marks: marks need positions in [0, 1]")
+    return out
+
 
 def reference_tf_texels(which: str) -> np.ndarray:
     """The 64 RGBA8 texels the reference writes into its R8G8B8A8_SRGB TF
-    textures (hostCode.cpp:528-572): colormap RGB gamma-encoded with
-    pow(c, 1/2.2), alpha linear; radius/density maps write only R.  The
-    gradient between marks is taken as piecewise linear (the widget's own
-    interpolation is not part of the reference tree)."""
+    textures for its default marks (hostCode.cpp:416-436,528-572)."""
     marks = {"colormap": REFERENCE_COLORMAP_MARKS, "radiusmap": REFERENCE_RADIUSMAP_MARKS,
              "densitymap": REFERENCE_DENSITYMAP_MARKS}[which]
-    out = np.zeros((64, 4), dtype=np.uint8)
-    for i in range(64):
-        c = _gradient(marks, np.float32(i) / np.float32(63.0))
-        if which == "colormap":
-            for k in range(3):
-                out[i, k] = make_8bit(float(np.float32(c[k]) ** np.float32(1.0 / 2.2)))
-            out[i, 3] = make_8bit(c[3])
-        else:
-            out[i, 0] = make_8bit(c[0])
-    return out
+    return tf_texels_from_marks(marks, TF_COLORMAP if which == "colormap" else TF_SCALAR)
 
 
 def srgb_decode_table(texels: np.ndarray) -> np.ndarray:

@@ -47,6 +47,30 @@ def import_particles(path: str) -> list[tuple[str, np.ndarray]]:
         lib.aarbf_timesteps_destroy(h)
 
 
+def import_step_device(ctx: Context, path: str, step: int, out=None, capacity: int | None = None):
+    """Timestep `step` of the file set `path` read straight into a device
+    float32 [N, 4] tensor (aarbf_import_step_device: chunked reads through
+    pinned staging buffers overlapped with the copies).  `out` (a device
+    tensor with room for `capacity` particles) is filled in place when
+    given; returns the tensor holding the step's particles."""
+    import torch
+    lib = _lib.load()
+    if out is None:
+        # the count first, then a buffer of that size
+        n = C.c_uint64()
+        st = lib.aarbf_import_step_device(ctx.h, path.encode(), step, 0, 0, C.byref(n))
+        if st not in (0, 1):
+            ctx.check(st, "aarbf_import_step_device")
+        out = torch.empty((max(1, n.value), 4), dtype=torch.float32, device=f"cuda:{ctx.device}")
+        capacity = n.value
+    cap = out.shape[0] if capacity is None else capacity
+    n = C.c_uint64()
+    torch.cuda.current_stream(ctx.device).synchronize()
+    ctx.check(lib.aarbf_import_step_device(ctx.h, path.encode(), step, out.data_ptr(), cap, C.byref(n)),
+              "aarbf_import_step_device")
+    return out[: n.value]
+
+
 def export_particles(path: str, steps) -> None:
     """Writes [(name, float32 [N, 4])] in the importer's format."""
     lib = _lib.load()

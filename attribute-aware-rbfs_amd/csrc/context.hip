@@ -129,6 +129,10 @@ int aarbf_context_destroy(aarbf_context *ctx) {
   if (ctx->scratch) hipFree(ctx->scratch);
   if (ctx->baked) hipFree(ctx->baked);
   if (ctx->lat_buf) hipFree(ctx->lat_buf);
+  for (int k = 0; k < 2; ++k) {
+    if (ctx->stage[k]) hipHostFree(ctx->stage[k]);
+    if (ctx->stage_ev[k]) hipEventDestroy(ctx->stage_ev[k]);
+  }
   for (auto &g : ctx->graphs) hipGraphExecDestroy(g.second);
   if (ctx->d_stats) hipFree(ctx->d_stats);
   if (ctx->d_counter) hipFree(ctx->d_counter);
@@ -298,6 +302,15 @@ int aarbf_tf_create_rgba8(aarbf_context *ctx, const uint8_t *rgba8, uint32_t n, 
       f[4 * i + c] = (srgb && c < 3) ? srgb_to_linear(v) : (float)v / 255.0f;
     }
   return aarbf_tf_create(ctx, f.data(), n, handle);
+}
+
+int aarbf_tf_create_from_marks(aarbf_context *ctx, const aarbf_tf_mark *marks, uint32_t n, int kind,
+                               uint64_t *handle) {
+  if (!ctx) return AARBF_ERR_INVALID_ARGUMENT;
+  uint8_t texels[4 * AARBF_TF_TEXELS];
+  AARBF_REQUIRE(ctx, aarbf_tf_texels_from_marks(marks, n, kind, texels) == AARBF_OK,
+                "aarbf_tf_create_from_marks: need 1 or more marks with positions in [0, 1] and a valid kind");
+  return aarbf_tf_create_rgba8(ctx, texels, AARBF_TF_TEXELS, 1, handle);
 }
 
 int aarbf_tf_destroy(aarbf_context *ctx, uint64_t handle) {

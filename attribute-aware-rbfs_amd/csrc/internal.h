@@ -100,6 +100,9 @@ struct aarbf_context {
   // per-view latency record of the last aarbf_render_batch: 2 words per view
   // (samples finished, wall clock at the last) and the clock at the start
   unsigned long long *lat_buf = nullptr;
+  // pinned staging buffers of the streaming importer (aarbf_import_step_device)
+  void *stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
   uint32_t lat_cap = 0, lat_n = 0;
   std::set<uint64_t> tfs, stbns, bvhs;
 };

@@ -3,8 +3,10 @@
 // A header text file lists one timestep name per line; timestep `name` is
 // the binary file <header path> + "_" + name holding a little-endian uint64
 // particle count followed by count float4 (x, y, z, attribute).  The
-// reference reads without checks; here a missing, short or oversized
-// timestep file is reported as an error instead of yielding garbage.
+// reference reads without checks; here a missing timestep file, or one
+// shorter than its count says, is reported as an error instead of yielding
+// garbage.  Bytes after the count-th particle are ignored, as the
+// reference ignores them.
 #include <cstdio>
 #include <cstring>
 #include <fstream>

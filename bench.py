@@ -677,7 +677,47 @@ def preprocess_leg(ctx, stream, wl, sc, cams, prep_s):
                 "time_to_first_frame": "fresh context: host arrays -> device, pre-process, bounds, BVH, first "
                                        f"{wl.spp}-spp {wl.width}x{wl.height} frame"})
     log(f"time to first frame (8M, fresh context): {ttff * 1e3:.0f} ms")
+    out.update(file_load_leg(wl, cams))
     return out
+
+
+def file_load_leg(wl, cams):
+    """The particle file (importParticles.h format, written to a temporary
+    directory just before, so it is read from the page cache) streamed to
+    the device by aarbf_import_step_device on a fresh context, then
+    pre-processed and rendered: load GB/s and the time to the first frame
+    from the file (radii, which the format does not hold, from memory)."""
+    import shutil
+    import tempfile
+    import torch
+    import aarbf
+    from aarbf import series
+    d = tempfile.mkdtemp(prefix="aarbf_bench_")
+    try:
+        path = os.path.join(d, "particles.txt")
+        series.export_particles(path, [("t0", wl.xyzw)])
+        torch.cuda.synchronize()
+        c2 = aarbf.Context(0)
+        t0 = time.time()
+        dev = series.import_step_device(c2, path, 0)
+        t_load = time.time() - t0
+        s2 = aarbf.Scene.prepare(c2, dev, wl.colormap, wl.radiusmap, wl.densitymap, wl.rbf_radius, radii=wl.radii,
+                                 ppl=1, branching=8)
+        img = torch.zeros((wl.height, wl.width, 4), dtype=torch.float32, device="cuda")
+        acc = torch.zeros_like(img)
+        c2.render(s2.raygen_data(img, acc, wl.width, wl.height), s2.push_constants(cams[0], wl.unit), wl.width,
+                  wl.height, wl.spp)
+        c2.sync()
+        ttff = time.time() - t0
+        c2.close()
+        del s2, img, acc, dev
+        torch.cuda.empty_cache()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    nbytes = 16 * wl.xyzw.shape[0]
+    log(f"file -> device: {nbytes / t_load / 1e9:.2f} GB/s; time to first frame from the file {ttff * 1e3:.0f} ms")
+    return {"file_load_ms": round(t_load * 1e3, 1), "file_load_gbs": round(nbytes / t_load / 1e9, 2),
+            "time_to_first_frame_from_file_ms": round(ttff * 1e3, 1)}
 
 
 def preprocess_67m_leg(ctx, stream):

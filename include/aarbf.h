@@ -212,6 +212,16 @@ uint32_t aarbf_timesteps_count(const aarbf_timesteps *ts);
 int aarbf_timesteps_get(const aarbf_timesteps *ts, uint32_t step, const float **xyzw,
                         uint64_t *count, const char **name);
 void aarbf_timesteps_destroy(aarbf_timesteps *ts);
+/* The device load path of the same format: timestep `step` of the header
+ * file `path` read straight into device float4 particles[capacity], in
+ * chunks through pinned staging buffers whose copies to the device overlap
+ * the next read (the reference reads into host vectors, importParticles.h
+ * :17-29, then copies through a mapped buffer, hostCode.cpp:759-763).
+ * *count receives the timestep's particle count, also when it exceeds
+ * capacity (then AARBF_ERR_INVALID_ARGUMENT and nothing is copied).
+ * Synchronises the context's stream. */
+int aarbf_import_step_device(aarbf_context *ctx, const char *path, uint32_t step, uint64_t particles,
+                             uint64_t capacity, uint64_t *count);
 /* Writes the same format (header plus one binary file per step). */
 int aarbf_export_particles(const char *path, uint32_t num_steps, const char *const *names,
                            const float *const *xyzw, const uint64_t *counts);
@@ -250,6 +260,27 @@ int aarbf_tf_create(aarbf_context *ctx, const float *rgba, uint32_t n, uint64_t 
 int aarbf_tf_create_rgba8(aarbf_context *ctx, const uint8_t *rgba8, uint32_t n, int srgb,
                           uint64_t *handle);
 int aarbf_tf_destroy(aarbf_context *ctx, uint64_t handle);
+/* The reference's transfer-function editing (hostCode.cpp:416-436,
+ * 524-572): a gradient of marks sampled at the 64 positions i / 63 and
+ * written as RGBA8 bytes.  The gradient is linear in RGBA between marks
+ * (sorted by position, each in [0, 1]) and takes the first / last mark's
+ * value outside them.  AARBF_TF_COLORMAP writes make_8bit(pow(c, 1 / 2.2))
+ * for RGB and make_8bit(alpha); AARBF_TF_SCALAR (radius and density maps)
+ * writes make_8bit(red) and zeros; make_8bit(f) = clamp(int(f * 256), 0,
+ * 255).  out: 64 * 4 bytes.  Host only. */
+typedef struct aarbf_tf_mark {
+  float position;
+  float rgba[4];
+} aarbf_tf_mark;
+#define AARBF_TF_TEXELS 64
+#define AARBF_TF_COLORMAP 0
+#define AARBF_TF_SCALAR 1
+int aarbf_tf_texels_from_marks(const aarbf_tf_mark *marks, uint32_t n, int kind, uint8_t *out);
+/* aarbf_tf_texels_from_marks followed by aarbf_tf_create_rgba8(srgb = 1):
+ * the table the reference's shaders sample after an edit of its gradient
+ * widget. */
+int aarbf_tf_create_from_marks(aarbf_context *ctx, const aarbf_tf_mark *marks, uint32_t n, int kind,
+                               uint64_t *handle);
 /* The blue-noise atlas: RGBA8 texels, width x height (the reference's
  * stbn.png is an 8x16 grid of 256x256 tiles, hostCode.cpp:336-352). */
 int aarbf_stbn_create(aarbf_context *ctx, const uint8_t *rgba8, int width, int height,
@@ -368,7 +399,11 @@ typedef struct aarbf_tune_result {
 } aarbf_tune_result;
 int aarbf_autotune(aarbf_context *ctx, const aarbf_request *reqs, uint32_t n, uint32_t reps,
                    aarbf_tune_result *out);
-/* ParticleSplatRayGen (deviceCodeSplat.hlsl:58-154). */
+/* ParticleSplatRayGen (deviceCodeSplat.hlsl:58-154).  Rays are clipped to
+ * the global AABB grown by one rbfRadius, as in the reference
+ * (deviceCodeSplat.hlsl:66-67); with per-particle radii set rbfRadius to at
+ * least the largest radii[i] * radiusmap, or samples of spheres reaching
+ * past that box are clipped. */
 int aarbf_render_splat(aarbf_context *ctx, const aarbf_raygen_data *rg,
                        const aarbf_push_constants *pc, int width, int height);
 /* ---- differentiable rendering (no counterpart in the reference) -------------

@@ -17,7 +17,8 @@
  *   - every multiply-add that should be fused is written as an explicit
  *     fmaf, and everything else must be compiled with -ffp-contract=off;
  *   - polynomial coefficients are this project's own least-squares fits
- *     (Chebyshev nodes, 100-bit reference values), rounded to binary32.
+ *     (64 Chebyshev nodes, 60-digit decimal arithmetic), rounded to
+ *     binary32; tests/golden/fit_detmath.py reproduces them.
  *
  * The header is part of the C ABI contract: the HIP kernels and any host
  * restatement of the renderer include it unchanged.  Accuracy against a
@@ -84,7 +85,7 @@ AA_FN float aa_logf(float x) {
   }
   float f = aa_u2f(mb) - 1.0f;
   float q = -0.07866667956113815f;
-  q = AA_FMAF(q, f, 0.12821750342845917f);
+  q = AA_FMAF(q, f, 0.12821748852729797f);
   q = AA_FMAF(q, f, -0.131016343832016f);
   q = AA_FMAF(q, f, 0.1418900042772293f);
   q = AA_FMAF(q, f, -0.1662759929895401f);
@@ -112,7 +113,7 @@ AA_FN float aa_log_step(float x) {
   e = hi ? e + 1 : e;
   float f = aa_u2f(mb) - 1.0f;
   float q = -0.07866667956113815f;
-  q = AA_FMAF(q, f, 0.12821750342845917f);
+  q = AA_FMAF(q, f, 0.12821748852729797f);
   q = AA_FMAF(q, f, -0.131016343832016f);
   q = AA_FMAF(q, f, 0.1418900042772293f);
   q = AA_FMAF(q, f, -0.1662759929895401f);
@@ -139,8 +140,8 @@ AA_FN float aa_expf(float x) {
   float k = AA_RINTF(x * 1.4426950216293335f);
   float r = AA_FMAF(k, -0.693145751953125f, x);
   r = AA_FMAF(k, -1.4286068203094172e-06f, r);
-  float p = 0.00019899285689461976f;
-  p = AA_FMAF(p, r, 0.0013933650916442275f);
+  float p = 0.00019899275503121316f;
+  p = AA_FMAF(p, r, 0.0013933643931522965f);
   p = AA_FMAF(p, r, 0.0083332983776927f);
   p = AA_FMAF(p, r, 0.04166646674275398f);
   p = AA_FMAF(p, r, 0.1666666716337204f);
@@ -207,8 +208,8 @@ AA_FN float aa_expf_mid(float x) {
   float k = AA_RINTF(x * 1.4426950216293335f);
   float r = AA_FMAF(k, -0.693145751953125f, x);
   r = AA_FMAF(k, -1.4286068203094172e-06f, r);
-  float p = 0.00019899285689461976f;
-  p = AA_FMAF(p, r, 0.0013933650916442275f);
+  float p = 0.00019899275503121316f;
+  p = AA_FMAF(p, r, 0.0013933643931522965f);
   p = AA_FMAF(p, r, 0.0083332983776927f);
   p = AA_FMAF(p, r, 0.04166646674275398f);
   p = AA_FMAF(p, r, 0.1666666716337204f);

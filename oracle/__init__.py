@@ -107,7 +107,7 @@ def _bind(L: C.CDLL) -> C.CDLL:
                                C.POINTER(C.c_double), C.c_int]
     L.orc_splat.restype = C.c_int
     L.orc_splat.argtypes = [C.POINTER(OrcParams), f32p, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32),
-                            C.c_uint64, f32p]
+                            C.c_uint64, f32p, u32p]
     L.orc_rng_stream.restype = C.c_uint32
     L.orc_rng_stream.argtypes = [C.c_uint32] * 4 + [f32p, C.c_uint32]
     return L
@@ -345,22 +345,27 @@ class Render:
             raise RuntimeError("orc_expected failed (delta tracking only)")
         return out
 
-    def splat(self, leaf_aabbs: np.ndarray, ppl: int = 1, pixels=None) -> np.ndarray:
+    def splat(self, leaf_aabbs: np.ndarray, ppl: int = 1, pixels=None, hits: bool = False):
         """ParticleSplatRayGen over all pixels (or the given (x, y) pixels);
-        leaf_aabbs = the GenRBFBounds output the BVH was built from."""
+        leaf_aabbs = the GenRBFBounds output the BVH was built from.  With
+        hits=True also returns, per rendered pixel, the largest hit count of
+        one slab and of 16 consecutive slabs (uint32 [n, 2])."""
         W, H = self.p.width, self.p.height
         image = np.zeros((H, W, 4), np.float32)
         a = np.ascontiguousarray(leaf_aabbs, np.float32).reshape(-1)
         nl = a.size // 6
+        n = W * H if pixels is None else len(pixels)
+        hbuf = np.zeros((n, 2), np.uint32) if hits else None
+        hp = hbuf.ctypes.data_as(u32p) if hits else None
         if pixels is not None:
             px = np.ascontiguousarray(np.asarray(pixels, dtype=np.int32).reshape(-1, 2))
             st = lib().orc_splat(C.byref(self.p), _f(a), nl, ppl, px.ctypes.data_as(C.POINTER(C.c_int32)),
-                                 px.shape[0], _f(image))
+                                 px.shape[0], _f(image), hp)
         else:
-            st = lib().orc_splat(C.byref(self.p), _f(a), nl, ppl, None, 0, _f(image))
+            st = lib().orc_splat(C.byref(self.p), _f(a), nl, ppl, None, 0, _f(image), hp)
         if st != 0:
             raise RuntimeError("orc_splat failed")
-        return image
+        return (image, hbuf) if hits else image
 
     def shadow_visibility(self, origin, t0: float) -> float:
         """Visibility of the marching tracker's NEE shadow ray from `origin`

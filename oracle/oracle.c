@@ -915,8 +915,12 @@ static int cmp_hit_p(const void *a, const void *b) {
   return (x->p > y->p) - (x->p < y->p);
 }
 
+/* hits: NULL, or receives the largest hit count of a slab and of a run of
+ * 16 consecutive slabs among the slabs the pixel visits (before the first
+ * 15 are kept), so tests can see which overflow cases a scene reaches. */
 static void splat_pixel(const orc_params *p, const float *leaves /* 8 floats per widened leaf */, uint32_t nleaf,
-                        uint32_t ppl, int px, int py, splat_hit *cand, splat_hit *slab, float out[4]) {
+                        uint32_t ppl, int px, int py, splat_hit *cand, splat_hit *slab, float out[4],
+                        uint32_t hits[2]) {
   const int W = p->width, H = p->height;
   float sx = ((float)px + 0.5f) / (float)W, sy = ((float)py + 0.5f) / (float)H;
   float lb[3], rt[3], dir[3], inv[3];
@@ -958,6 +962,7 @@ static void splat_pixel(const orc_params *p, const float *leaves /* 8 floats per
     const float fade = aa_expf(-p->unit * pps);
     float tslab = 0.0f;
     uint32_t lo = 0;
+    uint32_t window[16] = {0}, wsum = 0, nslab = 0;
     while (tslab < texit) {
       if (res[3] > 0.99f) break;
       if (!(tslab + sp > tslab)) break; /* unit below half an ulp of t: stop */
@@ -969,6 +974,13 @@ static void splat_pixel(const orc_params *p, const float *leaves /* 8 floats per
           const float *b = leaves + 8 * (uint64_t)(cand[k].p / ppl);
           if (aa_seg_box(o[0], o[1], o[2], inv[0], inv[1], inv[2], tmin, tmax, b[0], b[1], b[2], b[4], b[5], b[6]))
             slab[ns++] = cand[k];
+        }
+        if (hits) {
+          wsum += ns - window[nslab & 15u];
+          window[nslab & 15u] = ns;
+          ++nslab;
+          if (ns > hits[0]) hits[0] = ns;
+          if (wsum > hits[1]) hits[1] = wsum;
         }
         /* the payload keeps the first 15 hits in traversal order, then
          * SplatPayload::sort orders them by t (stable bubble sort) */
@@ -1016,8 +1028,9 @@ static void splat_pixel(const orc_params *p, const float *leaves /* 8 floats per
 
 /* Renders the given pixels (all when pixels == NULL) into image (W*H float4).
  * leaf_aabbs: the GenRBFBounds output (6 floats per leaf). */
+/* hits: NULL, or 2 counts per rendered pixel (see splat_pixel). */
 ORC_EXPORT int orc_splat(const orc_params *p, const float *leaf_aabbs, uint32_t num_leaves, uint32_t ppl,
-                         const int32_t *pixels, uint64_t npix, float *image) {
+                         const int32_t *pixels, uint64_t npix, float *image, uint32_t *hits) {
   if (ppl == 0) return -1;
   float *leaves = (float *)malloc(sizeof(float) * 8 * ((uint64_t)num_leaves + 1));
   splat_hit *cand = (splat_hit *)malloc(sizeof(splat_hit) * ((uint64_t)p->numParticles + 1));
@@ -1033,7 +1046,10 @@ ORC_EXPORT int orc_splat(const orc_params *p, const float *leaf_aabbs, uint32_t 
   for (uint64_t i = 0; i < n; ++i) {
     int px = pixels ? pixels[2 * i] : (int)(i % (uint64_t)p->width);
     int py = pixels ? pixels[2 * i + 1] : (int)(i / (uint64_t)p->width);
-    splat_pixel(p, leaves, num_leaves, ppl, px, py, cand, slab, image + 4 * ((uint64_t)px + (uint64_t)p->width * py));
+    uint32_t *h = hits ? hits + 2 * i : NULL;
+    if (h) h[0] = h[1] = 0;
+    splat_pixel(p, leaves, num_leaves, ppl, px, py, cand, slab, image + 4 * ((uint64_t)px + (uint64_t)p->width * py),
+                h);
   }
   free(leaves);
   free(cand);

@@ -3,7 +3,8 @@
   - the reference's own radius model (pc.radii = 0: radius = rbfRadius *
     radiusmap(w), deviceCodeRBF.hlsl:349-351) with a non-constant
     radiusmap, the transfer functions uploaded as 64 RGBA8 sRGB texels
-    through aarbf_tf_create_rgba8 (hostCode.cpp:355-365,524-572), the
+    (hostCode.cpp:355-365,524-572) through aarbf_tf_create_from_marks from
+    the reference's default marks and through aarbf_tf_create_rgba8, the
     default light (0, 0, 0.5) and unit = 0.1 rbfRadius (:481-486), one and
     four particles per leaf, both render schedules, both trackers;
   - tests/cpp/dropin_snippet: a C++ host program written the way
@@ -60,7 +61,11 @@ def test_reference_radius_model_and_rgba8_tfs(gpu_ctx, tracker, ppl):
     unit = F32(rbf * F32(0.1))
     xyzw = _scene(2500, 31 + ppl)
     cm8, rm8, dm8 = _texels()
-    handles = [gpu_ctx.tf_rgba8(t, srgb=True) for t in (cm8, rm8, dm8)]
+    # colormap and densitymap from the reference's default marks through
+    # the library's mark pipeline, the radiusmap from its RGBA8 bytes
+    handles = [gpu_ctx.tf_from_marks(scenes.REFERENCE_COLORMAP_MARKS, scenes.TF_COLORMAP),
+               gpu_ctx.tf_rgba8(rm8, srgb=True),
+               gpu_ctx.tf_from_marks(scenes.REFERENCE_DENSITYMAP_MARKS, scenes.TF_SCALAR)]
     tables = [scenes.srgb_decode_table(t) for t in (cm8, rm8, dm8)]
     try:
         sc = aarbf.Scene.prepare(gpu_ctx, xyzw, *handles, float(rbf), radii=None, ppl=ppl)

@@ -77,3 +77,28 @@ def test_time_series_refit(gpu_ctx, tmp_path, ppl):
         ts.set_step(k, rebuild=True)
         img2, acc2 = _render(gpu_ctx, ts.scene, cam, W, H, frame=k)
         assert np.array_equal(acc2, acc) and np.array_equal(img2, img)
+
+
+def test_import_step_device_streams_the_file(gpu_ctx, tmp_path):
+    """aarbf_import_step_device reads a timestep straight into device memory
+    (chunks larger than one staging buffer included): the same particles as
+    the host importer; a buffer too small for the step is refused with the
+    count reported; a step the header does not list is refused."""
+    import torch
+    from aarbf import series
+    rng = np.random.default_rng(4)
+    steps = [("a", rng.normal(size=(1000, 4)).astype(np.float32)),
+             ("b", rng.normal(size=(4_500_000, 4)).astype(np.float32))]  # 72 MB: two staging chunks
+    path = str(tmp_path / "particles.txt")
+    series.export_particles(path, steps)
+    for k, (_, a) in enumerate(steps):
+        got = series.import_step_device(gpu_ctx, path, k)
+        assert np.array_equal(got.cpu().numpy(), a), f"step {k}"
+    buf = torch.zeros((5_000_000, 4), dtype=torch.float32, device="cuda")
+    got = series.import_step_device(gpu_ctx, path, 1, out=buf)
+    assert got.data_ptr() == buf.data_ptr() and np.array_equal(got.cpu().numpy(), steps[1][1])
+    small = torch.zeros((999, 4), dtype=torch.float32, device="cuda")
+    with pytest.raises(aarbf.AarbfError, match="more particles than the buffer"):
+        series.import_step_device(gpu_ctx, path, 0, out=small)
+    with pytest.raises(aarbf.AarbfError, match="fewer timesteps"):
+        series.import_step_device(gpu_ctx, path, 2, out=buf)

@@ -22,7 +22,7 @@ def _compare(got, exp, what):
     assert np.array_equal(got, exp), f"{what}: within {RTOL} but not bit-exact"
 
 
-def _splat_both(ctx, xyzw, radii, tfs, rbf, unit, W, H, cam, ppl=1, K=8, vis=True):
+def _splat_both(ctx, xyzw, radii, tfs, rbf, unit, W, H, cam, ppl=1, K=8, vis=True, hits=False):
     import torch
     cm, rm, dm = tfs
     sc = aarbf.Scene.prepare(ctx, xyzw, cm, rm, dm, rbf, radii=radii, ppl=ppl, branching=K)
@@ -36,6 +36,9 @@ def _splat_both(ctx, xyzw, radii, tfs, rbf, unit, W, H, cam, ppl=1, K=8, vis=Tru
     leaves = oracle.gen_bounds(op, ppl, rbf, rm, radii=orad)
     assert np.array_equal(leaves, sc.aabbs.cpu().numpy(), equal_nan=True)
     r = oracle.Render(op, orad, sc.aabb, cm, rm, dm, cam, W, H, unit, rbf, visualize_attributes=vis)
+    if hits:
+        oimg, h = r.splat(leaves, ppl, hits=True)
+        return img.cpu().numpy(), oimg, h
     return img.cpu().numpy(), r.splat(leaves, ppl)
 
 
@@ -46,6 +49,8 @@ CASES = [
     ("global-radius-K2", 2000, 1, 2, False, True),
     ("density-colour-ppl2-K16", 2500, 2, 16, True, False),
     ("ppl1-K16", 2800, 1, 16, True, True),
+    # rbfRadius at least the largest radius: the splat clip box holds every sphere
+    ("rbf-covers-radii", 2600, 1, 8, True, True),
 ]
 
 
@@ -57,15 +62,19 @@ def test_splat_matches_oracle(gpu_ctx, name, n, ppl, K, per, vis):
     if not vis:
         tfs = (scenes.reference_tf("colormap"), scenes.reference_tf("radiusmap"), scenes.reference_tf("densitymap"))
     cam = default_camera(W, H, look_from=(0.4, 0.3, 2.2))
-    img, oimg = _splat_both(gpu_ctx, xyzw, radii, tfs, 0.06, 0.005, W, H, cam, ppl, K, vis)
+    rbf = float(radii.max()) if name == "rbf-covers-radii" else 0.06
+    img, oimg = _splat_both(gpu_ctx, xyzw, radii, tfs, rbf, 0.005, W, H, cam, ppl, K, vis)
     assert (oimg[..., :3] > 0).mean() > 0.05, "scene too empty to be a meaningful check"
     _compare(img, oimg, name)
 
 
 def test_splat_crowded_slabs(gpu_ctx):
-    """A tight cluster: many slabs have more than 15 hits and the per-chunk
-    hit lists overflow, so the kernel's chunk halving and the first-15 rule
-    both run."""
+    """A tight cluster where slabs hold more than 15 hits (the first-15
+    rule), some single slab holds more than the kernel's 32-entry hit list
+    (a one-slab chunk keeps its first 32), and runs of 16 slabs hold more
+    than 32 (a walk over several slabs stops at its first hit past the
+    list and the chunk shrinks to a quarter): the oracle's hit counts show
+    each case is reached."""
     W, H = 32, 32
     rng = np.random.default_rng(21)
     n = 5000
@@ -75,8 +84,11 @@ def test_splat_crowded_slabs(gpu_ctx):
     radii = rng.uniform(0.05, 0.1, n).astype(np.float32)
     tfs = (scenes.ramp_table(256), scenes.constant_table(n=4), scenes.constant_table((0.05, 0, 0, 0), n=1))
     cam = default_camera(W, H, look_from=(0.0, 0.2, 1.2))
-    img, oimg = _splat_both(gpu_ctx, xyzw, radii, tfs, 0.1, 0.01, W, H, cam)
+    img, oimg, hits = _splat_both(gpu_ctx, xyzw, radii, tfs, 0.1, 0.01, W, H, cam, hits=True)
     assert (oimg[..., :3] > 0).mean() > 0.1
+    assert hits[:, 0].max() > 32, f"no slab with more than 32 hits (largest {hits[:, 0].max()})"
+    assert ((hits[:, 1] > 32) & (hits[:, 0] <= 32)).any(), "no run of slabs overflowing the list without one slab doing so"
+    assert (hits[:, 0] > 15).mean() > 0.05
     _compare(img, oimg, "crowded")
 
 

@@ -413,3 +413,29 @@ def test_taa_fixed_point_and_clamp():
     taa = np.zeros_like(img)
     oracle.taa_resolve(img, taa, prev)
     assert taa[..., :3].max() < 0.7
+
+
+def test_detmath_coefficients_are_the_committed_fits():
+    """The polynomial coefficients in include/aarbf_detmath.h are exactly
+    what tests/golden/fit_detmath.py computes."""
+    import importlib.util
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("fit_detmath", os.path.join(root, "tests", "golden", "fit_detmath.py"))
+    fit = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(fit)
+    text = open(os.path.join(root, "include", "aarbf_detmath.h")).read()
+
+    def horner(fn):  # the literals of the Horner chain in the function's body
+        body = text[text.index(f"AA_FN float {fn}("):]
+        body = body[:body.index("\n}\n")]
+        start = re.search(r"float (?:q|p) = (-?[0-9.e-]+)f;", body)
+        rest = re.findall(r"AA_FMAF\((?:q|p), (?:f|r), (-?[0-9.e-]+)f\)", body)
+        return np.array([float(start.group(1))] + [float(v) for v in rest], np.float32)
+
+    log_q, exp_p = fit.log_coefficients()[::-1], fit.exp_coefficients()[::-1]
+    for fn in ("aa_logf", "aa_log_step"):
+        assert np.array_equal(horner(fn), log_q), fn
+    for fn in ("aa_expf", "aa_expf_mid"):
+        assert np.array_equal(horner(fn), exp_p), fn

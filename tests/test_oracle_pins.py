@@ -504,3 +504,70 @@ def test_camera_controller_closed_forms():
 @pytest.mark.parametrize("k", range(len(CONTROLLER_MUTANTS)))
 def test_camera_controller_catches_planted_mistakes(tmp_path, k):
     assert _fails(check_camera_controller, _host(host_mutant(tmp_path, "camera.cpp", *CONTROLLER_MUTANTS[k])))
+
+
+# ---- transfer-function editing: marks -> 64 RGBA8 texels -------------------------------
+
+def _texels(lib, marks, kind):
+    arr = (L.TFMark * len(marks))()
+    for k, (pos, rgba) in enumerate(marks):
+        arr[k].position = pos
+        arr[k].rgba[:] = list(rgba)
+    out = np.zeros((64, 4), np.uint8)
+    fn = lib.aarbf_tf_texels_from_marks
+    fn.restype, fn.argtypes = L.SIGNATURES["aarbf_tf_texels_from_marks"]
+    assert fn(arr, len(marks), kind, out.ctypes.data_as(C.POINTER(C.c_uint8))) == 0
+    return out
+
+
+def check_tf_texels(lib):
+    """Known answers of hostCode.cpp:524-572's texel writes."""
+    # a grey ramp: texel i samples i / 63; make_8bit truncates f * 256 and
+    # clamps (1.0 -> 256 -> 255); the colormap gamma-encodes RGB
+    ramp = [(0.0, (0, 0, 0, 0)), (1.0, (1, 1, 1, 1))]
+    t = _texels(lib, ramp, 0)
+    assert t[0].tolist() == [0, 0, 0, 0] and t[63].tolist() == [255, 255, 255, 255]
+    # i = 21: 1/3 -> alpha int(85.33) = 85; rgb (1/3)^(1/2.2) = 0.6069 -> 155
+    assert t[21].tolist() == [155, 155, 155, 85], t[21]
+    # i = 63 * 0.5 is not a texel; i = 9: 1/7 -> alpha 36, rgb 0.4129 -> 105
+    assert t[9].tolist() == [105, 105, 105, 36], t[9]
+    s = _texels(lib, ramp, 1)  # a scalar map: red only, linear
+    assert s[21].tolist() == [85, 0, 0, 0] and s[9].tolist() == [36, 0, 0, 0] and s[63].tolist() == [255, 0, 0, 0]
+    # the reference's default colormap: blue at 0, cyan-ish at 0.25, ..., red at 1
+    cm = _texels(lib, [(0.0, (0, 0, 1, 1)), (0.25, (0, 0.972549, 1, 1)), (0.5, (0, 1, 0.00784314, 1)),
+                       (0.75, (0.996078, 1, 0, 1)), (1.0, (1, 0, 0, 1))], 0)
+    assert cm[0].tolist() == [0, 0, 255, 255] and cm[63].tolist() == [255, 0, 0, 255]
+    # marks in any order; a single mark is a constant table; outside the
+    # marks the end colours hold
+    shuffled = _texels(lib, [(1.0, (1, 0, 0, 1)), (0.0, (0, 0, 1, 1)), (0.75, (0.996078, 1, 0, 1)),
+                             (0.5, (0, 1, 0.00784314, 1)), (0.25, (0, 0.972549, 1, 1))], 0)
+    assert np.array_equal(shuffled, cm)
+    one = _texels(lib, [(0.3, (0.75, 0.5, 0.5, 0.5))], 1)
+    assert np.all(one[:, 0] == 192) and not one[:, 1:].any()  # int(0.75 * 256), not round(0.75 * 255) = 191
+    step = _texels(lib, [(0.5, (0, 0, 0, 0)), (0.6, (1, 1, 1, 1))], 1)
+    assert np.all(step[:32, 0] == 0) and np.all(step[38:, 0] == 255)
+    # i = 35: 35/63 = 0.5556 -> t = 0.5556 -> int(142.2) = 142
+    assert step[35, 0] == 142, step[35]
+
+
+TF_MUTANTS = [
+    ("for (int k = 0; k < 3; ++k) t[k] = make_8bit(std::pow(c[k], 1.0f / 2.2f));",
+     "for (int k = 0; k < 3; ++k) t[k] = make_8bit(std::pow(c[k], 2.2f));"),
+    ("uint8_t make_8bit(float f) { return (uint8_t)std::min(255, std::max(0, (int)(f * 256.0f))); }",
+     "uint8_t make_8bit(float f) { return (uint8_t)std::min(255, std::max(0, (int)(f * 255.0f + 0.5f))); }"),
+    ("gradient_at(m, (float)i / 63.0f, c);", "gradient_at(m, (float)i / 64.0f, c);"),
+    ("for (int c = 0; c < 4; ++c) out[c] = a.rgba[c] + t * (b.rgba[c] - a.rgba[c]);",
+     "for (int c = 0; c < 4; ++c) out[c] = b.rgba[c] + t * (a.rgba[c] - b.rgba[c]);"),
+]
+
+
+def test_tf_texels_known_answers():
+    check_tf_texels(_host())
+    # and the Python presets are these texels
+    assert np.array_equal(scenes.reference_tf_texels("colormap"),
+                          _texels(_host(), scenes.REFERENCE_COLORMAP_MARKS, 0))
+
+
+@pytest.mark.parametrize("k", range(len(TF_MUTANTS)))
+def test_tf_texels_catch_planted_mistakes(tmp_path, k):
+    assert _fails(check_tf_texels, C.CDLL(host_mutant(tmp_path, "tf_marks.cpp", *TF_MUTANTS[k])))
