@@ -36,7 +36,7 @@ namespace {
 constexpr int kDBlock = 64;
 constexpr uint32_t kTexMax = 256; // texels per table with LDS gradient sums (larger tables: global atomics)
 #ifndef AARBF_DIFF_REGCACHE
-#define AARBF_DIFF_REGCACHE 1 // a one-entry adjoint cache in registers (0: the LDS cache of kCache entries)
+#define AARBF_DIFF_REGCACHE 1 // a four-entry adjoint cache in registers (0: the LDS cache of kCache entries)
 #endif
 #ifndef AARBF_DIFF_CACHE
 #define AARBF_DIFF_CACHE 4
@@ -62,10 +62,12 @@ struct DiffArgs {
   uint32_t rm_n;
   uint32_t spp;
   float4 *out;             // image (forward)
+  float4 *colours;         // per sample: its colour (forward), averaged into `out` by diff_fold
   float4 *path;            // per sample: sum_k F'(L_k) (c_k - c_{k+1}) (forward writes, adjoint reads), or null
   const float4 *gimg;      // dLoss/dimage (adjoint)
   float *g_pos, *g_rad, *g_attr, *g_cm, *g_dm, *g_rm;
   float *acc5;             // adjoint: per particle (r, alpha^3, rgb^2.2) sums, before the TF chain rule
+  const float2 *pm;        // per particle: m = -4.5 / r^2 and 1 / r (0, 0 for r = 0)
   uint32_t ratio;          // estimator: 0 expected marching, 1 ratio tracking
 };
 
@@ -75,25 +77,36 @@ constexpr float kRatioTmin = 9.5367431640625e-07f;
 // The expected image and its adjoint are compared with the FP64 oracle to a
 // tolerance, not bit for bit, so the march's transcendentals use the
 // hardware's exp2 / log2 (about 1 ulp) instead of the renderer's
-// deterministic polynomials.
+// deterministic polynomials.  The RBF weight of a particle at a point is
+// exp(e), e = |x - P|^2 m with the baked m = -4.5 / r^2 (the particle covers
+// the point where e > -4.5): no square root and no division per particle
+// and point; its derivatives are d exp(e) / dP = 2 m exp(e) (P - x) and
+// d exp(e) / dr = -2 e / r exp(e).
 __device__ inline float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 __device__ inline float fpow(float m, float y) { return m > 0.0f ? __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(m)) : 0.0f; }
-__device__ inline float frbf(float d, float r) {
-  const float q = (d * 3.0f) / r;
-  return fexp(-0.5f * (q * q));
-}
 
 __global__ __launch_bounds__(256) void diff_bake(const float4 *__restrict__ particles, const float *__restrict__ radii,
                                                  uint32_t n, const float4 *__restrict__ rmap, uint32_t rm_n,
                                                  const float4 *__restrict__ cmap, uint32_t cm_n,
-                                                 float4 *__restrict__ pr, float4 *__restrict__ pc) {
+                                                 float4 *__restrict__ pr, float4 *__restrict__ pc,
+                                                 float2 *__restrict__ pm) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float4 q = particles[i];
   float4 c = tf4(cmap, cm_n, q.w);
-  float radius = radii[i] * tf_r(rmap, rm_n, q.w);
-  pr[i] = make_float4(q.x, q.y, q.z, c.w == 0.0f ? 0.0f : radius);
+  float radius = c.w == 0.0f ? 0.0f : radii[i] * tf_r(rmap, rm_n, q.w);
+  pr[i] = make_float4(q.x, q.y, q.z, radius);
   pc[i] = make_float4(aa_cube(c.w), aa_powf(c.x, 2.2f), aa_powf(c.y, 2.2f), aa_powf(c.z, 2.2f));
+  pm[i] = radius > 0.0f ? make_float2(-4.5f / (radius * radius), 1.0f / radius) : make_float2(0.0f, 0.0f);
+}
+
+// e = |x - P|^2 m of particle p at x, and whether p covers x (e > -4.5;
+// a radius of 0 never covers)
+__device__ inline bool rbf_exponent(const DiffArgs &D, uint32_t p, const float4 &q, float x, float y, float z,
+                                    float &e) {
+  const float m = D.pm[p].x;
+  e = aa_dist2(q.x, q.y, q.z, x, y, z) * m;
+  return m < 0.0f && e > -4.5f;
 }
 
 // value, slope and texel weights of a linearly filtered table channel
@@ -106,20 +119,22 @@ __device__ inline float tf_grad(const float4 *t, uint32_t n, int c, float u, flo
 }
 
 __device__ inline float Fcdf(float L, float &dF) {
+  constexpr float inv = 1.0f / 0.99f;
   float e = fexp(-L);
-  float v = (1.0f - e) / 0.99f;
+  float v = (1.0f - e) * inv;
   if (v >= 1.0f) {
     dF = 0.0f;
     return 1.0f;
   }
-  dF = e / 0.99f;
+  dF = e * inv;
   return v;
 }
 
 // every particle covering x, in ascending order (the stackless walk of
-// point_query with a callback)
+// point_query with a callback): fn(p, baked position and radius, e)
 template <class Fn>
-__device__ inline void for_covering(const RenderArgs &A, float x, float y, float z, Fn fn) {
+__device__ inline void for_covering(const DiffArgs &D, float x, float y, float z, Fn fn) {
+  const RenderArgs &A = D.A;
   const float4 *__restrict__ N = A.bvh.nodes;
   const uint32_t log2K = A.bvh.log2K, kmask = (1u << log2K) - 1u, top = A.bvh.top;
   uint32_t level = top, idx = 0;
@@ -139,8 +154,8 @@ __device__ inline void for_covering(const RenderArgs &A, float x, float y, float
         uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
         for (uint32_t p = p0; p < p1; ++p) {
           float4 q = A.pr[p];
-          float d = aa_dist3(q.x, q.y, q.z, x, y, z);
-          if (d < q.w) fn(p, q, d);
+          float e;
+          if (rbf_exponent(D, p, q, x, y, z, e)) fn(p, q, e);
         }
       }
     }
@@ -167,13 +182,20 @@ struct DPay {
 struct DSeg {
   uint32_t *idx;
   float *tin, *tout;
-  uint32_t *stk; // per-level words of the walks (visit order / hit masks)
+  uint32_t *stk;  // per-level words of the walks (visit order / hit masks)
+  uint32_t *list; // the current point's covering particles (kList words), or null
 };
 
+// Covering particles a point's payload walk records for the adjoint's
+// scatter, which then reads them back instead of walking the tree again
+// (a point covered by more walks the tree again).
+constexpr uint32_t kList = 24;
+
 // The march of sample s of pixel (px, py): fn(x, y, z, payload, cover) at
-// every point some particle covers, where cover(g) calls g(p, q, d) for
-// each covering particle p (baked q, distance d) in the order the payload
-// summed them; fn returns false to end the march.
+// every point some particle covers, where cover(g) calls g(p, q, e) for
+// each covering particle p (baked q, RBF exponent e) in the order the
+// payload summed them; fn returns false to end the march.  With W.list set,
+// cover() reads the particles the payload walk recorded there.
 //
 // Points are t_k = t_{k-1} + unit (float recurrence from tenter + unit r1).
 // Which particles can cover them comes from a segment gather: one walk of
@@ -376,7 +398,7 @@ __device__ inline void march(const DiffArgs &D, const DSeg &W, int px, int py, u
     auto cover = [&](auto g) {
       if (dense) {
         if (!seg_ok) {
-          for_covering(A, x, y, z, g);
+          for_covering(D, x, y, z, g);
           return;
         }
         // point query over the quantised tree, leaves in ascending order
@@ -389,8 +411,8 @@ __device__ inline void march(const DiffArgs &D, const DSeg &W, int px, int py, u
                 const uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
                 for (uint32_t p = p0; p < p1; ++p) {
                   const float4 q = A.pr[p];
-                  const float dd = aa_dist3(q.x, q.y, q.z, x, y, z);
-                  if (dd < q.w) g(p, q, dd);
+                  float e;
+                  if (rbf_exponent(D, p, q, x, y, z, e)) g(p, q, e);
                 }
               }
               return false;
@@ -401,45 +423,83 @@ __device__ inline void march(const DiffArgs &D, const DSeg &W, int px, int py, u
       for (uint32_t b = act; b; b &= b - 1u) {
         const uint32_t p = W.idx[((uint32_t)__ffs(b) - 1u) * kDBlock];
         const float4 q = A.pr[p];
-        const float dd = aa_dist3(q.x, q.y, q.z, x, y, z);
-        if (dd < q.w) g(p, q, dd);
+        float e;
+        if (rbf_exponent(D, p, q, x, y, z, e)) g(p, q, e);
       }
     };
     DPay pl;
     pl.count = 0;
     pl.rho = pl.S[0] = pl.S[1] = pl.S[2] = 0.0f;
-    cover([&](uint32_t p, const float4 &q, float dd) {
+    cover([&](uint32_t p, const float4 &, float e) {
       const float4 k = A.pc[p];
-      const float w = frbf(dd, q.w) * k.x;
+      const float w = fexp(e) * k.x;
+      if (W.list && pl.count < kList) W.list[pl.count * kDBlock] = p;
       pl.count += 1;
       pl.rho += w;
       pl.S[0] += k.y * w;
       pl.S[1] += k.z * w;
       pl.S[2] += k.w * w;
     });
-    if (pl.count > 0 && pl.rho > 0.0f)
-      if (!fn(x, y, z, pl, cover)) return;
+    if (pl.count > 0 && pl.rho > 0.0f) {
+      auto recorded = [&](auto g) {
+        if (pl.count > kList) {
+          cover(g);
+          return;
+        }
+        for (uint32_t i = 0; i < pl.count; ++i) {
+          const uint32_t p = W.list[i * kDBlock];
+          const float4 q = A.pr[p];
+          g(p, q, aa_dist2(q.x, q.y, q.z, x, y, z) * D.pm[p].x);
+        }
+      };
+      if (W.list) {
+        if (!fn(x, y, z, pl, recorded)) return;
+      } else if (!fn(x, y, z, pl, cover)) {
+        return;
+      }
+    }
   }
 }
 
+// the point's colour (S / rho)^(1/2.2); rho > 0 at every point the march
+// reports
 __device__ inline void colour(const DPay &pl, float c[3]) {
-  for (int k = 0; k < 3; ++k) c[k] = fpow(pl.S[k] / pl.rho, 1.0f / 2.2f);
+  const float inv_rho = __builtin_amdgcn_rcpf(pl.rho);
+  for (int k = 0; k < 3; ++k) c[k] = fpow(pl.S[k] * inv_rho, 1.0f / 2.2f);
 }
 
-#define AARBF_DSEG_LDS                                                                                      \
+#define AARBF_DSEG_LDS(list)                                                                                \
   __shared__ uint32_t s_cidx[kDC * kDBlock];                                                                  \
   __shared__ float s_ctin[kDC * kDBlock], s_ctout[kDC * kDBlock];                                             \
   extern __shared__ uint32_t s_stk[]; /* walk stacks, max(top, 2) words per lane (sized at launch) */         \
-  const DSeg W{s_cidx + threadIdx.x, s_ctin + threadIdx.x, s_ctout + threadIdx.x, s_stk + threadIdx.x};
+  const DSeg W{s_cidx + threadIdx.x, s_ctin + threadIdx.x, s_ctout + threadIdx.x, s_stk + threadIdx.x, list};
+
+// One lane per (pixel, sample): the samples of a pixel sit on neighbouring
+// lanes (they march the same ray from different offsets, so the wave stays
+// coherent), and a pixel whose ray crosses a dense cluster spreads its cost
+// over spp lanes instead of running spp samples in a row.  Pixels are
+// visited in 8 x 8 tiles.  Returns false for lanes past the image.
+__device__ inline bool lane_sample(const DiffArgs &D, int &px, int &py, uint32_t &s) {
+  const RenderArgs &A = D.A;
+  const uint64_t g = (uint64_t)blockIdx.x * kDBlock + threadIdx.x;
+  s = (uint32_t)(g % D.spp);
+  const uint64_t p = g / D.spp;
+  const uint32_t tiles_x = ((uint32_t)A.W + 7u) / 8u;
+  const uint64_t tile = p >> 6;
+  const uint32_t in = (uint32_t)(p & 63u);
+  px = (int)((uint32_t)(tile % tiles_x) * 8u + (in & 7u));
+  py = (int)((uint32_t)(tile / tiles_x) * 8u + (in >> 3));
+  return px < A.W && py < A.H;
+}
 
 __global__ __launch_bounds__(kDBlock) void diff_forward(DiffArgs D) {
   const RenderArgs &A = D.A;
-  AARBF_DSEG_LDS
+  AARBF_DSEG_LDS(nullptr)
   load_levels(A);
-  const int px = (int)(blockIdx.x * 8u + (threadIdx.x & 7u)), py = (int)(blockIdx.y * 8u + (threadIdx.x >> 3));
-  if (px >= A.W || py >= A.H) return;
-  float acc[3] = {0.0f, 0.0f, 0.0f};
-  for (uint32_t s = 0; s < D.spp; ++s) {
+  int px, py;
+  uint32_t s;
+  if (!lane_sample(D, px, py, s)) return;
+  {
     float L = 0.0f, C[3] = {0.0f, 0.0f, 0.0f}, V[3] = {0.0f, 0.0f, 0.0f}, prev[3] = {0.0f, 0.0f, 0.0f};
     float T = 1.0f;
     bool have = false;
@@ -474,12 +534,26 @@ __global__ __launch_bounds__(kDBlock) void diff_forward(DiffArgs D) {
       Fcdf(L, dF);
       for (int k = 0; k < 3; ++k) V[k] += dF * prev[k];
     }
-    if (D.path)
-      D.path[((uint64_t)py * A.W + px) * D.spp + s] = make_float4(V[0], V[1], V[2], 0.0f);
-    for (int k = 0; k < 3; ++k) acc[k] += C[k];
+    const uint64_t at = ((uint64_t)py * A.W + px) * D.spp + s;
+    if (D.path) D.path[at] = make_float4(V[0], V[1], V[2], 0.0f);
+    D.colours[at] = make_float4(C[0], C[1], C[2], 0.0f);
+  }
+}
+
+// The pixel: the mean of its samples, summed in sample order.
+__global__ __launch_bounds__(256) void diff_fold(DiffArgs D) {
+  const RenderArgs &A = D.A;
+  const uint64_t p = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (p >= (uint64_t)A.W * A.H) return;
+  float acc[3] = {0.0f, 0.0f, 0.0f};
+  for (uint32_t s = 0; s < D.spp; ++s) {
+    const float4 c = D.colours[p * D.spp + s];
+    acc[0] += c.x;
+    acc[1] += c.y;
+    acc[2] += c.z;
   }
   const float inv = 1.0f / (float)D.spp;
-  D.out[(uint64_t)px + (uint64_t)A.W * py] = make_float4(acc[0] * inv, acc[1] * inv, acc[2] * inv, 1.0f);
+  D.out[p] = make_float4(acc[0] * inv, acc[1] * inv, acc[2] * inv, 1.0f);
 }
 
 __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
@@ -488,7 +562,8 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
 #if !AARBF_DIFF_REGCACHE
   __shared__ uint32_t s_cache[kCache * kSlot * kDBlock];
 #endif
-  AARBF_DSEG_LDS
+  __shared__ uint32_t s_list[kList * kDBlock];
+  AARBF_DSEG_LDS(s_list + threadIdx.x)
   load_levels(A);
   for (uint32_t j = threadIdx.x; j < kTexMax; j += kDBlock) s_gdm[j] = 0.0f;
   __syncthreads();
@@ -501,33 +576,56 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
   // adjoints (position, radius, alpha^3, colour^2.2) are summed in a small
   // per-lane cache and added to global memory when another particle takes
   // the entry; the chain rule through the radiusmap / colormap lookups
-  // follows per particle in diff_chain.  The one-entry cache in registers
-  // leaves the LDS to the march and runs at 4 waves per SIMD (the 4-entry
-  // LDS cache at 2); measured 1-2 % faster.
+  // follows per particle in diff_chain.  A few particles cover each point
+  // and the next point's are mostly the same, so a four-entry associative
+  // cache in registers keeps most of them (a one-entry cache flushed on 83 %
+  // of the updates); the LDS cache is the alternative build.
 #if AARBF_DIFF_REGCACHE
-  // one entry in registers (static indices only)
-  uint32_t rc_p = 0xffffffffu;
-  float rc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  auto flush_all = [&]() {
-    if (rc_p == 0xffffffffu) return;
-    atomicAdd(&D.g_pos[3 * rc_p], rc[0]);
-    atomicAdd(&D.g_pos[3 * rc_p + 1], rc[1]);
-    atomicAdd(&D.g_pos[3 * rc_p + 2], rc[2]);
+  // four entries in registers, fully associative (static indices only),
+  // first in first out
+  uint32_t ck[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
+  float cv[4][8];
+  auto flush_e = [&](const uint32_t p, const float (&v)[8]) {
+    atomicAdd(&D.g_pos[3 * p], v[0]);
+    atomicAdd(&D.g_pos[3 * p + 1], v[1]);
+    atomicAdd(&D.g_pos[3 * p + 2], v[2]);
 #pragma unroll
-    for (uint32_t w = 0; w < 5; ++w) atomicAdd(&D.acc5[5 * (uint64_t)rc_p + w], rc[3 + w]);
-    rc_p = 0xffffffffu;
+    for (uint32_t w = 0; w < 5; ++w) atomicAdd(&D.acc5[5 * (uint64_t)p + w], v[3 + w]);
+  };
+  auto flush_all = [&]() {
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (ck[e] != 0xffffffffu) {
+        flush_e(ck[e], cv[e]);
+        ck[e] = 0xffffffffu;
+      }
   };
   auto accumulate = [&](uint32_t p, const float (&v)[8]) {
-    if (rc_p != p) {
-      flush_all();
-      rc_p = p;
+    bool hit = false;
 #pragma unroll
-      for (int w = 0; w < 8; ++w) rc[w] = 0.0f;
+    for (int e = 0; e < 4; ++e)
+      if (ck[e] == p) {
+        hit = true;
+#pragma unroll
+        for (int w = 0; w < 8; ++w) cv[e][w] += v[w];
+      }
+    if (!hit) { // the oldest entry leaves, the others move down one
+      if (ck[3] != 0xffffffffu) flush_e(ck[3], cv[3]);
+#pragma unroll
+      for (int e = 3; e > 0; --e) {
+        ck[e] = ck[e - 1];
+#pragma unroll
+        for (int w = 0; w < 8; ++w) cv[e][w] = cv[e - 1][w];
+      }
+      ck[0] = p;
+#pragma unroll
+      for (int w = 0; w < 8; ++w) cv[0][w] = v[w];
     }
-#pragma unroll
-    for (int w = 0; w < 8; ++w) rc[w] += v[w];
   };
-  auto reset_cache = [&]() { rc_p = 0xffffffffu; };
+  auto reset_cache = [&]() {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) ck[e] = 0xffffffffu;
+  };
 #else
   uint32_t *cache = s_cache + threadIdx.x; // [kCache][kSlot] words, stride kDBlock
   auto word = [&](uint32_t e, uint32_t w) -> float & { return *(float *)&cache[(e * kSlot + w) * kDBlock]; };
@@ -557,12 +655,13 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
     for (uint32_t e = 0; e < kCache; ++e) cache[e * kSlot * kDBlock] = 0xffffffffu;
   };
 #endif
-  const int px = (int)(blockIdx.x * 8u + (threadIdx.x & 7u)), py = (int)(blockIdx.y * 8u + (threadIdx.x >> 3));
-  if (px < A.W && py < A.H) {
+  int px, py;
+  uint32_t s;
+  if (lane_sample(D, px, py, s)) {
     const float4 gi = D.gimg[(uint64_t)px + (uint64_t)A.W * py];
     const float inv = 1.0f / (float)D.spp;
     const float g[3] = {gi.x * inv, gi.y * inv, gi.z * inv};
-    for (uint32_t s = 0; s < D.spp; ++s) {
+    do {
       reset_cache();
       // pass 1: the total of dC/dL_k over the path (kept by the forward when
       // it had a path buffer)
@@ -595,7 +694,7 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
         have = true;
         return Fcdf(L, dF) < 1.0f;
       });
-      if (!have) continue;
+      if (!have) break;
       if (!D.path && !D.ratio) {
         float dF;
         Fcdf(L, dF);
@@ -620,7 +719,7 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
           const float gc = g[0] * c[0] + g[1] * c[1] + g[2] * c[2];
           const float Tn = T * (1.0f - Dv);
           more = Tn >= kRatioTmin;
-          const float qn = more ? (q - Dv * gc) / (1.0f - Dv) : 0.0f;
+          const float qn = more ? (q - Dv * gc) * __builtin_amdgcn_rcpf(1.0f - Dv) : 0.0f;
           aD = T * (gc - qn);
           pj = T * Dv;
           T = Tn;
@@ -640,24 +739,26 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
         add_dm(j0, aD * (1.0f - fD));
         add_dm(j1, aD * fD);
         float a_rho = aD * sD, a_S[3];
+        const float inv_rho = __builtin_amdgcn_rcpf(pl.rho);
         for (int k = 0; k < 3; ++k) {
-          // d(m^(1/2.2))/dm = (1/2.2) m^(1/2.2) / m, with m^(1/2.2) = c[k]
-          const float gc = g[k] * pj, m = pl.S[k] / pl.rho;
-          const float dcdm = m > 0.0f ? (1.0f / 2.2f) * (c[k] / m) : 0.0f;
-          a_S[k] = gc * dcdm / pl.rho;
-          a_rho -= gc * dcdm * m / pl.rho;
+          // c = m^(1/2.2), m = S / rho: dc/dm = (1/2.2) c / m, so
+          // dc/dS = (1/2.2) c / S and dc/drho = -(1/2.2) c / rho (both 0
+          // where S = 0, and c = 0 there)
+          const float gcc = (g[k] * pj) * ((1.0f / 2.2f) * c[k]);
+          a_S[k] = pl.S[k] > 0.0f ? gcc * __builtin_amdgcn_rcpf(pl.S[k]) : 0.0f;
+          a_rho -= gcc * inv_rho;
         }
-        auto scatter = [&](uint32_t p, const float4 &q, float d) {
+        auto scatter = [&](uint32_t p, const float4 &q, float e) {
           const float4 k = A.pc[p];
-          const float qq = (d * 3.0f) / q.w;
-          const float rbf = fexp(-0.5f * (qq * qq)), w = rbf * k.x;
+          const float2 mi = D.pm[p];
+          const float rbf = fexp(e), w = rbf * k.x;
           const float a_w = a_rho + a_S[0] * k.y + a_S[1] * k.z + a_S[2] * k.w;
           const float a_rbf = a_w * k.x;
-          const float kk = d > 0.0f ? a_rbf * (-9.0f / (q.w * q.w) * rbf) : 0.0f; // d(rbf)/dd / d
+          const float kk = a_rbf * ((2.0f * mi.x) * rbf); // d(rbf)/dP = 2 m rbf (P - x)
           const float v[8] = {kk * (q.x - x), kk * (q.y - y), kk * (q.z - z),
-                              a_rbf * (9.0f * d * d / (q.w * q.w * q.w) * rbf), // a_r
-                              a_w * rbf,                                         // a_a3
-                              a_S[0] * w, a_S[1] * w, a_S[2] * w};               // a_c22
+                              a_rbf * (rbf * (-2.0f * e) * mi.y), // a_r: d(rbf)/dr = -2 e / r rbf
+                              a_w * rbf,                           // a_a3
+                              a_S[0] * w, a_S[1] * w, a_S[2] * w}; // a_c22
           accumulate(p, v);
         };
         cover(scatter);
@@ -668,7 +769,7 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
         return more;
       });
       flush_all();
-    }
+    } while (false);
   }
   __syncthreads();
   if (lds_dm)
@@ -764,13 +865,21 @@ static int diff_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const aar
                 "aarbf_diff: the BVH has fewer leaves than numParticles / particlesPerLeaf");
   hipSetDevice(ctx->device);
   const uint32_t n = pc->numParticles;
-  float4 *pr = (float4 *)scratch(ctx, (32 + 20) * (size_t)(n ? n : 1));
+  // scratch: baked payload (32 B per particle), adjoint sums (20 B per
+  // particle), RBF coefficients (8 B per particle), then the per-sample
+  // colours of the forward (16 B per sample)
+  const size_t part_bytes = ((32 + 20 + 8) * (size_t)(n ? n : 1) + 255) & ~(size_t)255;
+  const size_t samples = (size_t)W * (size_t)H * spp;
+  float4 *pr = (float4 *)scratch(ctx, part_bytes + 16 * samples);
   if (!pr) return set_error(ctx, AARBF_ERR_OUT_OF_MEMORY, "aarbf_diff: payload");
   D.acc5 = (float *)(pr + 2 * (size_t)(n ? n : 1));
+  D.pm = (const float2 *)(D.acc5 + 5 * (size_t)(n ? n : 1));
+  D.colours = (float4 *)((char *)pr + part_bytes);
   float4 *pcol = pr + (n ? n : 1);
   if (n)
     diff_bake<<<(n + 255) / 256, 256, 0, ctx->stream>>>((const float4 *)pc->particles, (const float *)pc->radii, n,
-                                                         rm->table, rm->n, cm->table, cm->n, pr, pcol);
+                                                         rm->table, rm->n, cm->table, cm->n, pr, pcol,
+                                                         (float2 *)D.pm);
   AARBF_CHECK_LAUNCH(ctx, "diff_bake");
   RenderArgs &A = D.A;
   A = RenderArgs{};
@@ -813,6 +922,13 @@ static size_t diff_stack_bytes(const DiffArgs &D) {
   return sizeof(uint32_t) * kDBlock * levels;
 }
 
+// Blocks of kDBlock lanes over the (pixel, sample) pairs of whole 8 x 8
+// tiles (lane_sample drops the pixels past the image).
+static uint32_t diff_blocks(int W, int H, uint32_t spp) {
+  const uint64_t tiles = (uint64_t)((W + 7) / 8) * (uint64_t)((H + 7) / 8);
+  return (uint32_t)((tiles * 64u * spp + kDBlock - 1) / kDBlock);
+}
+
 extern "C" {
 
 int aarbf_set_diff_estimator(aarbf_context *ctx, int estimator) {
@@ -832,9 +948,11 @@ int aarbf_diff_forward(aarbf_context *ctx, const aarbf_raygen_data *rg, const aa
   if (rc != AARBF_OK) return rc;
   D.out = (float4 *)image;
   D.path = (float4 *)path;
-  dim3 grid((W + 7) / 8, (H + 7) / 8);
-  diff_forward<<<grid, kDBlock, diff_stack_bytes(D), ctx->stream>>>(D);
+  diff_forward<<<diff_blocks(W, H, spp), kDBlock, diff_stack_bytes(D), ctx->stream>>>(D);
   AARBF_CHECK_LAUNCH(ctx, "diff_forward");
+  const uint64_t npix = (uint64_t)W * (uint64_t)H;
+  diff_fold<<<(uint32_t)((npix + 255) / 256), 256, 0, ctx->stream>>>(D);
+  AARBF_CHECK_LAUNCH(ctx, "diff_fold");
   return AARBF_OK;
 }
 
@@ -855,10 +973,9 @@ int aarbf_diff_backward(aarbf_context *ctx, const aarbf_raygen_data *rg, const a
   D.g_cm = (float *)g_colormap;
   D.g_dm = (float *)g_densitymap;
   D.g_rm = (float *)g_radiusmap;
-  dim3 grid((W + 7) / 8, (H + 7) / 8);
   const uint64_t n5 = 5 * (uint64_t)D.A.numParticles;
   if (n5) diff_zero<<<(uint32_t)((n5 + 255) / 256), 256, 0, ctx->stream>>>(D.acc5, n5);
-  diff_adjoint<<<grid, kDBlock, diff_stack_bytes(D), ctx->stream>>>(D);
+  diff_adjoint<<<diff_blocks(W, H, spp), kDBlock, diff_stack_bytes(D), ctx->stream>>>(D);
   AARBF_CHECK_LAUNCH(ctx, "diff_adjoint");
   if (D.A.numParticles)
     diff_chain<<<(D.A.numParticles + kCBlock - 1) / kCBlock, kCBlock, 0, ctx->stream>>>(D);
