@@ -84,6 +84,17 @@ __device__ inline float fx16_r(uint32_t q, float step, float origin) { return q 
 __device__ inline float un16(uint32_t q) { return (float)q * (1.0f / 65535.0f); }
 
 // Baked particle p: position and radius, and (alpha^3, rgb^2.2).
+// One free-flight draw of the delta tracker: advances the LCG state s and
+// returns log(1 - xi), bit for bit aa_lcg_next followed by
+// aa_log_step(1.0f - xi), with 1 - xi as one fused multiply-add:
+// xi = (float)s * 2^-32 is exact, so fma((float)s, -2^-32, 1) rounds the
+// same value as 1 - xi.  tests/cpp/flight_identity.c checks this over all
+// 2^32 states.
+__device__ inline float flight_log(uint32_t &s) {
+  s = s * 1664525u + 1013904223u;
+  return aa_log_step(__builtin_fmaf((float)s, -2.3283064365386963e-10f, 1.0f));
+}
+
 __device__ inline float4 ld_pr(const RenderArgs &A, uint32_t p) {
   if (!A.q16) return A.pr[p];
   const uint4 w = A.qp[p];

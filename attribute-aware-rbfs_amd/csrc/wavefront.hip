@@ -38,6 +38,8 @@
 // sample order, exactly as the sequential loop does.
 #include "render_common.h"
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -1155,41 +1157,37 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
       // them step in each iteration) and committed only where a step is
       // taken.  Marching steps a fixed distance and draws nothing.
       uint32_t ns = L.trng;
-      float tn;
-      if (is_march<B>(A, L)) {
-        tn = L.t + A.unit;
-      } else {
-        float xi = aa_lcg_next(&ns);
-        tn = L.t - aa_log_step(1.0f - xi) * A.unit;
-      }
+      const bool march = is_march<B>(A, L);
+      const float tn = march ? L.t + A.unit : L.t - flight_log(ns) * A.unit;
       const bool can = !pending && L.i < A.max_depth && steps < step_cap;
       if (can && act == 0u && tn < lim) {
         L.trng = ns;
         L.t = tn;
-        ++L.i;
-        ++steps;
-        if (STATS) c.steps++;
         // more draws in the same iteration, each committed only when it
         // too lands in empty space before the next event (the first that
-        // does not is drawn again by the next iteration)
+        // does not is drawn again by the next iteration), within the depth
+        // and step budgets left after this one
+        const uint32_t room = min(A.max_depth - (L.i + 1u), step_cap - (steps + 1u));
+        uint32_t taken = 1;
+        // one unrolled loop per tracker (the choice is uniform in a wave
+        // of one view)
+        auto more = [&](auto marching) {
 #pragma unroll
-        for (int u = 1; u < kFastSteps; ++u) {
-          if (!(L.i < A.max_depth && steps < step_cap)) break;
-          uint32_t ns2 = L.trng;
-          float tn2;
-          if (is_march<B>(A, L)) {
-            tn2 = L.t + A.unit;
-          } else {
-            float xi = aa_lcg_next(&ns2);
-            tn2 = L.t - aa_log_step(1.0f - xi) * A.unit;
+          for (int u = 1; u < kFastSteps; ++u) {
+            if ((uint32_t)u > room) break;
+            uint32_t ns2 = L.trng;
+            const float tn2 = decltype(marching)::value ? L.t + A.unit : L.t - flight_log(ns2) * A.unit;
+            if (!(tn2 < lim)) break;
+            L.trng = ns2;
+            L.t = tn2;
+            ++taken;
           }
-          if (!(tn2 < lim)) break;
-          L.trng = ns2;
-          L.t = tn2;
-          ++L.i;
-          ++steps;
-          if (STATS) c.steps++;
-        }
+        };
+        if (march) more(std::true_type{});
+        else more(std::false_type{});
+        L.i += taken;
+        steps += taken;
+        if (STATS) c.steps += taken;
         continue;
       }
       bool end = false;
