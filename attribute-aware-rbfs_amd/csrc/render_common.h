@@ -83,7 +83,6 @@ __device__ inline float fx16_r(uint32_t q, float step, float origin) { return q 
 // alpha as 16-bit unorm
 __device__ inline float un16(uint32_t q) { return (float)q * (1.0f / 65535.0f); }
 
-// Baked particle p: position and radius, and (alpha^3, rgb^2.2).
 // One free-flight draw of the delta tracker: advances the LCG state s and
 // returns log(1 - xi), bit for bit aa_lcg_next followed by
 // aa_log_step(1.0f - xi), with 1 - xi as one fused multiply-add:
@@ -95,6 +94,7 @@ __device__ inline float flight_log(uint32_t &s) {
   return aa_log_step(__builtin_fmaf((float)s, -2.3283064365386963e-10f, 1.0f));
 }
 
+// Baked particle p: position and radius, and (alpha^3, rgb^2.2).
 __device__ inline float4 ld_pr(const RenderArgs &A, uint32_t p) {
   if (!A.q16) return A.pr[p];
   const uint4 w = A.qp[p];
