@@ -198,6 +198,7 @@ class Context:
     TUNE_STEP_CAP = 2
     TUNE_GRAPH_ROUNDS = 3
     TUNE_WALK_BUDGET = 4
+    TUNE_DIFF_WAVES = 5
 
     def set_tuning(self, param: int, value: int):
         """Schedule parameters of the default renderer (images do not change)."""

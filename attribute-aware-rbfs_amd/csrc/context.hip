@@ -188,6 +188,10 @@ int aarbf_set_tuning(aarbf_context *ctx, int param, int value) {
     AARBF_REQUIRE(ctx, value >= 1 && value <= 65536, "aarbf_set_tuning: walk budget must be in [1, 65536]");
     ctx->tune_walk_budget = value;
     return AARBF_OK;
+  case AARBF_TUNE_DIFF_WAVES:
+    AARBF_REQUIRE(ctx, value == 0 || value == 1, "aarbf_set_tuning: diff waves must be 0 or 1");
+    ctx->tune_diff_waves = value;
+    return AARBF_OK;
   default:
     return set_error(ctx, AARBF_ERR_INVALID_ARGUMENT, "aarbf_set_tuning: unknown parameter");
   }

@@ -777,6 +777,570 @@ __global__ __launch_bounds__(kDBlock) void diff_adjoint(DiffArgs D) {
       if (s_gdm[j] != 0.0f) atomicAdd(&D.g_dm[j], s_gdm[j]);
 }
 
+// ---- one wave per (pixel, sample): expected marching --------------------------
+// The lanes of a wave take 64 consecutive march points of one sample at a
+// time.  A cooperative stretch query (a level-synchronous walk of the
+// quantised tree, 64 child boxes per pass, the frontier in LDS) collects
+// every particle whose widened sphere meets the ray stretch [ta, tb] into
+// LDS, and each lane sums its point's payload over them.  The march's
+// optical depth is a prefix sum over the lanes, so a sample that crosses a
+// dense cluster costs its points / 64 passes, not one lane's serial march
+// (the serial schedule waits for its slowest samples).  The stretch grows
+// where it holds few particles and is halved where more than kWC meet it;
+// a point covered by more than kWC particles is summed by its lane alone
+// with the serial point query.  The adjoint's lanes then each own up to two
+// of the stretch's particles and sum their gradients over the chunk's
+// points in registers: one set of atomics per particle and stretch.
+constexpr uint32_t kWC = 128; // particles of a stretch
+constexpr uint32_t kWF = 512; // frontier of the stretch query
+
+struct WaveMem {
+  uint32_t cidx[kWC];
+  float ctin[kWC], ctout[kWC];
+  float4 cP[kWC];  // x, y, z, m = -4.5 / r^2
+  float4 cK[kWC];  // alpha^3, rgb^2.2
+  float cinv[kWC]; // 1 / r
+  uint32_t front[2][kWF];
+  float4 px[64]; // adjoint: a chunk's points (x, y, z, t; t = NaN where not covered)
+  float4 pa[64]; // adjoint: their a_rho, a_S
+  float tp[64];  // adjoint: the chunk's point parameters (ascending)
+};
+
+// k steps of the serial march's recurrence t <- t + unit (one binary32
+// rounding per step).  Inside a binade every step adds the same number of
+// units in the last place (unit / ulp rounded), unless that quotient is a
+// tie, so a run of steps that stays in the binade is one integer multiply;
+// binade crossings, ties, zero and subnormals are stepped one addition at
+// a time.
+__device__ inline float rec_advance(float t, float u, uint32_t k) {
+  const uint32_t bu = __float_as_uint(u), eu = (bu >> 23) & 0xffu, mu = (bu & 0x7fffffu) | 0x800000u;
+  while (k > 0u) {
+    const float t1 = t + u;
+    const uint32_t b0 = __float_as_uint(t), b1 = __float_as_uint(t1), e0 = (b0 >> 23) & 0xffu;
+    bool single = k == 1u || !(t > 0.0f) || e0 == 0u || e0 == 0xffu || ((b1 >> 23) & 0xffu) != e0 || eu == 0u;
+    if (!single && eu < e0) { // unit / ulp = mu / 2^(e0 - eu): a tie when its fraction is exactly 1/2
+      const uint32_t sh = e0 - eu;
+      single = sh <= 24u && (mu & ((1u << sh) - 1u)) == (1u << (sh - 1u));
+    }
+    if (single) {
+      t = t1;
+      --k;
+      continue;
+    }
+    const uint32_t M0 = b0 & 0x7fffffu, R = (b1 & 0x7fffffu) - M0;
+    uint32_t n = k;
+    if (R > 0u) n = min(k, (0x7fffffu - M0) / R); // steps that stay in the binade
+    if (n <= 1u) {
+      t = t1;
+      --k;
+      continue;
+    }
+    t = __uint_as_float((b0 & 0xff800000u) | (M0 + n * R));
+    k -= n;
+  }
+  return t;
+}
+
+__device__ inline float wave_sum(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline float wave_incl_scan(float v, uint32_t lane) {
+  for (int o = 1; o < 64; o <<= 1) {
+    const float y = __shfl_up(v, o, 64);
+    if (lane >= (uint32_t)o) v += y;
+  }
+  return v;
+}
+
+// Every particle whose sphere, widened by m, meets the ray stretch o + t d,
+// t in [ta, tb], into W (entry / exit parameters, payload); nc = how many.
+// False when the frontier or the list overflows.  Called by the whole wave.
+__device__ inline bool wave_query(const DiffArgs &D, WaveMem &W, float3 o, float3 d, float ta, float tb, float m,
+                                  uint32_t lane, uint32_t &nc) {
+  const RenderArgs &A = D.A;
+  const uint64_t lt = lane ? (~0ull >> (64u - lane)) : 0ull;
+  SegQuery q;
+  q.ax = o.x + ta * d.x;
+  q.ay = o.y + ta * d.y;
+  q.az = o.z + ta * d.z;
+  q.dx = d.x;
+  q.dy = d.y;
+  q.dz = d.z;
+  q.ix = safe_inv(d.x);
+  q.iy = safe_inv(d.y);
+  q.iz = safe_inv(d.z);
+  q.m = m;
+  q.len = (tb - ta) + 4.0f * m;
+  const QGroup *__restrict__ Qg = A.bvh.groups;
+  const uint32_t log2K = A.bvh.log2K, K = 1u << log2K, top = A.bvh.top;
+  const float4 *root = A.bvh.nodes + g_lvl_ofs[top];
+  uint32_t nf = seg_box(q, root[0], root[1]) ? 1u : 0u, cur = 0;
+  if (lane == 0) W.front[0][0] = 0;
+  __syncthreads();
+  for (uint32_t lvl = top; lvl >= 1u && nf > 0u; --lvl) {
+    const uint32_t total = nf << log2K;
+    uint32_t nn = 0;
+    for (uint32_t base = 0; base < total; base += 64u) {
+      const uint32_t j = base + lane;
+      bool hit = false;
+      uint32_t child = 0;
+      if (j < total) {
+        const uint32_t node = W.front[cur][j >> log2K], k = j & (K - 1u);
+        const QGroup *gp = Qg + g_lvl_qofs[lvl] + node;
+        const uint32_t first = node << log2K, mcnt = min(K, g_lvl_cnt[lvl - 1u] - first);
+        auto qb = [&](uint32_t field) { return (gp->q[2u * field + (k >> 2)] >> ((k & 3u) * 8u)) & 255u; };
+        const uint32_t lx = qb(0), hx = qb(3);
+        child = first + k;
+        if (k < mcnt && lx <= hx) {
+          const float4 ga = gp->a, gb = gp->b;
+          float4 lo, hi;
+          lo.x = ga.x + (float)lx * ga.w;
+          lo.y = ga.y + (float)qb(1) * gb.x;
+          lo.z = ga.z + (float)qb(2) * gb.y;
+          hi.x = ga.x + (float)hx * ga.w;
+          hi.y = ga.y + (float)qb(4) * gb.x;
+          hi.z = ga.z + (float)qb(5) * gb.y;
+          lo.w = hi.w = 0.0f;
+          hit = seg_box(q, lo, hi);
+        }
+      }
+      const uint64_t mk = __ballot(hit);
+      if (hit) {
+        const uint32_t at = nn + (uint32_t)__popcll(mk & lt);
+        if (at < kWF) W.front[cur ^ 1u][at] = child;
+      }
+      nn += (uint32_t)__popcll(mk);
+    }
+    __syncthreads();
+    if (nn > kWF) return false;
+    cur ^= 1u;
+    nf = nn;
+  }
+  // the leaves' particles
+  nc = 0;
+  const uint32_t ppl = A.ppl, total = nf * ppl;
+  for (uint32_t base = 0; base < total; base += 64u) {
+    const uint32_t j = base + lane;
+    bool hit = false;
+    uint32_t p = 0;
+    float tin = 0.0f, tout = 0.0f;
+    float4 P = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (j < total) {
+      p = W.front[cur][j / ppl] * ppl + j % ppl;
+      if (p < A.numParticles) {
+        P = A.pr[p];
+        if (P.w > 0.0f) {
+          const float vx = P.x - o.x, vy = P.y - o.y, vz = P.z - o.z;
+          const float tc = vx * d.x + vy * d.y + vz * d.z;
+          const float hx = vx - tc * d.x, hy = vy - tc * d.y, hz = vz - tc * d.z;
+          const float rr = P.w + m;
+          const float w2 = rr * rr - (hx * hx + hy * hy + hz * hz);
+          if (w2 >= 0.0f) {
+            const float w = sqrtf(w2);
+            tin = tc - w - m;
+            tout = tc + w + m;
+            hit = tout >= ta && tin <= tb;
+          }
+        }
+      }
+    }
+    const uint64_t mk = __ballot(hit);
+    if (hit) {
+      const uint32_t at = nc + (uint32_t)__popcll(mk & lt);
+      if (at < kWC) {
+        const float2 mi = D.pm[p];
+        W.cidx[at] = p;
+        W.ctin[at] = tin;
+        W.ctout[at] = tout;
+        W.cP[at] = make_float4(P.x, P.y, P.z, mi.x);
+        W.cK[at] = A.pc[p];
+        W.cinv[at] = mi.y;
+      }
+    }
+    nc += (uint32_t)__popcll(mk);
+  }
+  __syncthreads();
+  return nc <= kWC;
+}
+
+// The sample of block b (pixel-major over 8 x 8 pixel tiles, the samples of
+// a pixel on consecutive blocks).  False past the image.
+__device__ inline bool block_sample(const DiffArgs &D, int &px, int &py, uint32_t &s) {
+  const RenderArgs &A = D.A;
+  const uint64_t b = blockIdx.x;
+  s = (uint32_t)(b % D.spp);
+  const uint64_t p = b / D.spp;
+  const uint32_t tiles_x = ((uint32_t)A.W + 7u) / 8u;
+  const uint64_t tile = p >> 6;
+  const uint32_t in = (uint32_t)(p & 63u);
+  px = (int)((uint32_t)(tile % tiles_x) * 8u + (in & 7u));
+  py = (int)((uint32_t)(tile / tiles_x) * 8u + (in >> 3));
+  return px < A.W && py < A.H;
+}
+
+// The wave march of sample s of pixel (px, py): fn(i0, i1, t, x, y, z, pl,
+// dense) for each run of lanes [i0, i1) whose points share a stretch, in
+// march order (lane i holds point t; pl its payload, count 0 where nothing
+// covers it; dense: the run is one point summed by the serial query).  fn
+// returns false to end the sample.  flush() runs before a new stretch
+// replaces the candidates.
+template <class Fn, class Flush>
+__device__ inline void wave_march(const DiffArgs &D, WaveMem &W, int px, int py, uint32_t s, float3 &o_out,
+                                  float3 &d_out, Fn fn, Flush flush) {
+  const RenderArgs &A = D.A;
+  const uint32_t lane = threadIdx.x;
+  uint32_t fid = (A.frameID + s) & 0xffffu;
+  uint32_t rng = aa_rng_seed(fid, (uint32_t)px, (uint32_t)py, (uint32_t)A.W);
+  const float r1 = aa_lcg_next(&rng);
+  const float sx = ((float)px + 0.5f) / (float)A.W, sy = ((float)py + 0.5f) / (float)A.H;
+  float3 d;
+  d.x = (A.d00.x + sx * A.du.x) + sy * A.dv.x;
+  d.y = (A.d00.y + sx * A.du.y) + sy * A.dv.y;
+  d.z = (A.d00.z + sx * A.du.z) + sy * A.dv.z;
+  const float len = sqrtf((d.x * d.x + d.y * d.y) + d.z * d.z);
+  d.x = d.x / len;
+  d.y = d.y / len;
+  d.z = d.z / len;
+  float te, tx;
+  aabb_intersection(A.pos, d, 0.0f, 10000.0f, A.lb, A.rt, te, tx);
+  const float3 o = A.pos;
+  o_out = o;
+  d_out = d;
+  if (!(te < tx)) return;
+  const float m = 2e-5f * (1.0f + fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) + fmaxf(fabsf(te), fabsf(tx)));
+  float t_last = te + A.unit * r1; // the first point is t_last + unit
+  uint32_t k_done = 0, nc = 0;
+  float sa = AA_INF_F, sb = -AA_INF_F, lenq = 64.0f * A.unit;
+  bool have_stretch = false;
+  for (;;) {
+    const float t = rec_advance(t_last, A.unit, lane + 1u);
+    const bool valid = t < tx && k_done + lane + 1u <= A.max_depth;
+    const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid)); // a prefix: t grows with the lane
+    if (nvalid == 0u) break;
+    uint32_t i0 = 0;
+    while (i0 < nvalid) {
+      const float ti0 = __shfl(t, (int)i0, 64);
+      bool dense = false;
+      if (!(have_stretch && ti0 >= sa && ti0 < sb)) {
+        if (have_stretch) flush();
+        float ql = lenq;
+        bool ok = false;
+        for (;;) {
+          const float tb = fminf(ti0 + ql, tx);
+          ok = wave_query(D, W, o, d, ti0, tb, m, lane, nc);
+          if (ok) {
+            sa = ti0;
+            sb = tb >= tx ? AA_INF_F : tb;
+            lenq = nc == 0u ? 4.0f * ql : (nc < kWC / 4u ? 2.0f * ql : ql);
+            break;
+          }
+          ql *= 0.5f;
+          if (ql < 0.5f * A.unit) break;
+        }
+        have_stretch = ok;
+        if (!ok) { // more than kWC particles at point ti0
+          dense = true;
+          lenq = A.unit;
+        }
+      }
+      const uint32_t i1 = dense ? i0 + 1u : (uint32_t)__popcll(__ballot(valid && t < sb));
+      const bool act = lane >= i0 && lane < i1;
+      const float x = o.x + t * d.x, y = o.y + t * d.y, z = o.z + t * d.z;
+      DPay pl;
+      pl.count = 0;
+      pl.rho = pl.S[0] = pl.S[1] = pl.S[2] = 0.0f;
+      if (dense) {
+        if (act)
+          for_covering(D, x, y, z, [&](uint32_t p, const float4 &, float e) {
+            const float4 k = A.pc[p];
+            const float w = fexp(e) * k.x;
+            pl.count += 1;
+            pl.rho += w;
+            pl.S[0] += k.y * w;
+            pl.S[1] += k.z * w;
+            pl.S[2] += k.w * w;
+          });
+      } else {
+        const float tlo = __shfl(t, (int)i0, 64), thi = __shfl(t, (int)i1 - 1, 64);
+        for (uint32_t c = 0; c < nc; ++c) {
+          const float cin = W.ctin[c], cout = W.ctout[c];
+          if (!(cin <= thi && cout >= tlo)) continue; // meets none of the run's points
+          if (act && cin <= t && t <= cout) {
+            const float4 P = W.cP[c];
+            const float e = aa_dist2(P.x, P.y, P.z, x, y, z) * P.w;
+            if (P.w < 0.0f && e > -4.5f) {
+              const float4 k = W.cK[c];
+              const float w = fexp(e) * k.x;
+              pl.count += 1;
+              pl.rho += w;
+              pl.S[0] += k.y * w;
+              pl.S[1] += k.z * w;
+              pl.S[2] += k.w * w;
+            }
+          }
+        }
+      }
+      if (!act) pl.count = 0;
+      if (!fn(i0, i1, t, x, y, z, pl, dense, nc)) {
+        if (have_stretch) flush();
+        return;
+      }
+      i0 = i1;
+      if (dense) have_stretch = false;
+    }
+    if (nvalid < 64u) break;
+    t_last = __shfl(t, 63, 64);
+    k_done += 64u;
+  }
+  if (have_stretch) flush();
+}
+
+// The march state the lanes share: the optical depth so far, and the last
+// covered point's F'(L) and colour (its term of the path vector waits for
+// the next covered point's colour).
+struct WaveAcc {
+  float L = 0.0f;
+  bool have = false;
+  float pdF = 0.0f, pc[3] = {0.0f, 0.0f, 0.0f};
+};
+
+// Per lane of a run: D and the colour where covered; the run's inclusive
+// optical depth, F before and after, F'(L) after; the next covered lane's
+// colour (cn, valid where nxt) and the run's first / last covered lanes.
+struct WaveRun {
+  bool cov, nxt;
+  float Dv, sD, fD, c[3], cn[3], F0, F1, dF1, Lend;
+  uint32_t j0, j1;
+  uint64_t covm;
+  uint32_t first, last;
+};
+
+__device__ inline void wave_run(const DiffArgs &D, const WaveAcc &acc, const DPay &pl, uint32_t lane, WaveRun &r) {
+  const RenderArgs &A = D.A;
+  r.cov = pl.count > 0 && pl.rho > 0.0f;
+  r.Dv = 0.0f;
+  r.sD = r.fD = 0.0f;
+  r.j0 = r.j1 = 0;
+  r.c[0] = r.c[1] = r.c[2] = 0.0f;
+  if (r.cov) {
+    r.Dv = tf_grad(A.dmap, A.dm_n, 0, pl.rho, r.sD, r.j0, r.j1, r.fD);
+    colour(pl, r.c);
+  }
+  const float Lin = acc.L + wave_incl_scan(r.Dv, lane);
+  r.Lend = __shfl(Lin, 63, 64);
+  float dF0;
+  r.F1 = Fcdf(Lin, r.dF1);
+  r.F0 = Fcdf(Lin - r.Dv, dF0);
+  r.covm = __ballot(r.cov);
+  r.first = r.covm ? (uint32_t)__ffsll((unsigned long long)r.covm) - 1u : 0u;
+  r.last = r.covm ? 63u - (uint32_t)__clzll((long long)r.covm) : 0u;
+  const uint64_t above = lane == 63u ? 0ull : (r.covm & (~0ull << (lane + 1u)));
+  r.nxt = r.cov && above != 0ull;
+  const int nl = above ? (int)__ffsll((unsigned long long)above) - 1 : 0;
+  for (int k = 0; k < 3; ++k) r.cn[k] = __shfl(r.c[k], nl, 64);
+}
+
+__global__ __launch_bounds__(64) void diff_forward_w(DiffArgs D) {
+  const RenderArgs &A = D.A;
+  __shared__ WaveMem W;
+  load_levels(A);
+  __syncthreads();
+  int px, py;
+  uint32_t s;
+  if (!block_sample(D, px, py, s)) return; // the whole block
+  const uint32_t lane = threadIdx.x;
+  WaveAcc acc;
+  float C[3] = {0.0f, 0.0f, 0.0f}, V[3] = {0.0f, 0.0f, 0.0f};
+  float3 o, d;
+  wave_march(
+      D, W, px, py, s, o, d,
+      [&](uint32_t, uint32_t, float, float, float, float, const DPay &pl, bool, uint32_t) {
+        WaveRun r;
+        wave_run(D, acc, pl, lane, r);
+        for (int k = 0; k < 3; ++k) {
+          C[k] += wave_sum(r.cov ? (r.F1 - r.F0) * r.c[k] : 0.0f);
+          V[k] += wave_sum(r.nxt ? r.dF1 * (r.c[k] - r.cn[k]) : 0.0f);
+        }
+        if (!r.covm) return true;
+        const float cf[3] = {__shfl(r.c[0], (int)r.first, 64), __shfl(r.c[1], (int)r.first, 64),
+                             __shfl(r.c[2], (int)r.first, 64)};
+        if (acc.have)
+          for (int k = 0; k < 3; ++k) V[k] += acc.pdF * (acc.pc[k] - cf[k]);
+        acc.have = true;
+        acc.pdF = __shfl(r.dF1, (int)r.last, 64);
+        for (int k = 0; k < 3; ++k) acc.pc[k] = __shfl(r.c[k], (int)r.last, 64);
+        acc.L = r.Lend;
+        // the serial march stops after the first point where F reaches 1
+        return __shfl(r.F1, (int)r.last, 64) < 1.0f;
+      },
+      [&]() {});
+  (void)o;
+  (void)d;
+  if (acc.have)
+    for (int k = 0; k < 3; ++k) V[k] += acc.pdF * acc.pc[k];
+  if (lane == 0) {
+    const uint64_t at = ((uint64_t)py * A.W + px) * D.spp + s;
+    if (D.path) D.path[at] = make_float4(V[0], V[1], V[2], 0.0f);
+    D.colours[at] = make_float4(C[0], C[1], C[2], 0.0f);
+  }
+}
+
+__global__ __launch_bounds__(64) void diff_adjoint_w(DiffArgs D) {
+  const RenderArgs &A = D.A;
+  __shared__ WaveMem W;
+  __shared__ float s_gdm[kTexMax];
+  load_levels(A);
+  for (uint32_t j = threadIdx.x; j < kTexMax; j += 64u) s_gdm[j] = 0.0f;
+  __syncthreads();
+  const bool lds_dm = A.dm_n <= kTexMax;
+  auto add_dm = [&](uint32_t j, float v) {
+    if (lds_dm) atomicAdd(&s_gdm[j], v);
+    else atomicAdd(&D.g_dm[j], v);
+  };
+  int px, py;
+  uint32_t s;
+  const uint32_t lane = threadIdx.x;
+  if (block_sample(D, px, py, s)) {
+    const float4 gi = D.gimg[(uint64_t)px + (uint64_t)A.W * py];
+    const float inv = 1.0f / (float)D.spp;
+    const float g[3] = {gi.x * inv, gi.y * inv, gi.z * inv};
+    const float4 PV = D.path[((uint64_t)py * A.W + px) * D.spp + s];
+    const float total = g[0] * PV.x + g[1] * PV.y + g[2] * PV.z;
+    WaveAcc acc;
+    float prefix = 0.0f;
+    // each lane owns particles lane and lane + 64 of the current stretch
+    float ac[2][8];
+    for (int u = 0; u < 2; ++u)
+      for (int w = 0; w < 8; ++w) ac[u][w] = 0.0f;
+    uint32_t cur_nc = 0;
+    auto flush_p = [&](uint32_t p, const float (&v)[8]) {
+      atomicAdd(&D.g_pos[3 * p], v[0]);
+      atomicAdd(&D.g_pos[3 * p + 1], v[1]);
+      atomicAdd(&D.g_pos[3 * p + 2], v[2]);
+      for (uint32_t w = 0; w < 5; ++w) atomicAdd(&D.acc5[5 * (uint64_t)p + w], v[3 + w]);
+    };
+    auto flush = [&]() {
+      for (uint32_t u = 0; u < 2; ++u) {
+        const uint32_t c = lane + 64u * u;
+        bool any = false;
+        for (int w = 0; w < 8; ++w) any |= ac[u][w] != 0.0f;
+        if (c < cur_nc && any) flush_p(W.cidx[c], ac[u]);
+        for (int w = 0; w < 8; ++w) ac[u][w] = 0.0f;
+      }
+      __syncthreads(); // the candidates may be replaced next
+    };
+    // the chain rule at one point: v = d(loss) / d(particle) for a covering
+    // particle (P, m; k; 1/r) at x with RBF exponent e
+    auto grad = [&](const float4 &P, const float4 &k, float inv_r, float x, float y, float z, float e,
+                    const float4 &a, float (&v)[8]) {
+      const float rbf = fexp(e), w = rbf * k.x;
+      const float a_w = a.x + a.y * k.y + a.z * k.z + a.w * k.w;
+      const float a_rbf = a_w * k.x;
+      const float kk = a_rbf * ((2.0f * P.w) * rbf); // d(rbf)/dP = 2 m rbf (P - x)
+      v[0] = kk * (P.x - x);
+      v[1] = kk * (P.y - y);
+      v[2] = kk * (P.z - z);
+      v[3] = a_rbf * (rbf * (-2.0f * e) * inv_r); // d(rbf)/dr = -2 e / r rbf
+      v[4] = a_w * rbf;
+      v[5] = a.y * w;
+      v[6] = a.z * w;
+      v[7] = a.w * w;
+    };
+    float3 o, d;
+    wave_march(
+        D, W, px, py, s, o, d,
+        [&](uint32_t i0, uint32_t i1, float t, float x, float y, float z, const DPay &pl, bool dense, uint32_t nc) {
+          cur_nc = dense ? 0u : nc;
+          WaveRun r;
+          wave_run(D, acc, pl, lane, r);
+          // dC/dD_j = total - (sum over the covered points k before j of
+          // F'(L_k) g.(c_k - c_{k+1}))
+          const float term = r.nxt ? r.dF1 * (g[0] * (r.c[0] - r.cn[0]) + g[1] * (r.c[1] - r.cn[1]) +
+                                              g[2] * (r.c[2] - r.cn[2]))
+                                   : 0.0f;
+          const float ex = wave_incl_scan(term, lane) - term;
+          float pterm = 0.0f;
+          if (acc.have && r.covm) {
+            const float cf[3] = {__shfl(r.c[0], (int)r.first, 64), __shfl(r.c[1], (int)r.first, 64),
+                                 __shfl(r.c[2], (int)r.first, 64)};
+            pterm = acc.pdF * (g[0] * (acc.pc[0] - cf[0]) + g[1] * (acc.pc[1] - cf[1]) + g[2] * (acc.pc[2] - cf[2]));
+          }
+          const float aD = total - ((prefix + (lane >= r.first ? pterm : 0.0f)) + ex);
+          const float pj = r.F1 - r.F0;
+          float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          if (r.cov) {
+            add_dm(r.j0, aD * (1.0f - r.fD));
+            add_dm(r.j1, aD * r.fD);
+            float a_rho = aD * r.sD;
+            float a_S[3];
+            const float inv_rho = __builtin_amdgcn_rcpf(pl.rho);
+            for (int k = 0; k < 3; ++k) {
+              // c = (S / rho)^(1/2.2): dc/dS = (1/2.2) c / S, dc/drho = -(1/2.2) c / rho
+              const float gcc = (g[k] * pj) * ((1.0f / 2.2f) * r.c[k]);
+              a_S[k] = pl.S[k] > 0.0f ? gcc * __builtin_amdgcn_rcpf(pl.S[k]) : 0.0f;
+              a_rho -= gcc * inv_rho;
+            }
+            a = make_float4(a_rho, a_S[0], a_S[1], a_S[2]);
+          }
+          prefix += pterm + wave_sum(term);
+          if (r.covm) {
+            acc.have = true;
+            acc.pdF = __shfl(r.dF1, (int)r.last, 64);
+            for (int k = 0; k < 3; ++k) acc.pc[k] = __shfl(r.c[k], (int)r.last, 64);
+            acc.L = r.Lend;
+          }
+          if (dense) { // one point, summed by its lane: direct atomics per particle
+            if (r.cov)
+              for_covering(D, x, y, z, [&](uint32_t p, const float4 &, float e) {
+                const float4 P4 = A.pr[p];
+                const float2 mi = D.pm[p];
+                float v[8];
+                grad(make_float4(P4.x, P4.y, P4.z, mi.x), A.pc[p], mi.y, x, y, z, e, a, v);
+                flush_p(p, v);
+              });
+          } else if (r.covm) {
+            W.px[lane] = make_float4(x, y, z, r.cov ? t : AA_NAN_F);
+            W.pa[lane] = a;
+            W.tp[lane] = t;
+            __syncthreads();
+            for (uint32_t u = 0; u < 2; ++u) {
+              const uint32_t c = lane + 64u * u;
+              if (c >= nc) continue;
+              const float cin = W.ctin[c], cout = W.ctout[c];
+              const float4 P = W.cP[c], k = W.cK[c];
+              const float inv_r = W.cinv[c];
+              // the run's points inside [cin, cout]: from the first at or past cin
+              uint32_t i = i0;
+              for (uint32_t hi = i1; i < hi;) {
+                const uint32_t mid = (i + hi) >> 1;
+                if (W.tp[mid] < cin) i = mid + 1u;
+                else hi = mid;
+              }
+              for (; i < i1 && W.tp[i] <= cout; ++i) {
+                const float4 q = W.px[i];
+                if (!(q.w == q.w)) continue; // not covered
+                const float e = aa_dist2(P.x, P.y, P.z, q.x, q.y, q.z) * P.w;
+                if (!(P.w < 0.0f && e > -4.5f)) continue;
+                float v[8];
+                grad(P, k, inv_r, q.x, q.y, q.z, e, W.pa[i], v);
+                for (int w = 0; w < 8; ++w) ac[u][w] += v[w];
+              }
+            }
+            __syncthreads();
+          }
+          return __shfl(r.F1, (int)r.last, 64) < 1.0f || !r.covm;
+        },
+        flush);
+    (void)o;
+    (void)d;
+  }
+  __syncthreads();
+  if (lds_dm)
+    for (uint32_t j = threadIdx.x; j < A.dm_n; j += 64u)
+      if (s_gdm[j] != 0.0f) atomicAdd(&D.g_dm[j], s_gdm[j]);
+}
+
 // The chain rule of each particle's summed adjoints through its radiusmap
 // and colormap lookups: r = radius * radiusmap(w), alpha^3 and rgb^2.2 from
 // colormap(w).  Texel gradients are summed per block in LDS first.
@@ -929,6 +1493,16 @@ static uint32_t diff_blocks(int W, int H, uint32_t spp) {
   return (uint32_t)((tiles * 64u * spp + kDBlock - 1) / kDBlock);
 }
 
+// The wave march: expected marching over a tree with quantised groups.
+static bool wave_schedule(const aarbf_context *ctx, const DiffArgs &D) {
+  return ctx->tune_diff_waves == 1 && D.ratio == 0 && D.A.bvh.groups != nullptr;
+}
+
+// One block (one wave) per (pixel, sample) of whole 8 x 8 tiles.
+static uint32_t wave_blocks(int W, int H, uint32_t spp) {
+  return (uint32_t)((uint64_t)((W + 7) / 8) * (uint64_t)((H + 7) / 8) * 64u * spp);
+}
+
 extern "C" {
 
 int aarbf_set_diff_estimator(aarbf_context *ctx, int estimator) {
@@ -948,8 +1522,13 @@ int aarbf_diff_forward(aarbf_context *ctx, const aarbf_raygen_data *rg, const aa
   if (rc != AARBF_OK) return rc;
   D.out = (float4 *)image;
   D.path = (float4 *)path;
-  diff_forward<<<diff_blocks(W, H, spp), kDBlock, diff_stack_bytes(D), ctx->stream>>>(D);
-  AARBF_CHECK_LAUNCH(ctx, "diff_forward");
+  if (wave_schedule(ctx, D)) {
+    diff_forward_w<<<wave_blocks(W, H, spp), 64, 0, ctx->stream>>>(D);
+    AARBF_CHECK_LAUNCH(ctx, "diff_forward_w");
+  } else {
+    diff_forward<<<diff_blocks(W, H, spp), kDBlock, diff_stack_bytes(D), ctx->stream>>>(D);
+    AARBF_CHECK_LAUNCH(ctx, "diff_forward");
+  }
   const uint64_t npix = (uint64_t)W * (uint64_t)H;
   diff_fold<<<(uint32_t)((npix + 255) / 256), 256, 0, ctx->stream>>>(D);
   AARBF_CHECK_LAUNCH(ctx, "diff_fold");
@@ -975,8 +1554,13 @@ int aarbf_diff_backward(aarbf_context *ctx, const aarbf_raygen_data *rg, const a
   D.g_rm = (float *)g_radiusmap;
   const uint64_t n5 = 5 * (uint64_t)D.A.numParticles;
   if (n5) diff_zero<<<(uint32_t)((n5 + 255) / 256), 256, 0, ctx->stream>>>(D.acc5, n5);
-  diff_adjoint<<<diff_blocks(W, H, spp), kDBlock, diff_stack_bytes(D), ctx->stream>>>(D);
-  AARBF_CHECK_LAUNCH(ctx, "diff_adjoint");
+  if (wave_schedule(ctx, D) && D.path) { // the wave march starts from the forward's path vector
+    diff_adjoint_w<<<wave_blocks(W, H, spp), 64, 0, ctx->stream>>>(D);
+    AARBF_CHECK_LAUNCH(ctx, "diff_adjoint_w");
+  } else {
+    diff_adjoint<<<diff_blocks(W, H, spp), kDBlock, diff_stack_bytes(D), ctx->stream>>>(D);
+    AARBF_CHECK_LAUNCH(ctx, "diff_adjoint");
+  }
   if (D.A.numParticles)
     diff_chain<<<(D.A.numParticles + kCBlock - 1) / kCBlock, kCBlock, 0, ctx->stream>>>(D);
   AARBF_CHECK_LAUNCH(ctx, "diff_chain");

@@ -82,6 +82,7 @@ struct aarbf_context {
   int tune_step_cap = 4096;      // tracker steps per ray per wavefront round
   int tune_graph_rounds = 0;     // wavefront rounds per captured hipGraph (0 = host-driven rounds)
   int tune_walk_budget = 256;    // nodes a one-lane gather walk opens before handing its ray to a wave
+  int tune_diff_waves = 1;       // differentiable expected marching: 0 a lane per sample, 1 a wave per sample
   std::vector<std::pair<std::string, hipGraphExec_t>> graphs; // captured rounds, keyed by their arguments
   // kernel timing (HIP events around each kernel class of the render rounds)
   bool timing = false;

@@ -167,12 +167,17 @@ int aarbf_set_render_mode(aarbf_context *ctx, int mode);
  * (cached per set of arguments; kernels read their counts on the device)
  * and replayed with one read-back per replay; AARBF_TUNE_WALK_BUDGET: tree
  * nodes a one-lane gather walk may open before it hands its ray to a
- * one-wave-per-ray gather in the same round (1..65536, default 256).  No
- * counterpart in the reference. */
+ * one-wave-per-ray gather in the same round (1..65536, default 256);
+ * AARBF_TUNE_DIFF_WAVES: the differentiable renderer's expected marching
+ * (estimator 0) with one lane per (pixel, sample) (0) or one wave per
+ * (pixel, sample), its lanes on consecutive march points (1, default;
+ * ratio tracking always runs one lane per sample).  No counterpart in the
+ * reference. */
 #define AARBF_TUNE_STEP_WAVES 1
 #define AARBF_TUNE_STEP_CAP 2
 #define AARBF_TUNE_GRAPH_ROUNDS 3
 #define AARBF_TUNE_WALK_BUDGET 4
+#define AARBF_TUNE_DIFF_WAVES 5
 int aarbf_set_tuning(aarbf_context *ctx, int param, int value);
 /* Enable (1) or disable (0) render statistics; read them back (and reset
  * them) with aarbf_render_stats_read. */

@@ -2,7 +2,9 @@
 the FP64 oracle (oracle/adjoint.c, itself pinned by central finite
 differences), and the torch.autograd.Function wrapper.  Tolerances: the
 forward within 1e-4 absolute per pixel; each gradient block within 1e-3
-relative L2 of the oracle's (float32 kernels against float64)."""
+relative L2 of the oracle's (float32 kernels against float64).  Expected
+marching runs in both schedules (one wave per sample, the default, and
+one lane per sample); ratio tracking in the lane schedule."""
 import time
 
 import numpy as np
@@ -35,11 +37,20 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("estimator", [0, 1])
-def test_diff_forward_and_adjoint_match_oracle(gpu_ctx, estimator):
-    """estimator 0: expected marching (path replay, total - prefix);
-    1: ratio tracking (path replay with the remaining-colour recursion,
-    against the oracle's stored-points reverse recursion)."""
+@pytest.fixture
+def schedule(gpu_ctx, request):
+    """AARBF_TUNE_DIFF_WAVES for the test, back to the default after it."""
+    gpu_ctx.set_tuning(gpu_ctx.TUNE_DIFF_WAVES, request.param)
+    yield request.param
+    gpu_ctx.set_tuning(gpu_ctx.TUNE_DIFF_WAVES, 1)
+
+
+@pytest.mark.parametrize("estimator,schedule", [(0, 1), (0, 0), (1, 0)], indirect=["schedule"])
+def test_diff_forward_and_adjoint_match_oracle(gpu_ctx, estimator, schedule):
+    """estimator 0: expected marching (path replay, total - prefix), in the
+    wave and the lane schedule; 1: ratio tracking (path replay with the
+    remaining-colour recursion, against the oracle's stored-points reverse
+    recursion)."""
     import torch
     W = H = 24
     spp, unit = 2, 0.01
@@ -68,8 +79,8 @@ def test_diff_forward_and_adjoint_match_oracle(gpu_ctx, estimator):
         assert e < 1e-3, k
 
 
-@pytest.mark.parametrize("estimator", [0, 1])
-def test_diff_backward_cost(gpu_ctx, estimator):
+@pytest.mark.parametrize("estimator,schedule", [(0, 1), (0, 0), (1, 0)], indirect=["schedule"])
+def test_diff_backward_cost(gpu_ctx, estimator, schedule):
     """forward + backward per view against forward alone (<= 3x): the
     library's forward and adjoint calls, each timed to the end of its work
     (the per-view scene rebuild, common to both, is not part of either),
@@ -146,3 +157,39 @@ def test_diff_renderer_releases_its_handles(gpu_ctx):
     del img
     gc.collect()
     assert gpu_ctx.handle_counts() == base, "output dropped without backward"
+
+
+@pytest.mark.parametrize("spp", [1, 4])
+def test_diff_schedules_agree(gpu_ctx, spp):
+    """Expected marching in the wave schedule against the lane schedule on a
+    crowded scene (points covered by more particles than a stretch holds
+    take the serial query): the same forward to float rounding, the same
+    path vectors and gradients to 1e-4 relative L2."""
+    import torch
+    from aarbf.diff import _Render
+    W, H = 64, 48
+    pos, attr, radii, cm, dm, rm = _scene(seed=9, n=20000)
+    radii *= 0.6
+    cam = default_camera(W, H, look_from=(0.25, 0.15, 1.2))
+    T = lambda a: torch.tensor(a, device="cuda")
+    r = _Render(gpu_ctx, cam, W, H, 0.004, spp, 2, 0.2, 1, 8, T(pos), T(attr), T(radii), T(cm), T(dm), T(rm))
+    g = torch.tensor(np.random.default_rng(4).normal(size=(H, W, 3)).astype(np.float32), device="cuda")
+    out = {}
+    try:
+        for waves in (0, 1):
+            gpu_ctx.set_tuning(gpu_ctx.TUNE_DIFF_WAVES, waves)
+            img = r.forward()
+            path = r.path.clone()
+            grads = [x.clone() for x in r.backward(g)]
+            out[waves] = (img, path, grads)
+    finally:
+        gpu_ctx.set_tuning(gpu_ctx.TUNE_DIFF_WAVES, 1)
+        r.close()
+    (i0, p0, g0), (i1, p1, g1) = out[0], out[1]
+    assert float(i0.mean()) > 0.05
+    assert float((i0 - i1).abs().max()) < 1e-5
+    assert float(torch.linalg.norm((p0 - p1).flatten()) / torch.linalg.norm(p0.flatten())) < 1e-4
+    for k, (a, b) in enumerate(zip(g0, g1)):
+        e = float(torch.linalg.norm((a - b).flatten()) / max(float(torch.linalg.norm(a.flatten())), 1e-30))
+        print(f"gradient block {k}: relative L2 {e:.2e}")
+        assert e < 1e-4, k
