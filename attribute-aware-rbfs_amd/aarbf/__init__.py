@@ -199,6 +199,7 @@ class Context:
     TUNE_GRAPH_ROUNDS = 3
     TUNE_WALK_BUDGET = 4
     TUNE_DIFF_WAVES = 5
+    TUNE_STEP_MIN_LIVE = 6
 
     def set_tuning(self, param: int, value: int):
         """Schedule parameters of the default renderer (images do not change)."""
@@ -413,7 +414,8 @@ class Context:
         self.check(self.lib.aarbf_autotune(self.h, self._requests(requests), len(requests), reps, C.byref(res)),
                    "aarbf_autotune")
         return {"step_cap": res.step_cap, "step_waves": res.step_waves, "graph_rounds": res.graph_rounds,
-                "walk_budget": res.walk_budget, "candidates": res.candidates, "baseline_ms": res.baseline_ms,
+                "walk_budget": res.walk_budget, "step_min_live": res.step_min_live, "candidates": res.candidates,
+                "baseline_ms": res.baseline_ms,
                 "best_ms": res.best_ms}
 
     def batch_latency(self, n: int) -> np.ndarray:

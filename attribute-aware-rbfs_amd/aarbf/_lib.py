@@ -76,7 +76,7 @@ class Request(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("step_cap", C.c_int32), ("step_waves", C.c_int32), ("graph_rounds", C.c_int32),
                 ("candidates", u32), ("baseline_ms", C.c_double), ("best_ms", C.c_double),
-                ("walk_budget", C.c_int32)]
+                ("walk_budget", C.c_int32), ("step_min_live", C.c_int32)]
 
 
 class CameraState(C.Structure):
@@ -184,7 +184,7 @@ def _expected_layout():
         C.sizeof(RayGenData), RayGenData.fbSize.offset, RayGenData.globalAABBMax.offset,
         C.sizeof(RenderStats),
         C.sizeof(Request), Request.pc.offset, Request.spp.offset,
-        C.sizeof(TuneResult), TuneResult.walk_budget.offset,
+        C.sizeof(TuneResult), TuneResult.walk_budget.offset, TuneResult.step_min_live.offset,
         C.sizeof(CameraState), CameraState.firstFrame.offset,
         C.sizeof(CameraInput), CameraInput.left_shift.offset,
     ]

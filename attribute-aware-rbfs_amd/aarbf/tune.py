@@ -3,7 +3,7 @@
 The leaf size (particles per leaf) is a property of the scene: each
 candidate rebuilds the bounds and the BVH (Scene.prepare) and runs the
 schedule autotuner (aarbf_autotune: step budget, occupancy, graph rounds,
-gather walk budget) on the views, each leaf size from the same starting
+gather walk budget, step-kernel live-lane threshold) on the views, each leaf size from the same starting
 schedule; the fastest (leaf size, schedule) is returned with its scene,
 and the context keeps that schedule.
 """
@@ -25,10 +25,11 @@ def tune_leaf_size(ctx, xyzw, colormap, radiusmap, densitymap, rbf_radius: float
         sc = Scene.prepare(ctx, xyzw, colormap, radiusmap, densitymap, rbf_radius, radii=radii, ppl=ppl,
                            branching=branching)
         reqs = make_requests(sc)
-        ctx.set_tuning(ctx.TUNE_STEP_CAP, 4096)
+        ctx.set_tuning(ctx.TUNE_STEP_CAP, 8192)
         ctx.set_tuning(ctx.TUNE_STEP_WAVES, 0)
         ctx.set_tuning(ctx.TUNE_GRAPH_ROUNDS, 0)
         ctx.set_tuning(ctx.TUNE_WALK_BUDGET, 256)
+        ctx.set_tuning(ctx.TUNE_STEP_MIN_LIVE, 16)
         r = ctx.autotune(reqs, reps=reps)
         per[ppl] = r["best_ms"]
         settings[ppl] = r
@@ -40,6 +41,8 @@ def tune_leaf_size(ctx, xyzw, colormap, radiusmap, densitymap, rbf_radius: float
     ctx.set_tuning(ctx.TUNE_STEP_WAVES, r["step_waves"])
     ctx.set_tuning(ctx.TUNE_GRAPH_ROUNDS, r["graph_rounds"])
     ctx.set_tuning(ctx.TUNE_WALK_BUDGET, r["walk_budget"])
+    ctx.set_tuning(ctx.TUNE_STEP_MIN_LIVE, r["step_min_live"])
     return sc, {"particles_per_leaf": ppl, "step_cap": r["step_cap"], "step_waves": r["step_waves"],
-                "graph_rounds": r["graph_rounds"], "walk_budget": r["walk_budget"], "best_ms": r["best_ms"],
+                "graph_rounds": r["graph_rounds"], "walk_budget": r["walk_budget"],
+                "step_min_live": r["step_min_live"], "best_ms": r["best_ms"],
                 "per_leaf_size": per}

@@ -192,6 +192,10 @@ int aarbf_set_tuning(aarbf_context *ctx, int param, int value) {
     AARBF_REQUIRE(ctx, value == 0 || value == 1, "aarbf_set_tuning: diff waves must be 0 or 1");
     ctx->tune_diff_waves = value;
     return AARBF_OK;
+  case AARBF_TUNE_STEP_MIN_LIVE:
+    AARBF_REQUIRE(ctx, value >= 0 && value <= 64, "aarbf_set_tuning: step min-live lanes must be in [0, 64]");
+    ctx->tune_step_min_live = value;
+    return AARBF_OK;
   default:
     return set_error(ctx, AARBF_ERR_INVALID_ARGUMENT, "aarbf_set_tuning: unknown parameter");
   }
@@ -394,6 +398,7 @@ uint32_t aarbf_abi_layout(uint32_t *out, uint32_t n) {
       (uint32_t)offsetof(aarbf_request, spp),
       (uint32_t)sizeof(aarbf_tune_result),
       (uint32_t)offsetof(aarbf_tune_result, walk_budget),
+      (uint32_t)offsetof(aarbf_tune_result, step_min_live),
       (uint32_t)sizeof(aarbf_camera_state),
       (uint32_t)offsetof(aarbf_camera_state, firstFrame),
       (uint32_t)sizeof(aarbf_camera_input),

@@ -79,9 +79,10 @@ struct aarbf_context {
   int diff_estimator = 0;        // differentiable renderer: 0 expected marching, 1 ratio tracking
   int cus = 0;                   // compute units of the device
   int tune_step_waves = 0;       // occupancy target of the step kernel (0 = default, 4 waves per SIMD)
-  int tune_step_cap = 4096;      // tracker steps per ray per wavefront round
+  int tune_step_cap = 8192;      // tracker steps per ray per wavefront round (base, see step_budget)
   int tune_graph_rounds = 0;     // wavefront rounds per captured hipGraph (0 = host-driven rounds)
   int tune_walk_budget = 256;    // nodes a one-lane gather walk opens before handing its ray to a wave
+  int tune_step_min_live = 16;   // step-kernel waves leave their loop below this many live lanes (0 = never)
   int tune_diff_waves = 1;       // differentiable expected marching: 0 a lane per sample, 1 a wave per sample
   std::vector<std::pair<std::string, hipGraphExec_t>> graphs; // captured rounds, keyed by their arguments
   // kernel timing (HIP events around each kernel class of the render rounds)

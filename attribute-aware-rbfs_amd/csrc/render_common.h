@@ -63,6 +63,10 @@ struct RenderArgs {
   const uint32_t *tile_req;
   uint32_t nreq;
   unsigned long long *lat;
+  // step kernel: a wave leaves its loop when fewer than this many of its
+  // lanes are still stepping (0 = never), so that the next round packs the
+  // rays it held into full waves
+  uint32_t min_live;
 };
 
 // Default schedule (wavefront.hip): rounds of gather and step kernels.
@@ -106,6 +110,24 @@ __device__ inline float4 ld_pc(const RenderArgs &A, uint32_t p) {
   if (!A.q16) return A.pc[p];
   const uint4 w = A.qp[p];
   return make_float4(aa_cube(un16(w.z & 0xffffu)), h16(w.z >> 16), h16(w.w & 0xffffu), h16(w.w >> 16));
+}
+// Everything a candidate test reads of particle p -- position and radius,
+// colour payload and RBF coefficient -- with every load issued before any
+// value is used (one memory round trip; the same values as ld_pr, ld_pc and
+// A.pm[p]).
+__device__ inline void ld_cand(const RenderArgs &A, uint32_t p, float4 &pr, float4 &pc, float &m) {
+  if (!A.q16) {
+    pr = A.pr[p];
+    pc = A.pc[p];
+    m = A.pm[p];
+    return;
+  }
+  const uint4 w = A.qp[p];
+  const float4 o = A.qc[2 * (p >> kQChunkLog2)], s = A.qc[2 * (p >> kQChunkLog2) + 1];
+  m = A.pm[p];
+  pr = make_float4(fx16(w.x & 0xffffu, s.x, o.x), fx16(w.x >> 16, s.y, o.y), fx16(w.y & 0xffffu, s.z, o.z),
+                   fx16_r(w.y >> 16, s.w, o.w));
+  pc = make_float4(aa_cube(un16(w.z & 0xffffu)), h16(w.z >> 16), h16(w.w & 0xffffu), h16(w.w >> 16));
 }
 
 namespace {

@@ -7,17 +7,19 @@
 // round drains a ray before it re-queues), the step kernel's occupancy
 // target (waves per SIMD, which sets its register budget, i.e. the
 // workgroup shape on a CU) and the number of rounds captured per hipGraph
-// (0 = host-driven rounds), and the node budget of a one-lane gather walk
+// (0 = host-driven rounds), the node budget of a one-lane gather walk
 // before it hands its ray to a one-wave gather (how many walks the wave
-// gather takes depends on the scene).  None changes an image, only its
-// cost.  The BVH
+// gather takes depends on the scene), and the live-lane count below which a
+// step-kernel wave leaves its loop so its rays are repacked in the next
+// round.  None changes an image, only its cost.  The BVH
 // leaf size is a property of the scene and is tuned one level up, by
 // rebuilding the scene per leaf size (aarbf.tune in the Python package).
 //
 // Search: coordinate descent from the current setting -- the step budget
-// over {1024, 2048, 4096, 8192}, then the occupancy over {4, 5, 6}, then the
-// graph rounds over {0, 8}, then the walk budget over {256, 1024, 65536} --
-// each candidate timed on the stream with HIP
+// over {2048, 4096, 8192, 16384}, then the occupancy over {4, 5, 6}, then the
+// graph rounds over {0, 8}, then the walk budget over {256, 1024, 65536},
+// then the step kernel's live-lane threshold over {0, 16} -- each candidate
+// timed on the stream with HIP
 // events (best of `reps` runs after one warm-up run).  The views render
 // into the tuner's own buffers; the caller's images are not touched.
 #include <vector>
@@ -54,12 +56,13 @@ int aarbf_autotune(aarbf_context *ctx, const aarbf_request *reqs, uint32_t n, ui
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = AARBF_OK;
   const int keep_cap = ctx->tune_step_cap, keep_waves = ctx->tune_step_waves, keep_graph = ctx->tune_graph_rounds,
-            keep_budget = ctx->tune_walk_budget;
-  auto run = [&](int cap, int waves, int graph, int budget, double &ms) -> int {
+            keep_budget = ctx->tune_walk_budget, keep_live = ctx->tune_step_min_live;
+  auto run = [&](int cap, int waves, int graph, int budget, int live, double &ms) -> int {
     ctx->tune_step_cap = cap;
     ctx->tune_step_waves = waves;
     ctx->tune_graph_rounds = graph;
     ctx->tune_walk_budget = budget;
+    ctx->tune_step_min_live = live;
     ms = 1e300;
     for (uint32_t r = 0; r <= reps; ++r) {
       AARBF_HIP(ctx, hipMemsetAsync(buf, 0, px * 2 * 4 * sizeof(float), ctx->stream));
@@ -79,17 +82,18 @@ int aarbf_autotune(aarbf_context *ctx, const aarbf_request *reqs, uint32_t n, ui
       rc = set_error(ctx, AARBF_ERR_HIP, "aarbf_autotune: events");
       break;
     }
-    int best_cap = keep_cap, best_waves = keep_waves, best_graph = keep_graph, best_budget = keep_budget;
+    int best_cap = keep_cap, best_waves = keep_waves, best_graph = keep_graph, best_budget = keep_budget,
+        best_live = keep_live;
     double best = 0.0;
-    if ((rc = run(best_cap, best_waves, best_graph, best_budget, best)) != AARBF_OK) break;
+    if ((rc = run(best_cap, best_waves, best_graph, best_budget, best_live, best)) != AARBF_OK) break;
     out->baseline_ms = best;
     uint32_t tried = 1;
-    const int caps[] = {1024, 2048, 4096, 8192}, waves[] = {4, 5, 6}, graphs[] = {0, 8},
-              budgets[] = {256, 1024, 65536};
+    const int caps[] = {2048, 4096, 8192, 16384}, waves[] = {4, 5, 6}, graphs[] = {0, 8},
+              budgets[] = {256, 1024, 65536}, lives[] = {0, 16};
     for (int c : caps) {
       if (c == best_cap) continue;
       double ms;
-      if ((rc = run(c, best_waves, best_graph, best_budget, ms)) != AARBF_OK) break;
+      if ((rc = run(c, best_waves, best_graph, best_budget, best_live, ms)) != AARBF_OK) break;
       ++tried;
       if (ms < best) best = ms, best_cap = c;
     }
@@ -97,7 +101,7 @@ int aarbf_autotune(aarbf_context *ctx, const aarbf_request *reqs, uint32_t n, ui
     for (int w : waves) {
       if (w == (best_waves ? best_waves : 4)) continue;
       double ms;
-      if ((rc = run(best_cap, w, best_graph, best_budget, ms)) != AARBF_OK) break;
+      if ((rc = run(best_cap, w, best_graph, best_budget, best_live, ms)) != AARBF_OK) break;
       ++tried;
       if (ms < best) best = ms, best_waves = w;
     }
@@ -105,7 +109,7 @@ int aarbf_autotune(aarbf_context *ctx, const aarbf_request *reqs, uint32_t n, ui
     for (int g : graphs) {
       if (g == best_graph) continue;
       double ms;
-      if ((rc = run(best_cap, best_waves, g, best_budget, ms)) != AARBF_OK) break;
+      if ((rc = run(best_cap, best_waves, g, best_budget, best_live, ms)) != AARBF_OK) break;
       ++tried;
       if (ms < best) best = ms, best_graph = g;
     }
@@ -113,15 +117,24 @@ int aarbf_autotune(aarbf_context *ctx, const aarbf_request *reqs, uint32_t n, ui
     for (int b : budgets) {
       if (b == best_budget) continue;
       double ms;
-      if ((rc = run(best_cap, best_waves, best_graph, b, ms)) != AARBF_OK) break;
+      if ((rc = run(best_cap, best_waves, best_graph, b, best_live, ms)) != AARBF_OK) break;
       ++tried;
       if (ms < best) best = ms, best_budget = b;
+    }
+    if (rc != AARBF_OK) break;
+    for (int l : lives) {
+      if (l == best_live) continue;
+      double ms;
+      if ((rc = run(best_cap, best_waves, best_graph, best_budget, l, ms)) != AARBF_OK) break;
+      ++tried;
+      if (ms < best) best = ms, best_live = l;
     }
     if (rc != AARBF_OK) break;
     out->step_cap = best_cap;
     out->step_waves = best_waves;
     out->graph_rounds = best_graph;
     out->walk_budget = best_budget;
+    out->step_min_live = best_live;
     out->best_ms = best;
     out->candidates = tried;
   } while (false);
@@ -134,11 +147,13 @@ int aarbf_autotune(aarbf_context *ctx, const aarbf_request *reqs, uint32_t n, ui
     ctx->tune_step_waves = out->step_waves;
     ctx->tune_graph_rounds = out->graph_rounds;
     ctx->tune_walk_budget = out->walk_budget;
+    ctx->tune_step_min_live = out->step_min_live;
   } else {
     ctx->tune_step_cap = keep_cap;
     ctx->tune_step_waves = keep_waves;
     ctx->tune_graph_rounds = keep_graph;
     ctx->tune_walk_budget = keep_budget;
+    ctx->tune_step_min_live = keep_live;
   }
   return rc;
 }

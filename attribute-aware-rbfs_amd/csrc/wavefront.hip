@@ -41,6 +41,7 @@
 #include <type_traits>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -78,11 +79,16 @@ constexpr uint32_t kWaveSlice = AARBF_WAVE_SLICE; // nodes a lane opens between 
 // Empty-space steps a lane may take per iteration of the step loop: a wave
 // iterates as long as its slowest lane, and most steps are empty-space
 // steps, so taking several per iteration cuts the iterations (and the
-// per-iteration overhead every lane pays) roughly by this factor.
+// per-iteration overhead every lane pays) roughly by this factor.  Each
+// such step is 29 VALU instructions (LCG, 1 - xi, logarithm, t update and
+// compare); 16 measured best against 8, 10, 12, 14, 20 and 24.
 #ifndef AARBF_FAST_STEPS
-#define AARBF_FAST_STEPS 8
+#define AARBF_FAST_STEPS 16
 #endif
 constexpr int kFastSteps = AARBF_FAST_STEPS;
+// Step queues at least this long let sparse waves leave early (RenderArgs
+// min_live); shorter ones (the end of a pass) would only add rounds.
+constexpr uint32_t kMinLiveQueue = 65536;
 
 constexpr uint32_t F_SHADOW = 1u, F_PENDING = 2u, F_DENSE = 4u;
 // F_HEAD: the segment is the pixel's head list (shared by the primary rays
@@ -1068,6 +1074,8 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
   };
   Counters c = {};
   uint64_t simt[3] = {0, 0, 0};
+  // early leaving only while the queue can fill the GPU again
+  const bool may_leave = A.min_live != 0u && n_a >= kMinLiveQueue;
   // grid-stride, so a fixed grid (a captured graph) covers any count
   for (uint32_t base = blockIdx.x * kSBlock; base < n_a; base += gridDim.x * kSBlock) {
   const uint32_t gid = base + threadIdx.x;
@@ -1141,8 +1149,15 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
     };
     setup();
 
-    uint32_t steps = 0;
+    uint32_t steps = 0, it = 0;
     for (;;) {
+      // a wave whose lanes have mostly left (for gathers) stops after at
+      // least one iteration; its remaining rays go on in the next round,
+      // packed into full waves, instead of stepping at low lane occupancy
+      if (may_leave && ++it > 1u && (uint32_t)__popcll(__ballot(1)) < A.min_live) {
+        outcome = 1;
+        break;
+      }
       if (STATS) { // SIMT accounting: iterations, lanes in them, iterations touching candidates
         uint64_t live = __ballot(1);
         if ((uint32_t)__ffsll((unsigned long long)live) - 1u == (threadIdx.x & 63u)) {
@@ -1170,17 +1185,26 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
         const uint32_t room = min(A.max_depth - (L.i + 1u), step_cap - (steps + 1u));
         uint32_t taken = 1;
         // one unrolled loop per tracker (the choice is uniform in a wave
-        // of one view)
+        // of one view); the budgets are checked per draw only where fewer
+        // than kFastSteps - 1 steps are left (a rolled loop, rarely taken)
         auto more = [&](auto marching) {
-#pragma unroll
-          for (int u = 1; u < kFastSteps; ++u) {
-            if ((uint32_t)u > room) break;
+          auto draw = [&]() {
             uint32_t ns2 = L.trng;
             const float tn2 = decltype(marching)::value ? L.t + A.unit : L.t - flight_log(ns2) * A.unit;
-            if (!(tn2 < lim)) break;
+            if (!(tn2 < lim)) return false;
             L.trng = ns2;
             L.t = tn2;
             ++taken;
+            return true;
+          };
+          if (room >= (uint32_t)(kFastSteps - 1)) {
+#pragma unroll
+            for (int u = 1; u < kFastSteps; ++u)
+              if (!draw()) break;
+          } else {
+#pragma nounroll
+            for (uint32_t u = 1; u <= room; ++u)
+              if (!draw()) break;
           }
         };
         if (march) more(std::true_type{});
@@ -1222,16 +1246,9 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
         pl.count = 0;
         pl.dens = 0.0f;
         pl.r = pl.g = pl.b = 0.0f;
-        uint32_t m = act;
-        while (m) {
-          uint32_t k = (uint32_t)__ffs(m) - 1u;
-          m &= m - 1u;
-          uint32_t p = li[k * kSBlock];
-          // both halves of the payload in one round trip
-          float4 qq = ld_pr(A, p);
-          float4 kk = ld_pc(A, p);
+        auto test = [&](const float4 &qq, const float4 &kk, float pm) {
           if (STATS) c.particles++;
-          const float e = aa_dist2(qq.x, qq.y, qq.z, x, y, z) * A.pm[p];
+          const float e = aa_dist2(qq.x, qq.y, qq.z, x, y, z) * pm;
           if (e > -4.5f) {
             float wa = aa_expf_mid(e) * kk.x;
             if (STATS) c.covering++;
@@ -1242,6 +1259,22 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
             pl.b += kk.w * wa;
             if (!A.visAttr) pl.dens = fminf(pl.dens, 1.0f);
           }
+        };
+        // two active candidates per pass (ascending list position, i.e.
+        // ascending particle index), their loads in one round trip
+        uint32_t m = act;
+        while (m) {
+          const uint32_t k0 = (uint32_t)__ffs(m) - 1u;
+          m &= m - 1u;
+          const bool two = m != 0u;
+          const uint32_t k1 = two ? (uint32_t)__ffs(m) - 1u : k0;
+          m &= m - 1u;
+          float4 q0, c0, q1, c1;
+          float m0, m1;
+          ld_cand(A, li[k0 * kSBlock], q0, c0, m0);
+          ld_cand(A, li[k1 * kSBlock], q1, c1, m1);
+          test(q0, c0, m0);
+          if (two) test(q1, c1, m1);
         }
         L.flags &= ~F_PENDING;
         const bool shadow = (L.flags & F_SHADOW) != 0;
@@ -1841,6 +1874,23 @@ __global__ void wf_lat_start(unsigned long long *lat, uint32_t nv) {
 // step from the host's counts (gb, sb), the dense kernel from its count read
 // back after the step (db == 0), or all three fixed (graph capture; the
 // kernels read their counts on the device and loop over them).
+// Step budget per ray per round for a pass over nslots slots: the tuned
+// base for queues up to 8x the lanes the GPU keeps resident (4 waves per
+// SIMD), scaled by (queue / that)^(2/3) above it and rounded to the nearest
+// power of two, at most 65536.  A round's step launch ends with a tail in
+// which its last waves run alone; a long queue has many waves per resident
+// slot, so its tail is a small part of the launch and longer rounds (fewer
+// exits at the budget, fewer launches) pay off.  Measured with waves
+// leaving at 16 live lanes: 1920x1080 (2.1M slots, 8x the resident lanes
+// of 256 CUs) best at 8192, eight such frames in one batch (16.6M, 63x) at
+// 32768 (16384: 2.5 % slower, 65536: 1 %).
+static uint32_t step_budget(const aarbf_context *ctx, uint64_t nslots) {
+  const double resident = 16.0 * 64.0 * (double)std::max(ctx->cus, 1); // lanes: 4 waves x 4 SIMDs per CU
+  const double f = std::max(1.0, std::cbrt(std::pow((double)nslots / (8.0 * resident), 2.0)));
+  const double cap = std::exp2(std::round(std::log2((double)ctx->tune_step_cap * f)));
+  return (uint32_t)std::min(65536.0, std::max((double)ctx->tune_step_cap, cap));
+}
+
 template <bool B>
 static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &S, uint32_t *const lists[4],
                         uint32_t *listD, uint32_t *listL, uint32_t *cnt, int cur, uint32_t nG, uint32_t gb,
@@ -1887,7 +1937,7 @@ static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &
   wf_zero<<<1, 64, 0, ctx->stream>>>(cN, 2);
   wf_zero<<<1, 64, 0, ctx->stream>>>(cnt + 4, 1);
   if (tm) AARBF_HIP(ctx, hipEventRecord(ctx->tev[1], ctx->stream));
-  const uint32_t cap = (uint32_t)ctx->tune_step_cap;
+  const uint32_t cap = step_budget(ctx, S.nslots);
   if (ctx->render_mode == 2 && !A.blue) {
     if (st)
       wf_step_m<true, B><<<sb, kSBlock, 0, ctx->stream>>>(A, S, lA, cA, nA2, nG2, cN, listD, cnt + 4, cap);
@@ -2139,6 +2189,7 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A0, const ReqArgs *vi
   A.tile_req = nullptr;
   A.nreq = nv;
   A.lat = nullptr;
+  A.min_live = (uint32_t)ctx->tune_step_min_live;
   AARBF_REQUIRE(ctx, batch || nv == 1, "render_wavefront: several views need the batch schedule");
   if (batch) {
     // the view table and the view of every tile, then the latency record

@@ -35,7 +35,7 @@ def log(*a):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8, help="timed frames")
+    ap.add_argument("--steps", type=int, default=16, help="timed frames")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp", type=int, default=16)
     ap.add_argument("--orbit-frames", type=int, default=32)
@@ -46,8 +46,10 @@ def main() -> int:
     ap.add_argument("--mode", type=int, default=0, help="0 wavefront gather/step kernels, 1 point queries")
     ap.add_argument("--payload", type=int, default=0, help="0 FP32 particle payload, 1 FP16 leaf-local quantised")
     ap.add_argument("--step-waves", type=int, default=0)
-    ap.add_argument("--step-cap", type=int, default=4096)
+    ap.add_argument("--step-cap", type=int, default=8192)
     ap.add_argument("--graph-rounds", type=int, default=0, help="rounds per captured hipGraph (0 = host-driven)")
+    ap.add_argument("--step-min-live", type=int, default=16,
+                    help="step-kernel waves leave their loop below this many live lanes (0 = never)")
     ap.add_argument("--walk-budget", type=int, default=256,
                     help="nodes a one-lane gather walk opens before its ray goes to a one-wave gather")
     ap.add_argument("--cpu-threads", type=int, default=0,
@@ -62,9 +64,9 @@ def main() -> int:
                     help="camera requests of the batching leg (0 = skip the leg)")
     ap.add_argument("--batch-particles", type=int, default=16777216)
     ap.add_argument("--batch-payload", type=int, default=1)
-    ap.add_argument("--frame-batch", type=int, default=8,
-                    help="orbit frames of the main workload per aarbf_render_batch call in the frame-batch leg "
-                         "(FP16 and FP32 payload; 0 = skip)")
+    ap.add_argument("--frame-batch", type=int, default=32,
+                    help="the frame-batch leg renders the timed frames (at most this many) in one "
+                         "aarbf_render_batch call, FP16 and FP32 payload (0 = skip)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "render_traffic.json"),
                     help="HBM bytes per render launch measured with rocprofv3 --pmc")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -104,6 +106,7 @@ def main() -> int:
     ctx.set_tuning(ctx.TUNE_STEP_CAP, a.step_cap)
     ctx.set_tuning(ctx.TUNE_GRAPH_ROUNDS, a.graph_rounds)
     ctx.set_tuning(ctx.TUNE_WALK_BUDGET, a.walk_budget)
+    ctx.set_tuning(ctx.TUNE_STEP_MIN_LIVE, a.step_min_live)
     torch.cuda.synchronize()
     tp = time.time()
     sc = aarbf.Scene.prepare(ctx, wl.xyzw, wl.colormap, wl.radiusmap, wl.densitymap, wl.rbf_radius,
@@ -301,21 +304,27 @@ def main() -> int:
 
 
 def frame_batch_leg(ctx, stream, a, sc, wl, cams, plain_msps):
-    """The main workload's orbit frames 1..K (1920x1080, 16 spp delta
-    tracking each) rendered by one aarbf_render_batch call, with the FP16
-    leaf-local payload and with the FP32 payload, against the plain FP32
-    path (one aarbf_render_rbf_spp call per frame, the line's value).  Each
-    view's image equals its own call's on the same payload bit for bit."""
+    """The frames the line's value is timed on (the main workload's orbit
+    frames warmup .. warmup + steps - 1, 1920x1080, 16 spp delta tracking
+    each; at most --frame-batch of them) rendered by one aarbf_render_batch
+    call, with the FP16 leaf-local payload and with the FP32 payload,
+    against the plain FP32 path (one aarbf_render_rbf_spp call per frame,
+    the line's value; over fewer frames than it times, the same frames are
+    also timed one call each).  Each view's image equals its own call's on
+    the same payload bit for bit."""
     import torch
-    W, H, S, K = wl.width, wl.height, wl.spp, a.frame_batch
+    W, H, S = wl.width, wl.height, wl.spp
+    K = min(a.steps, a.frame_batch)
+    f0 = a.warmup
     bufs = [(torch.zeros((H, W, 4), dtype=torch.float32, device="cuda"),
              torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")) for _ in range(K)]
     reqs = []
     for k in range(K):
-        f = 1 + k
+        f = f0 + k
         pc = sc.push_constants(cams[f % len(cams)], wl.unit, frame_id=(f * S) & 0xFFFF, accum_id=1)
         reqs.append((sc.raygen_data(bufs[k][0], bufs[k][1], W, H), pc, W, H, S))
-    out = {"workload": f"orbit frames 1..{K} of the main workload, one aarbf_render_batch call", "frames": K}
+    out = {"workload": f"orbit frames {f0}..{f0 + K - 1} of the main workload (the timed frames), one "
+                       f"aarbf_render_batch call", "frames": K}
     for payload, name in ((1, "fp16"), (0, "fp32")):
         ctx.set_payload(payload)
         ctx.render_batch(reqs)  # warm-up (slot pool, payload)
@@ -329,15 +338,17 @@ def frame_batch_leg(ctx, stream, a, sc, wl, cams, plain_msps):
         out[f"{name}_batched_ms"] = round(ms, 2)
         out[f"{name}_batched_msamples_per_s"] = round(W * H * S * K / (ms * 1e-3) / 1e6, 3)
         log(f"frame batch ({name} payload): {K} frames in {ms:.1f} ms")
-        if payload == 1:  # the same frames one call each
-            e0.record(stream)
-            for rq in reqs:
-                ctx.render(*rq)
-            e1.record(stream)
-            torch.cuda.synchronize()
-            ms1 = e0.elapsed_time(e1)
-            out["fp16_one_call_msamples_per_s"] = round(W * H * S * K / (ms1 * 1e-3) / 1e6, 3)
+        # the same frames one call each
+        e0.record(stream)
+        for rq in reqs:
+            ctx.render(*rq)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms1 = e0.elapsed_time(e1)
+        out[f"{name}_one_call_msamples_per_s"] = round(W * H * S * K / (ms1 * 1e-3) / 1e6, 3)
     ctx.set_payload(0)
+    if plain_msps and K < a.steps:
+        plain_msps = out["fp32_one_call_msamples_per_s"]
     if plain_msps:
         out["plain_fp32_msamples_per_s"] = round(plain_msps, 3)
         out["fp16_batched_over_plain_fp32"] = round(out["fp16_batched_msamples_per_s"] / plain_msps, 3)

@@ -160,7 +160,10 @@ int aarbf_set_render_mode(aarbf_context *ctx, int mode);
 /* Schedule parameters of the default renderer (none changes the image):
  * AARBF_TUNE_STEP_WAVES: occupancy target of the step kernel in waves per
  * SIMD (0 = default 4, or 4..6); AARBF_TUNE_STEP_CAP: tracker steps
- * per ray per wavefront round (16..65536, default 4096);
+ * per ray per wavefront round (16..65536, default 8192) for a call of up
+ * to 8x the lanes the GPU keeps resident (4 waves per SIMD: 2,097,152
+ * pixels on 256 CUs); a call with more pixels scales it by
+ * (pixels / that)^(2/3), to the nearest power of two, at most 65536;
  * AARBF_TUNE_GRAPH_ROUNDS: 0 (default) = the host reads the queue lengths
  * back after every round and sizes the next round's grids; an even R in
  * 2..64 = R rounds with fixed grids are captured once into a hipGraph
@@ -171,13 +174,18 @@ int aarbf_set_render_mode(aarbf_context *ctx, int mode);
  * AARBF_TUNE_DIFF_WAVES: the differentiable renderer's expected marching
  * (estimator 0) with one lane per (pixel, sample) (0) or one wave per
  * (pixel, sample), its lanes on consecutive march points (1, default;
- * ratio tracking always runs one lane per sample).  No counterpart in the
- * reference. */
+ * ratio tracking always runs one lane per sample);
+ * AARBF_TUNE_STEP_MIN_LIVE: a step-kernel wave leaves its loop once fewer
+ * than this many of its 64 lanes are still stepping (their rays go on in
+ * the next round, packed into full waves with the others); 0 = never,
+ * else 1..64 (default 16), applied only while the step queue holds at
+ * least 65,536 rays.  No counterpart in the reference. */
 #define AARBF_TUNE_STEP_WAVES 1
 #define AARBF_TUNE_STEP_CAP 2
 #define AARBF_TUNE_GRAPH_ROUNDS 3
 #define AARBF_TUNE_WALK_BUDGET 4
 #define AARBF_TUNE_DIFF_WAVES 5
+#define AARBF_TUNE_STEP_MIN_LIVE 6
 int aarbf_set_tuning(aarbf_context *ctx, int param, int value);
 /* Enable (1) or disable (0) render statistics; read them back (and reset
  * them) with aarbf_render_stats_read. */
@@ -388,9 +396,10 @@ int aarbf_render_batch(aarbf_context *ctx, const aarbf_request *reqs, uint32_t n
 int aarbf_batch_latency(aarbf_context *ctx, double *ms, uint32_t n);
 /* Schedule autotuner: renders the views (into its own buffers; the
  * callers' images are untouched) under candidate schedule parameters --
- * AARBF_TUNE_STEP_CAP over {1024, 2048, 4096, 8192}, AARBF_TUNE_STEP_WAVES
+ * AARBF_TUNE_STEP_CAP over {2048, 4096, 8192, 16384}, AARBF_TUNE_STEP_WAVES
  * over {4, 5, 6}, AARBF_TUNE_GRAPH_ROUNDS over {0, 8},
- * AARBF_TUNE_WALK_BUDGET over {256, 1024, 65536}, by coordinate descent
+ * AARBF_TUNE_WALK_BUDGET over {256, 1024, 65536},
+ * AARBF_TUNE_STEP_MIN_LIVE over {0, 16}, by coordinate descent
  * from the current setting, each timed as the best of `reps` runs after a
  * warm-up -- and keeps the fastest in the context.  The BVH leaf size is a
  * scene property, tuned by rebuilding the scene per leaf size around this
@@ -401,6 +410,7 @@ typedef struct aarbf_tune_result {
   double baseline_ms;   /* the setting at entry */
   double best_ms;       /* the setting kept */
   int32_t walk_budget;
+  int32_t step_min_live; /* AARBF_TUNE_STEP_MIN_LIVE kept */
 } aarbf_tune_result;
 int aarbf_autotune(aarbf_context *ctx, const aarbf_request *reqs, uint32_t n, uint32_t reps,
                    aarbf_tune_result *out);

@@ -106,11 +106,14 @@ AA_FN float aa_logf(float x) {
  * is identical bit for bit; checked exhaustively over the domain). */
 AA_FN float aa_log_step(float x) {
   uint32_t u = aa_f2u(x);
-  int e = (int)(u >> 23) - 127;
-  uint32_t mb = (u & 0x007fffffu) | 0x3f800000u;
-  const int hi = mb > 0x3fb504f3u;
-  mb = hi ? mb - 0x00800000u : mb;
-  e = hi ? e + 1 : e;
+  /* The same m and e as aa_logf's "m > sqrt(2)" adjustment, from one
+   * integer subtraction: u - bits(0x3f3504f4) has the exponent of
+   * x / m in its top bits (arithmetic shift; m in [0x3f3504f4, 0x3fb504f3]
+   * as a float), and removing that exponent from u leaves m.  x = 0 gives
+   * e = -127, m = 1 as aa_logf does, before e is replaced by -inf. */
+  const uint32_t tmp = u - 0x3f3504f4u;
+  int e = (int)((int32_t)tmp >> 23);
+  uint32_t mb = u - (tmp & 0xff800000u);
   float f = aa_u2f(mb) - 1.0f;
   float q = -0.07866667956113815f;
   q = AA_FMAF(q, f, 0.12821748852729797f);

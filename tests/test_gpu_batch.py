@@ -183,9 +183,10 @@ def test_autotune_keeps_images_and_caller_buffers(gpu_ctx):
     try:
         r = gpu_ctx.autotune(reqs, reps=1)
         print("autotune:", r)
-        assert r["step_cap"] in (1024, 2048, 4096, 8192) and r["step_waves"] in (0, 4, 5, 6)
+        assert r["step_cap"] in (2048, 4096, 8192, 16384) and r["step_waves"] in (0, 4, 5, 6)
         assert r["graph_rounds"] in (0, 8) and r["walk_budget"] in (256, 1024, 65536)
-        assert r["candidates"] >= 9 and 0 < r["best_ms"] <= r["baseline_ms"]
+        assert r["step_min_live"] in (0, 16)
+        assert r["candidates"] >= 10 and 0 < r["best_ms"] <= r["baseline_ms"]
         for v, (i0, a0) in zip(probe, before):  # untouched
             assert np.array_equal(v[1].cpu().numpy(), i0) and np.array_equal(v[2].cpu().numpy(), a0)
         got = _views(gpu_ctx, sc, VIEWS, stbn)
@@ -206,8 +207,9 @@ def test_autotune_keeps_images_and_caller_buffers(gpu_ctx):
         assert info["particles_per_leaf"] in (1, 2) and best.ppl == info["particles_per_leaf"]
     finally:
         for k in (gpu_ctx.TUNE_STEP_CAP, gpu_ctx.TUNE_STEP_WAVES, gpu_ctx.TUNE_GRAPH_ROUNDS):
-            gpu_ctx.set_tuning(k, 4096 if k == gpu_ctx.TUNE_STEP_CAP else 0)
+            gpu_ctx.set_tuning(k, 8192 if k == gpu_ctx.TUNE_STEP_CAP else 0)
         gpu_ctx.set_tuning(gpu_ctx.TUNE_WALK_BUDGET, 256)
+        gpu_ctx.set_tuning(gpu_ctx.TUNE_STEP_MIN_LIVE, 16)
 
 
 @pytest.mark.parametrize("rounds", [0, 4])

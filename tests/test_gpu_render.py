@@ -194,6 +194,37 @@ def test_taa_matches_oracle(gpu_ctx, gui_layer, disable):
                           f"{oframe[tuple(bad[0])]:08x}"
 
 
+def test_sparse_waves_leaving_early_is_exact(gpu_ctx):
+    """Step-kernel waves that leave their loop once fewer than
+    AARBF_TUNE_STEP_MIN_LIVE lanes still step (their rays go on in the next
+    round) render bit for bit the same image as waves that run until their
+    last lane leaves.  The view has 81,920 pixels, so the step queue passes
+    the 65,536 rays from which the early exit applies; thresholds 0 (never),
+    16 (the default) and 48 (waves leave when 17 lanes are gone)."""
+    W, H = 512, 160
+    xyzw, radii = random_scene(20000, seed=91, extent=1.0, r=(0.02, 0.06))
+    tfs = (scenes.ramp_table(256), scenes.constant_table(n=4), scenes.ramp_table(64))
+    sc, op, orad = _setup(gpu_ctx, xyzw, radii, tfs, 0.06)
+    cam = default_camera(W, H, look_from=(0.2, 0.3, 2.2))
+    kw = dict(light=(0.3, 0.2, 0.4))
+    out = {}
+    try:
+        for live in (0, 16, 48):
+            gpu_ctx.set_tuning(gpu_ctx.TUNE_STEP_MIN_LIVE, live)
+            out[live] = _gpu_render(gpu_ctx, sc, cam, W, H, 0.004, spp=2, frame=5, **kw)
+    finally:
+        gpu_ctx.set_tuning(gpu_ctx.TUNE_STEP_MIN_LIVE, 16)
+    img0, acc0 = out[0]
+    assert (acc0[..., :3] > 0).mean() > 0.1
+    for live in (16, 48):
+        _compare(out[live][1], acc0, f"accum, min-live {live} vs 0")
+        _compare(out[live][0], img0, f"image, min-live {live} vs 0")
+    px = np.array([(x, y) for y in range(2, H, 13) for x in range(3, W, 29)], np.int32)
+    r = oracle.Render(op, orad, sc.aabb, *tfs, cam, W, H, 0.004, 0.06, frame_id=5, accum_id=1, **kw)
+    oimg, oacc, _ = r.run(pixels=px, spp=2, threads=8)
+    _compare(acc0[px[:, 1], px[:, 0]], oacc[px[:, 1], px[:, 0]], "accum vs oracle")
+
+
 @pytest.mark.parametrize("blue_noise", [False, True])
 def test_walk_budget_handoff_is_exact(gpu_ctx, blue_noise):
     """Gather walks that open more nodes than the budget hand their ray to
