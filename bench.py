@@ -278,6 +278,7 @@ def main() -> int:
             "payload": ("fp32", "fp16 leaf-local quantised")[a.payload], "max_depth": aarbf.MAX_DEPTH,
             "exhausted_fraction": st["exhausted"] / samples, "preprocess_ms": round(prep_s * 1e3, 1),
             "refit_ms": round(refit_ms, 2), "rebuild_ms": round(rebuild_ms, 2),
+            "step_cap": a.step_cap, "step_min_live": a.step_min_live,
             "parallelism": "single GPU", "timed_schedule": "host-driven rounds, no per-kernel events",
         },
         "roofline": {
