@@ -453,11 +453,27 @@ __global__ __launch_bounds__(256) void wf_fold(RenderArgs A, WFState S) {
 }
 
 // ---- gather --------------------------------------------------------------------
+// One lane per ray, persistent: lanes take rays from the queue through a
+// shared head counter (one atomic per wave for the lanes that take one).
+// Walk lengths differ by an order of magnitude between rays, and a wave
+// steps as long as its longest walk, so a wave in which fewer than
+// kGatherRefill lanes are still walking stops stepping: the lanes whose
+// walks ended write their lists and take new rays together, and the wave
+// steps on.  Each ray's walk is the same as with one ray per lane for the
+// whole launch, so the lists are too.  Measured on the 1080p workload
+// (ms per frame): one ray per lane 360, refilling below 16 / 32 / 48 / 56 /
+// 62 walking lanes 351 / 350 / 348 / 350 / 369 (walk budget 256; 512 and
+// 1024 were 4-6 % slower with refills).
+#ifndef AARBF_GATHER_REFILL
+#define AARBF_GATHER_REFILL 48
+#endif
+constexpr uint32_t kGatherRefill = AARBF_GATHER_REFILL;
+
 template <bool STATS>
 __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderArgs A, WFState S, const uint32_t *__restrict__ listG,
                                                      const uint32_t *__restrict__ countG,
                                                      uint32_t *__restrict__ listL, uint32_t *__restrict__ countL,
-                                                     uint32_t budget) {
+                                                     uint32_t budget, uint32_t *__restrict__ head) {
   __shared__ uint32_t s_idx[kCand * kGBlock];
   __shared__ float s_tin[kCand * kGBlock];
   // the walk's per-level visiting words for levels 1 .. top-1, kGBlock
@@ -469,22 +485,119 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
   if (blockIdx.x * kGBlock >= n_g) return; // the whole block: no barrier is skipped
   load_levels(A);
   Counters c = {};
-  // grid-stride, so a fixed grid (a captured graph) covers any count
-  for (uint32_t base = blockIdx.x * kGBlock; base < n_g; base += gridDim.x * kGBlock) {
-  const uint32_t gid = base + threadIdx.x;
-  if (gid < n_g) {
-    const uint32_t slot = listG[gid];
-    uint32_t *gi = s_idx + threadIdx.x;
-    float *gtin = s_tin + threadIdx.x;
-    uint32_t *stk = s_stack + threadIdx.x;
+  uint32_t *gi = s_idx + threadIdx.x;
+  float *gtin = s_tin + threadIdx.x;
+  uint32_t *stk = s_stack + threadIdx.x;
+  const QGroup *__restrict__ Qg = A.bvh.groups;
+  const uint32_t log2K = A.bvh.log2K, K = 1u << log2K, top = A.bvh.top;
+  // the lane's ray and walk
+  uint32_t slot = 0, pix = 0, flags = 0, n = 0, maxpos = 0, nodes0 = 0;
+  uint32_t lvl = 0, base = 0, ord = 0, opened = 0;
+  uint4 u = make_uint4(0u, 0u, 0u, 0u);
+  bool head_list = false, handed = false;
+  float3 o = make_float3(0.0f, 0.0f, 0.0f), d = o;
+  float t0 = 0.0f, tmax = 0.0f, m = 0.0f, maxtin = -AA_INF_F, t_cut = AA_INF_F;
+  SegQuery q;
+  // conservative entry/exit of the particle's sphere widened by m; false
+  // when the line misses it.  Exit parameters are not kept during the
+  // walk (LDS for more resident blocks): the output recomputes them with
+  // the same operations.
+  auto chord = [&](const float4 &P, float &tin, float &tout) -> bool {
+    float vx = P.x - o.x, vy = P.y - o.y, vz = P.z - o.z;
+    float tc = vx * d.x + vy * d.y + vz * d.z;
+    float hx = vx - tc * d.x, hy = vy - tc * d.y, hz = vz - tc * d.z;
+    float rr = P.w + m;
+    float w2 = rr * rr - (hx * hx + hy * hy + hz * hz);
+    if (!(w2 >= 0.0f)) return false;
+    float w = sqrtf(w2);
+    tin = tc - w - m;
+    tout = tc + w + m;
+    return true;
+  };
+  auto take = [&](uint32_t p, const float4 &P) {
+    if (!(P.w > 0.0f)) return;
+    float tin, tout;
+    if (!chord(P, tin, tout)) return;
+    if (!(tout >= t0) || !(tin <= tmax) || !(tin < t_cut)) {
+      if (tout >= t0 && tin <= tmax) t_cut = fminf(t_cut, tin);
+      return;
+    }
+    if (n < kCand) {
+      gi[n * kGBlock] = p;
+      gtin[n * kGBlock] = tin;
+      if (tin > maxtin) {
+        maxtin = tin;
+        maxpos = n;
+      }
+      ++n;
+      return;
+    }
+    if (tin < maxtin) { // evict the latest-entering candidate
+      t_cut = fminf(t_cut, maxtin);
+      gi[maxpos * kGBlock] = p;
+      gtin[maxpos * kGBlock] = tin;
+      maxtin = -AA_INF_F;
+      for (uint32_t k = 0; k < kCand; ++k) {
+        float v = gtin[k * kGBlock];
+        if (v > maxtin) {
+          maxtin = v;
+          maxpos = k;
+        }
+      }
+    } else {
+      t_cut = fminf(t_cut, tin);
+    }
+    q.len = fminf(tmax, t_cut + 4.0f * m) - t0;
+  };
+  // Front-to-back walk over the quantised tree: the hit children of a
+  // node are visited in order of where the segment enters their box, so
+  // the list fills with the nearest candidates first and t_cut prunes the
+  // rest of the tree early.  The per-level visiting order is kept as
+  // nibbles (child + 1) in an LDS word.
+  auto ordered = [&](uint32_t lv, uint32_t node) -> uint32_t {
+    if (STATS) c.nodes += min(K, g_lvl_cnt[lv - 1] - (node << log2K));
+    return ftb_children(A, q, lv, node);
+  };
+  auto leaf_batch = [&](uint32_t lb, uint32_t od) {
+    if (A.ppl == 1) {
+      float4 P[8];
+      uint32_t pid[8];
+#pragma unroll
+      for (uint32_t r = 0; r < 8; ++r) {
+        uint32_t nib = (od >> (4u * r)) & 15u;
+        pid[r] = nib ? lb + nib - 1u : 0xffffffffu;
+        if (nib) P[r] = ld_pr(A, pid[r]);
+      }
+#pragma unroll
+      for (uint32_t r = 0; r < 8; ++r)
+        if (pid[r] != 0xffffffffu) {
+          if (STATS) c.particles++;
+          take(pid[r], P[r]);
+        }
+      return;
+    }
+    for (; od; od >>= 4) {
+      uint32_t leaf = lb + (od & 15u) - 1u;
+      uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
+      for (uint32_t p = p0; p < p1; ++p) {
+        if (STATS) c.particles++;
+        take(p, ld_pr(A, p));
+      }
+    }
+  };
+  // set-up of ray `gid` of the queue; false when its segment misses the
+  // root box (the walk is over at once)
+  auto start = [&](uint32_t gid) -> bool {
+    slot = listG[gid];
     float4 a0 = S.od0[slot], a1 = S.od1[slot];
-    const float3 o = make_float3(a0.x, a0.y, a0.z), d = make_float3(a1.x, a1.y, a1.z);
-    uint4 u = S.u[slot];
-    uint32_t flags = u.z >> 16;
-    const bool head = (flags & F_HEADGATHER) != 0;
-    const uint32_t pix = S.pix[slot];
-    const float t0 = head ? S.seg[slot].x : a0.w, tmax = a1.w;
-    SegQuery q;
+    o = make_float3(a0.x, a0.y, a0.z);
+    d = make_float3(a1.x, a1.y, a1.z);
+    u = S.u[slot];
+    flags = u.z >> 16;
+    head_list = (flags & F_HEADGATHER) != 0;
+    pix = S.pix[slot];
+    t0 = head_list ? S.seg[slot].x : a0.w;
+    tmax = a1.w;
     q.ax = o.x + t0 * d.x;
     q.ay = o.y + t0 * d.y;
     q.az = o.z + t0 * d.z;
@@ -494,142 +607,56 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
     q.ix = safe_inv(d.x);
     q.iy = safe_inv(d.y);
     q.iz = safe_inv(d.z);
-    const float m = 2e-5f * (1.0f + fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) + fmaxf(fabsf(t0), fabsf(tmax)));
+    m = 2e-5f * (1.0f + fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) + fmaxf(fabsf(t0), fabsf(tmax)));
     q.m = m;
     q.len = tmax - t0 + 4.0f * m;
-    uint32_t n = 0, maxpos = 0;
-    float maxtin = -AA_INF_F, t_cut = AA_INF_F;
+    n = 0;
+    maxpos = 0;
+    maxtin = -AA_INF_F;
+    t_cut = AA_INF_F;
+    handed = false;
+    opened = 0;
     if (STATS) c.queries++;
-    const uint32_t nodes0 = c.nodes;
-    // conservative entry/exit of the particle's sphere widened by m; false
-    // when the line misses it.  Exit parameters are not kept during the
-    // walk (LDS for more resident blocks): the output recomputes them with
-    // the same operations.
-    auto chord = [&](const float4 &P, float &tin, float &tout) -> bool {
-      float vx = P.x - o.x, vy = P.y - o.y, vz = P.z - o.z;
-      float tc = vx * d.x + vy * d.y + vz * d.z;
-      float hx = vx - tc * d.x, hy = vy - tc * d.y, hz = vz - tc * d.z;
-      float rr = P.w + m;
-      float w2 = rr * rr - (hx * hx + hy * hy + hz * hz);
-      if (!(w2 >= 0.0f)) return false;
-      float w = sqrtf(w2);
-      tin = tc - w - m;
-      tout = tc + w + m;
-      return true;
-    };
-    auto take = [&](uint32_t p, const float4 &P) {
-      if (!(P.w > 0.0f)) return;
-      float tin, tout;
-      if (!chord(P, tin, tout)) return;
-      if (!(tout >= t0) || !(tin <= tmax) || !(tin < t_cut)) {
-        if (tout >= t0 && tin <= tmax) t_cut = fminf(t_cut, tin);
-        return;
-      }
-      if (n < kCand) {
-        gi[n * kGBlock] = p;
-        gtin[n * kGBlock] = tin;
-        if (tin > maxtin) {
-          maxtin = tin;
-          maxpos = n;
-        }
-        ++n;
-        return;
-      }
-      if (tin < maxtin) { // evict the latest-entering candidate
-        t_cut = fminf(t_cut, maxtin);
-        gi[maxpos * kGBlock] = p;
-        gtin[maxpos * kGBlock] = tin;
-        maxtin = -AA_INF_F;
-        for (uint32_t k = 0; k < kCand; ++k) {
-          float v = gtin[k * kGBlock];
-          if (v > maxtin) {
-            maxtin = v;
-            maxpos = k;
-          }
-        }
-      } else {
-        t_cut = fminf(t_cut, tin);
-      }
-      q.len = fminf(tmax, t_cut + 4.0f * m) - t0;
-    };
-    // Front-to-back walk over the quantised tree: the hit children of a
-    // node are visited in order of where the segment enters their box, so
-    // the list fills with the nearest candidates first and t_cut prunes the
-    // rest of the tree early.  The per-level visiting order is kept as
-    // nibbles (child + 1) in an LDS word.
-    const QGroup *__restrict__ Qg = A.bvh.groups;
-    const uint32_t log2K = A.bvh.log2K, K = 1u << log2K, top = A.bvh.top;
-    auto ordered = [&](uint32_t lvl, uint32_t node) -> uint32_t {
-      if (STATS) c.nodes += min(K, g_lvl_cnt[lvl - 1] - (node << log2K));
-      return ftb_children(A, q, lvl, node);
-    };
-    auto leaf_batch = [&](uint32_t lb, uint32_t ord) {
-      if (A.ppl == 1) {
-        float4 P[8];
-        uint32_t pid[8];
-#pragma unroll
-        for (uint32_t r = 0; r < 8; ++r) {
-          uint32_t nib = (ord >> (4u * r)) & 15u;
-          pid[r] = nib ? lb + nib - 1u : 0xffffffffu;
-          if (nib) P[r] = ld_pr(A, pid[r]);
-        }
-#pragma unroll
-        for (uint32_t r = 0; r < 8; ++r)
-          if (pid[r] != 0xffffffffu) {
-            if (STATS) c.particles++;
-            take(pid[r], P[r]);
-          }
-        return;
-      }
-      for (; ord; ord >>= 4) {
-        uint32_t leaf = lb + (ord & 15u) - 1u;
-        uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
-        for (uint32_t p = p0; p < p1; ++p) {
-          if (STATS) c.particles++;
-          take(p, ld_pr(A, p));
-        }
-      }
-    };
-    // A walk that opens more than `budget` nodes hands its ray to the
-    // long-walk list, which wf_gather_wave takes in the same round: the
-    // launch then lasts as long as its typical walks, not its longest.
-    bool handed = false;
-    {
-      const float4 *root = A.bvh.nodes + g_lvl_ofs[top];
-      if (STATS) c.nodes++;
-      if (seg_box(q, root[0], root[1])) {
-        uint32_t lvl = top - 1, base = 0, opened = 0;
-        uint32_t ord = ordered(top, 0);
-        for (;;) {
-          if (lvl == 0) { // a whole leaf group, in entry order
-            leaf_batch(base, ord);
-            ord = 0;
-          }
-          uint32_t nib = ord & 15u;
-          if (nib == 0) {
-            if (lvl == top - 1) break;
-            ++lvl;
-            ord = stk[(lvl - 1u) * kGBlock];
-            base = ((base >> log2K) >> log2K) << log2K;
-            continue;
-          }
-          if (listL && ++opened > budget) {
-            handed = true;
-            break;
-          }
-          ord >>= 4;
-          uint32_t node = base + nib - 1u;
-          stk[(lvl - 1u) * kGBlock] = ord; // lvl >= 1 here
-          base = node << log2K;
-          ord = ordered(lvl, node);
-          --lvl;
-        }
-      }
+    nodes0 = c.nodes;
+    const float4 *root = A.bvh.nodes + g_lvl_ofs[top];
+    if (STATS) c.nodes++;
+    if (!seg_box(q, root[0], root[1])) return false;
+    lvl = top - 1;
+    base = 0;
+    ord = ordered(top, 0);
+    return true;
+  };
+  // one pass of the walk; false when the walk is over (or handed over: a
+  // walk that opens more than `budget` nodes hands its ray to the
+  // long-walk list, which wf_gather_wave takes in the same round, so a
+  // launch lasts as long as its typical walks, not its longest)
+  auto advance = [&]() -> bool {
+    if (lvl == 0) { // a whole leaf group, in entry order
+      leaf_batch(base, ord);
+      ord = 0;
     }
-    if (handed) {
-      push(listL, countL, slot);
-    } else {
-
+    uint32_t nib = ord & 15u;
+    if (nib == 0) {
+      if (lvl == top - 1) return false;
+      ++lvl;
+      ord = stk[(lvl - 1u) * kGBlock];
+      base = ((base >> log2K) >> log2K) << log2K;
+      return true;
+    }
+    if (listL && ++opened > budget) {
+      handed = true;
+      return false;
+    }
+    ord >>= 4;
+    uint32_t node = base + nib - 1u;
+    stk[(lvl - 1u) * kGBlock] = ord; // lvl >= 1 here
+    base = node << log2K;
+    ord = ordered(lvl, node);
+    --lvl;
+    return true;
+  };
+  // the ray's list and segment end into the slot state
+  auto finish = [&]() {
     if (STATS) {
       uint32_t cost = c.nodes - nodes0;
       uint32_t b = cost ? 31u - (uint32_t)__clz(cost) : 0u;
@@ -642,10 +669,10 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
       n = 0;
     } else {
       flags &= ~F_DENSE;
-      uint32_t *oidx = head ? S.hidx : S.cidx;
-      float *otin = head ? S.htin : S.ctin, *otout = head ? S.htout : S.ctout;
-      uint8_t *oeo = head ? S.heo : S.ceo;
-      const uint64_t row = head ? pix : slot, stride = head ? S.npix : S.nslots;
+      uint32_t *oidx = head_list ? S.hidx : S.cidx;
+      float *otin = head_list ? S.htin : S.ctin, *otout = head_list ? S.htout : S.ctout;
+      uint8_t *oeo = head_list ? S.heo : S.ceo;
+      const uint64_t row = head_list ? pix : slot, stride = head_list ? S.npix : S.nslots;
       seg_end = t_cut;
       // ascending particle order (insertion sort, n <= kCand)
       for (uint32_t a = 1; a < n; ++a) {
@@ -677,7 +704,7 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
         otout[k * stride + row] = tout;
       }
     }
-    if (head) {
+    if (head_list) {
       flags = (flags & ~F_HEADGATHER) | F_HEAD;
       S.head[pix] = make_float4(t0, seg_end, __uint_as_float(n),
                                  __uint_as_float(1u | ((flags & F_DENSE) ? 2u : 0u)));
@@ -687,8 +714,53 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
     S.u[slot] = make_uint4(u.x, u.y, (u.z & 0xffffu) | (flags << 16), n);
     float4 g = S.seg[slot];
     S.seg[slot] = make_float4(t0, seg_end, g.z, g.w);
-    } // not handed
-  }
+  };
+  // rays from the queue for the lanes that ask, in queue order
+  const int lane = threadIdx.x & 63;
+  auto fetch = [&](bool want) -> uint32_t {
+    const uint64_t mask = __ballot(want);
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t b = 0;
+    if (lane == leader) b = atomicAdd(head, (uint32_t)__popcll(mask));
+    b = __shfl(b, leader, 64);
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    return b + (uint32_t)__popcll(mask & lt);
+  };
+  // lanes: walking (a ray whose walk goes on), done (a ray whose walk
+  // ended), or idle (the queue is empty)
+  bool walking = false, done = false, idle = false;
+  for (;;) {
+    // done lanes write their lists; they and the lanes without a ray take
+    // new rays until a walk does not end at its start
+    if (done) {
+      if (handed) push(listL, countL, slot);
+      else finish();
+      done = false;
+    }
+    bool want = !walking && !idle;
+    while (__ballot(want)) {
+      const uint32_t gid = fetch(want);
+      if (want) {
+        if (gid >= n_g) {
+          idle = true;
+        } else if (start(gid)) {
+          walking = true;
+        } else {
+          finish(); // the segment misses the root box: an empty list
+        }
+      }
+      want = !walking && !idle;
+    }
+    if (!__ballot(walking)) break;
+    // step the walks while enough lanes walk (all of them once the queue
+    // is empty)
+    const bool drain = __ballot(idle) != 0;
+    do {
+      if (walking && !advance()) {
+        walking = false;
+        done = true;
+      }
+    } while ((uint32_t)__popcll(__ballot(walking)) >= (drain ? 1u : kGatherRefill));
   }
   flush_stats<STATS>(A, c);
 }
@@ -1913,11 +1985,14 @@ static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &
       // that ran over the budget (a fixed grid looping over their count)
       const size_t stack_bytes = sizeof(uint32_t) * kGBlock * std::max<uint32_t>(A.bvh.top, 2u) - sizeof(uint32_t) * kGBlock;
       uint32_t *cL = cnt + 6;
-      wf_zero<<<1, 64, 0, ctx->stream>>>(cL, 1);
+      wf_zero<<<1, 64, 0, ctx->stream>>>(cL, 2); // long-walk count, queue head (cnt[7])
+      // persistent lanes: enough blocks to fill the GPU, each taking rays
+      // from the queue
+      const uint32_t gp = std::min<uint32_t>(gb, 8u * (uint32_t)std::max(ctx->cus, 1));
       if (st)
-        wf_gather<true><<<gb, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1, listL, cL, budget);
+        wf_gather<true><<<gp, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1, listL, cL, budget, cnt + 7);
       else
-        wf_gather<false><<<gb, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1, listL, cL, budget);
+        wf_gather<false><<<gp, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1, listL, cL, budget, cnt + 7);
       AARBF_CHECK_LAUNCH(ctx, "wf_gather");
       if (st)
         wf_gather_wave<true><<<wfix, 64, wstack, ctx->stream>>>(A, S, listL, cL, ~0u);
