@@ -111,6 +111,21 @@ __device__ inline float4 ld_pc(const RenderArgs &A, uint32_t p) {
   const uint4 w = A.qp[p];
   return make_float4(aa_cube(un16(w.z & 0xffffu)), h16(w.z >> 16), h16(w.w & 0xffffu), h16(w.w >> 16));
 }
+// The FP16 chunk frame of particle p (origin, step), for ld_pr_in: the
+// children of one BVH group (at most 16 consecutive leaves, starting at a
+// multiple of the branching factor) lie in one 32-particle chunk, so a
+// leaf batch loads the frame once.
+__device__ inline void ld_chunk(const RenderArgs &A, uint32_t p, float4 &o, float4 &s) {
+  o = A.qc[2 * (p >> kQChunkLog2)];
+  s = A.qc[2 * (p >> kQChunkLog2) + 1];
+}
+// ld_pr with particle p's chunk frame already loaded (FP16 payload)
+__device__ inline float4 ld_pr_in(const RenderArgs &A, uint32_t p, const float4 &o, const float4 &s) {
+  if (!A.q16) return A.pr[p];
+  const uint4 w = A.qp[p];
+  return make_float4(fx16(w.x & 0xffffu, s.x, o.x), fx16(w.x >> 16, s.y, o.y), fx16(w.y & 0xffffu, s.z, o.z),
+                     fx16_r(w.y >> 16, s.w, o.w));
+}
 // Everything a candidate test reads of particle p -- position and radius,
 // colour payload and RBF coefficient -- with every load issued before any
 // value is used (one memory round trip; the same values as ld_pr, ld_pc and
@@ -537,10 +552,11 @@ __device__ inline void point_query_q(const RenderArgs &A, uint16_t *stk, float x
   };
   auto leaves = [&](uint32_t lb, uint32_t mask) -> bool {
     if (A.ppl == 1) {
-      float4 q[8];
+      float4 q[8], co, cs;
+      if (A.q16) ld_chunk(A, lb, co, cs);
 #pragma unroll
       for (uint32_t k = 0; k < 8; ++k)
-        if ((mask >> k) & 1u) q[k] = ld_pr(A, lb + k);
+        if ((mask >> k) & 1u) q[k] = ld_pr_in(A, lb + k, co, cs);
 #pragma unroll
       for (uint32_t k = 0; k < 8; ++k)
         if ((mask >> k) & 1u) {
@@ -589,10 +605,11 @@ __device__ inline void point_query_b(const RenderArgs &A, uint16_t *stk, float x
   };
   auto leaves = [&](uint32_t lb, uint32_t m) -> bool {
     if (A.ppl == 1) {
-      float4 q[8];
+      float4 q[8], co, cs;
+      if (A.q16) ld_chunk(A, lb, co, cs);
 #pragma unroll
       for (uint32_t k = 0; k < 8; ++k)
-        if (k < m) q[k] = ld_pr(A, lb + k);
+        if (k < m) q[k] = ld_pr_in(A, lb + k, co, cs);
       if (STATS) c.particles += m;
 #pragma unroll
       for (uint32_t k = 0; k < 8; ++k)

@@ -560,13 +560,14 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
   };
   auto leaf_batch = [&](uint32_t lb, uint32_t od) {
     if (A.ppl == 1) {
-      float4 P[8];
+      float4 P[8], co, cs;
       uint32_t pid[8];
+      if (A.q16) ld_chunk(A, lb, co, cs); // the group's leaves share an FP16 chunk
 #pragma unroll
       for (uint32_t r = 0; r < 8; ++r) {
         uint32_t nib = (od >> (4u * r)) & 15u;
         pid[r] = nib ? lb + nib - 1u : 0xffffffffu;
-        if (nib) P[r] = ld_pr(A, pid[r]);
+        if (nib) P[r] = ld_pr_in(A, pid[r], co, cs);
       }
 #pragma unroll
       for (uint32_t r = 0; r < 8; ++r)
@@ -941,13 +942,14 @@ __global__ __launch_bounds__(64) void wf_gather_wave(RenderArgs A, WFState S, co
     };
     auto leaf_batch = [&](uint32_t lb, uint32_t ord) {
       if (A.ppl == 1) {
-        float4 P[8];
+        float4 P[8], co, cs;
         uint32_t pid[8];
+        if (A.q16) ld_chunk(A, lb, co, cs); // the group's leaves share an FP16 chunk
 #pragma unroll
         for (uint32_t r = 0; r < 8; ++r) {
           uint32_t nib = (ord >> (4u * r)) & 15u;
           pid[r] = nib ? lb + nib - 1u : 0xffffffffu;
-          if (nib) P[r] = ld_pr(A, pid[r]);
+          if (nib) P[r] = ld_pr_in(A, pid[r], co, cs);
         }
 #pragma unroll
         for (uint32_t r = 0; r < 8; ++r)
@@ -1333,7 +1335,9 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
           }
         };
         // two active candidates per pass (ascending list position, i.e.
-        // ascending particle index), their loads in one round trip
+        // ascending particle index), their loads in one round trip (three
+        // or four per pass measured 1-2 % slower with FP32 and 2-6 % with
+        // FP16: the absent ones are loaded and decoded too)
         uint32_t m = act;
         while (m) {
           const uint32_t k0 = (uint32_t)__ffs(m) - 1u;
