@@ -129,9 +129,11 @@ __device__ inline float4 ld_pr_in(const RenderArgs &A, uint32_t p, const float4 
 // Everything a candidate test reads of particle p -- position and radius,
 // colour payload and RBF coefficient -- with every load issued before any
 // value is used (one memory round trip; the same values as ld_pr, ld_pc and
-// A.pm[p]).
+// A.pm[p]).  Q: 0 = the payload A holds, 1 = FP32, 2 = FP16 (for kernels
+// instantiated per payload, which A.q16 must match).
+template <int Q = 0>
 __device__ inline void ld_cand(const RenderArgs &A, uint32_t p, float4 &pr, float4 &pc, float &m) {
-  if (!A.q16) {
+  if (Q == 1 || (Q == 0 && !A.q16)) {
     pr = A.pr[p];
     pc = A.pc[p];
     m = A.pm[p];
