@@ -98,9 +98,12 @@ __device__ inline float flight_log(uint32_t &s) {
   return aa_log_step(__builtin_fmaf((float)s, -2.3283064365386963e-10f, 1.0f));
 }
 
-// Baked particle p: position and radius, and (alpha^3, rgb^2.2).
+// Baked particle p: position and radius, and (alpha^3, rgb^2.2).  Q: 0 =
+// the payload A holds, 1 = FP32, 2 = FP16 (kernels instantiated per
+// payload; A.q16 must match).
+template <int Q = 0>
 __device__ inline float4 ld_pr(const RenderArgs &A, uint32_t p) {
-  if (!A.q16) return A.pr[p];
+  if (Q == 1 || (Q == 0 && !A.q16)) return A.pr[p];
   const uint4 w = A.qp[p];
   const float4 o = A.qc[2 * (p >> kQChunkLog2)], s = A.qc[2 * (p >> kQChunkLog2) + 1];
   return make_float4(fx16(w.x & 0xffffu, s.x, o.x), fx16(w.x >> 16, s.y, o.y), fx16(w.y & 0xffffu, s.z, o.z),
@@ -120,8 +123,9 @@ __device__ inline void ld_chunk(const RenderArgs &A, uint32_t p, float4 &o, floa
   s = A.qc[2 * (p >> kQChunkLog2) + 1];
 }
 // ld_pr with particle p's chunk frame already loaded (FP16 payload)
+template <int Q = 0>
 __device__ inline float4 ld_pr_in(const RenderArgs &A, uint32_t p, const float4 &o, const float4 &s) {
-  if (!A.q16) return A.pr[p];
+  if (Q == 1 || (Q == 0 && !A.q16)) return A.pr[p];
   const uint4 w = A.qp[p];
   return make_float4(fx16(w.x & 0xffffu, s.x, o.x), fx16(w.x >> 16, s.y, o.y), fx16(w.y & 0xffffu, s.z, o.z),
                      fx16_r(w.y >> 16, s.w, o.w));

@@ -469,7 +469,7 @@ __global__ __launch_bounds__(256) void wf_fold(RenderArgs A, WFState S) {
 #endif
 constexpr uint32_t kGatherRefill = AARBF_GATHER_REFILL;
 
-template <bool STATS>
+template <bool STATS, int Q>
 __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderArgs A, WFState S, const uint32_t *__restrict__ listG,
                                                      const uint32_t *__restrict__ countG,
                                                      uint32_t *__restrict__ listL, uint32_t *__restrict__ countL,
@@ -562,12 +562,12 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
     if (A.ppl == 1) {
       float4 P[8], co, cs;
       uint32_t pid[8];
-      if (A.q16) ld_chunk(A, lb, co, cs); // the group's leaves share an FP16 chunk
+      if (Q == 2 || (Q == 0 && A.q16)) ld_chunk(A, lb, co, cs); // the group's leaves share an FP16 chunk
 #pragma unroll
       for (uint32_t r = 0; r < 8; ++r) {
         uint32_t nib = (od >> (4u * r)) & 15u;
         pid[r] = nib ? lb + nib - 1u : 0xffffffffu;
-        if (nib) P[r] = ld_pr_in(A, pid[r], co, cs);
+        if (nib) P[r] = ld_pr_in<Q>(A, pid[r], co, cs);
       }
 #pragma unroll
       for (uint32_t r = 0; r < 8; ++r)
@@ -582,7 +582,7 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
       uint32_t p0 = leaf * A.ppl, p1 = min(p0 + A.ppl, A.numParticles);
       for (uint32_t p = p0; p < p1; ++p) {
         if (STATS) c.particles++;
-        take(p, ld_pr(A, p));
+        take(p, ld_pr<Q>(A, p));
       }
     }
   };
@@ -698,7 +698,7 @@ __global__ __launch_bounds__(kGBlock, AARBF_GATHER_WAVES) void wf_gather(RenderA
         }
         const uint32_t pk = gi[k * kGBlock];
         float tin, tout;
-        chord(ld_pr(A, pk), tin, tout); // the exit parameter take() computed
+        chord(ld_pr<Q>(A, pk), tin, tout); // the exit parameter take() computed
         oeo[r * stride + row] = (uint8_t)k;
         oidx[k * stride + row] = pk;
         otin[k * stride + row] = tk;
@@ -1996,9 +1996,11 @@ static int launch_round(aarbf_context *ctx, const RenderArgs &A, const WFState &
       // from the queue
       const uint32_t gp = std::min<uint32_t>(gb, 8u * (uint32_t)std::max(ctx->cus, 1));
       if (st)
-        wf_gather<true><<<gp, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1, listL, cL, budget, cnt + 7);
+        wf_gather<true, 0><<<gp, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1, listL, cL, budget, cnt + 7);
+      else if (A.q16)
+        wf_gather<false, 2><<<gp, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1, listL, cL, budget, cnt + 7);
       else
-        wf_gather<false><<<gp, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1, listL, cL, budget, cnt + 7);
+        wf_gather<false, 1><<<gp, kGBlock, stack_bytes, ctx->stream>>>(A, S, lG, cA + 1, listL, cL, budget, cnt + 7);
       AARBF_CHECK_LAUNCH(ctx, "wf_gather");
       if (st)
         wf_gather_wave<true><<<wfix, 64, wstack, ctx->stream>>>(A, S, listL, cL, ~0u);
