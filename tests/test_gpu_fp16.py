@@ -134,3 +134,36 @@ def test_fp16_small_and_ragged_scenes(gpu_ctx, n):
     oimg, oacc, _ = r.run(spp=4, threads=8)
     assert np.isfinite(acc).all()
     assert np.array_equal(acc, oacc) and np.array_equal(img, oimg)
+
+
+def test_fp16_long_queues_with_and_without_statistics(gpu_ctx):
+    """A 160x128 view (20,480 rays, above the 16,384 from which the
+    one-lane gather with persistent lanes takes the queue) with the FP16
+    payload: the payload-specific kernels (statistics off) and the
+    statistics kernels (which read the payload kind at run time) render the
+    same image bit for bit, equal to the oracle on the decoded scene on a
+    pixel grid."""
+    sc, tfs = _setup(gpu_ctx, n=6000, seed=12)
+    W, H = 160, 128
+    cam = default_camera(W, H, look_from=(0.35, 0.25, 2.0))
+    try:
+        gpu_ctx.set_payload(gpu_ctx.PAYLOAD_FP16)
+        img, acc = _render(gpu_ctx, sc, cam, W, H, 0.004, spp=2, frame=9)
+        gpu_ctx.set_stats(True)
+        try:
+            img_s, acc_s = _render(gpu_ctx, sc, cam, W, H, 0.004, spp=2, frame=9)
+        finally:
+            gpu_ctx.set_stats(False)
+            gpu_ctx.read_stats(reset=True)
+            gpu_ctx.read_histogram(reset=True)
+        q = gpu_ctx.payload_read(sc.n)
+    finally:
+        gpu_ctx.set_payload(gpu_ctx.PAYLOAD_FP32)
+    assert (acc[..., :3] > 0).mean() > 0.1
+    assert np.array_equal(acc, acc_s) and np.array_equal(img, img_s)
+    px = np.array([(x, y) for y in range(1, H, 11) for x in range(2, W, 13)], np.int32)
+    r = oracle.Render(sc.particles.cpu().numpy(), sc.radii.cpu().numpy(), sc.aabb, *tfs, cam, W, H, 0.004, 0.09,
+                      frame_id=9, accum_id=1, light=(0.3, 0.2, 0.4), baked=q)
+    oimg, oacc, _ = r.run(pixels=px, spp=2, threads=8)
+    sel = (px[:, 1], px[:, 0])
+    assert np.array_equal(acc[sel], oacc[sel]) and np.array_equal(img[sel], oimg[sel])
