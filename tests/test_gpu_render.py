@@ -194,13 +194,16 @@ def test_taa_matches_oracle(gpu_ctx, gui_layer, disable):
                           f"{oframe[tuple(bad[0])]:08x}"
 
 
-def test_sparse_waves_leaving_early_is_exact(gpu_ctx):
+@pytest.mark.parametrize("graph_rounds", [0, 4])
+def test_sparse_waves_leaving_early_is_exact(gpu_ctx, graph_rounds):
     """Step-kernel waves that leave their loop once fewer than
     AARBF_TUNE_STEP_MIN_LIVE lanes still step (their rays go on in the next
     round) render bit for bit the same image as waves that run until their
     last lane leaves.  The view has 81,920 pixels, so the step queue passes
     the 65,536 rays from which the early exit applies; thresholds 0 (never),
-    16 (the default) and 48 (waves leave when 17 lanes are gone)."""
+    16 (the default) and 48 (waves leave when 17 lanes are gone); with
+    host-driven rounds and with rounds captured in a hipGraph (fixed grids,
+    queue lengths read on the device)."""
     W, H = 512, 160
     xyzw, radii = random_scene(20000, seed=91, extent=1.0, r=(0.02, 0.06))
     tfs = (scenes.ramp_table(256), scenes.constant_table(n=4), scenes.ramp_table(64))
@@ -209,11 +212,13 @@ def test_sparse_waves_leaving_early_is_exact(gpu_ctx):
     kw = dict(light=(0.3, 0.2, 0.4))
     out = {}
     try:
+        gpu_ctx.set_tuning(gpu_ctx.TUNE_GRAPH_ROUNDS, graph_rounds)
         for live in (0, 16, 48):
             gpu_ctx.set_tuning(gpu_ctx.TUNE_STEP_MIN_LIVE, live)
             out[live] = _gpu_render(gpu_ctx, sc, cam, W, H, 0.004, spp=2, frame=5, **kw)
     finally:
         gpu_ctx.set_tuning(gpu_ctx.TUNE_STEP_MIN_LIVE, 16)
+        gpu_ctx.set_tuning(gpu_ctx.TUNE_GRAPH_ROUNDS, 0)
     img0, acc0 = out[0]
     assert (acc0[..., :3] > 0).mean() > 0.1
     for live in (16, 48):
