@@ -354,7 +354,7 @@ __device__ inline void march(const DiffArgs &D, const DSeg &W, int px, int py, u
   for (uint32_t i = 0; i < A.max_depth; ++i) {
     if (ratio) {
       const float xi = aa_lcg_next(&rng);
-      t = t - aa_log_step(1.0f - xi) * A.unit;
+      t = aa_flight(t, 1.0f - xi, A.unit);
     } else {
       t = t + A.unit;
     }

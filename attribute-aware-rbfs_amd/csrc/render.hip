@@ -186,7 +186,7 @@ __device__ void track(const RenderArgs &A, bool marching, bool shadow, uint32_t 
     for (; i < A.max_depth; ++i) {
       if (STATS) c.steps++;
       float xi = aa_lcg_next(&rng);
-      t = t - aa_log_step(1.0f - xi) * A.unit;
+      t = aa_flight(t, 1.0f - xi, A.unit);
       if (t >= tmax) return;
       point_query<STATS>(A, o.x + t * d.x, o.y + t * d.y, o.z + t * d.z, pl, c);
       if (pl.count == 0) continue;

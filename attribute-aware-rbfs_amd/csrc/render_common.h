@@ -88,14 +88,23 @@ __device__ inline float fx16_r(uint32_t q, float step, float origin) { return q 
 __device__ inline float un16(uint32_t q) { return (float)q * (1.0f / 65535.0f); }
 
 // One free-flight draw of the delta tracker: advances the LCG state s and
-// returns log(1 - xi), bit for bit aa_lcg_next followed by
-// aa_log_step(1.0f - xi), with 1 - xi as one fused multiply-add:
+// returns the next t, bit for bit aa_lcg_next followed by
+// aa_flight(t, 1.0f - xi, unit), with 1 - xi as one fused multiply-add:
 // xi = (float)s * 2^-32 is exact, so fma((float)s, -2^-32, 1) rounds the
 // same value as 1 - xi.  tests/cpp/flight_identity.c checks this over all
 // 2^32 states.
-__device__ inline float flight_log(uint32_t &s) {
+__device__ inline float flight(uint32_t &s, float t, float unit) {
   s = s * 1664525u + 1013904223u;
-  return aa_log_step(__builtin_fmaf((float)s, -2.3283064365386963e-10f, 1.0f));
+  return aa_flight(t, __builtin_fmaf((float)s, -2.3283064365386963e-10f, 1.0f), unit);
+}
+// The same draw without the x = 1 - xi = 0 case (ln 0 = -inf, t = +inf):
+// zero tells the caller that the returned t is not aa_flight's (the
+// caller draws again with flight()).
+__device__ inline float flight_nz(uint32_t &s, float t, float unit, bool &zero) {
+  s = s * 1664525u + 1013904223u;
+  const float x = __builtin_fmaf((float)s, -2.3283064365386963e-10f, 1.0f);
+  zero = x == 0.0f;
+  return __builtin_fmaf(aa_log_step_nz(x), -unit, t);
 }
 
 // Baked particle p: position and radius, and (alpha^3, rgb^2.2).  Q: 0 =

@@ -1249,7 +1249,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
       // taken.  Marching steps a fixed distance and draws nothing.
       uint32_t ns = L.trng;
       const bool march = is_march<B>(A, L);
-      const float tn = march ? L.t + A.unit : L.t - flight_log(ns) * A.unit;
+      const float tn = march ? L.t + A.unit : flight(ns, L.t, A.unit);
       const bool can = !pending && L.i < A.max_depth && steps < step_cap;
       if (can && act == 0u && tn < lim) {
         L.trng = ns;
@@ -1266,8 +1266,9 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
         auto more = [&](auto marching) {
           auto draw = [&]() {
             uint32_t ns2 = L.trng;
-            const float tn2 = decltype(marching)::value ? L.t + A.unit : L.t - flight_log(ns2) * A.unit;
-            if (!(tn2 < lim)) return false;
+            bool zero = false;
+            const float tn2 = decltype(marching)::value ? L.t + A.unit : flight_nz(ns2, L.t, A.unit, zero);
+            if (!(tn2 < lim) || zero) return false;
             L.trng = ns2;
             L.t = tn2;
             ++taken;
@@ -1819,7 +1820,7 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
           L.t = L.t + A.unit;
         } else {
           float xi = aa_lcg_next(&L.trng);
-          L.t = L.t - aa_log_step(1.0f - xi) * A.unit;
+          L.t = aa_flight(L.t, 1.0f - xi, A.unit);
         }
         if (L.t >= L.tmax) end = true;
         else L.flags |= F_PENDING;

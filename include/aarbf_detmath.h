@@ -100,11 +100,15 @@ AA_FN float aa_logf(float x) {
   return AA_FMAF(ef, 0.693145751953125f, r);
 }
 
-/* aa_logf restricted to the arguments a tracker step can produce, 1 - xi
- * with xi an LCG draw in [0, 1]: 0 or a normal number in [2^-24, 1].  Same
- * operations as aa_logf on that domain, without its branches (the result
- * is identical bit for bit; checked exhaustively over the domain). */
-AA_FN float aa_log_step(float x) {
+/* ln(x) for the arguments a tracker step can produce, 1 - xi with xi an
+ * LCG draw in [0, 1]: a normal number in [2^-24, 1] (x = 0 is
+ * aa_log_step's).  The reduction and polynomial of aa_logf without its
+ * branches, and a shorter tail: e is in [-24, 0] here, so e * ln2 in one
+ * fused multiply-add with binary32 ln2 (error at most 24 * 1.9e-9, under
+ * 0.03 ulp of the result) replaces the two-part product, and f + f^2 q
+ * is one fused multiply-add.  Within 0.873 ulp of ln(x) on the whole
+ * domain (checked exhaustively against the double-precision logarithm). */
+AA_FN float aa_log_step_nz(float x) {
   uint32_t u = aa_f2u(x);
   /* The same m and e as aa_logf's "m > sqrt(2)" adjustment, from one
    * integer subtraction: u - bits(0x3f3504f4) has the exponent of
@@ -125,12 +129,19 @@ AA_FN float aa_log_step(float x) {
   q = AA_FMAF(q, f, 0.33333322405815125f);
   q = AA_FMAF(q, f, -0.4999999701976776f);
   float f2 = f * f;
-  /* x = 0: e = -inf carries -inf through both fused terms */
-  float ef = u == 0u ? -AA_INF_F : (float)e;
-  float lo = AA_FMAF(ef, 1.4286068203094172e-06f, f2 * q);
-  float r = lo + f;
-  return AA_FMAF(ef, 0.693145751953125f, r);
+  float r = AA_FMAF(f2, q, f);
+  return AA_FMAF((float)e, 0.693147182464599609375f, r);
 }
+
+/* The tracker-step logarithm on its whole domain: 0 (ln 0 = -inf) and
+ * aa_log_step_nz's [2^-24, 1]. */
+AA_FN float aa_log_step(float x) { return x == 0.0f ? -AA_INF_F : aa_log_step_nz(x); }
+
+/* A delta-tracking free flight, t - ln(x) * unit (x = 1 - xi) with the
+ * product and the difference rounded once (one fused multiply-add, as a
+ * shader compiler may contract "t - (log(1 - xi) / majorant) * unit" with
+ * majorant 1, deviceCodeRBF.hlsl:153).  x = 0 gives +inf. */
+AA_FN float aa_flight(float t, float x, float unit) { return AA_FMAF(aa_log_step(x), -unit, t); }
 
 /* e^x.  k = rint(x / ln2), r = x - k*ln2 (two-part, fused), e^r = 1 + r +
  * r^2 * P(r) with P a degree-5 fit on |r| <= ln2/2, then scaled by 2^k

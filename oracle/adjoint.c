@@ -196,7 +196,7 @@ static void march(const orc_diff *p, const dpart *pp, int px, int py, uint32_t s
   for (uint32_t i = 0; i < maxd; ++i) {
     if (p->estimator == 1) {
       float xi = aa_lcg_next(&rng);
-      t = t - aa_log_step(1.0f - xi) * p->unit;
+      t = aa_flight(t, 1.0f - xi, p->unit);
     } else {
       t = t + p->unit;
     }

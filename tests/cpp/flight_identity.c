@@ -1,11 +1,11 @@
-/* The step kernel's free-flight draw (flight_log in
+/* The step kernel's free-flight draw (flight and flight_nz in
  * attribute-aware-rbfs_amd/csrc/render_common.h) computes 1 - xi as
  * fma((float)s, -2^-32, 1) instead of 1.0f - (float)s * 2^-32, the
  * reference's xi = lcg() followed by 1 - xi (rng.h, deviceCodeRBF.hlsl).
  * The product (float)s * 2^-32 is exact (a power-of-two scaling of a value
  * >= 1 or of 0), so both round the same real number once; this program
  * checks the two against each other for every 32-bit LCG state (equal bits
- * in, equal aa_log_step out).
+ * in, equal aa_flight out).
  *
  *   gcc -O2 -ffp-contract=off -I include tests/cpp/flight_identity.c -lm
  *

@@ -199,11 +199,15 @@ def test_expf_mid_equals_expf_on_its_domain():
     assert L.orc_expf_mid_mismatches() == 0
 
 
-def test_log_step_equals_logf_on_its_domain():
-    # exhaustive: 0 and all 2e8 binary32 values in [2^-24, 1]
+def test_log_step_within_one_ulp_on_its_domain():
+    # exhaustive: all 2e8 binary32 values in [2^-24, 1] against the double
+    # logarithm; ln 1 = 0 exactly, ln 0 = -inf
     L = oracle.lib()
-    L.orc_log_step_mismatches.restype = C.c_uint64
-    assert L.orc_log_step_mismatches() == 0
+    L.orc_log_step_max_ulp.restype = C.c_double
+    at = C.c_float(0.0)
+    err = L.orc_log_step_max_ulp(C.byref(at))
+    assert 0.0 <= err <= 0.9, (err, at.value)
+    assert _math(5, np.array([0.0, 1.0], np.float32)).tolist() == [-np.inf, 0.0]
 
 
 def test_expf_within_one_ulp():
@@ -435,7 +439,7 @@ def test_detmath_coefficients_are_the_committed_fits():
         return np.array([float(start.group(1))] + [float(v) for v in rest], np.float32)
 
     log_q, exp_p = fit.log_coefficients()[::-1], fit.exp_coefficients()[::-1]
-    for fn in ("aa_logf", "aa_log_step"):
+    for fn in ("aa_logf", "aa_log_step_nz"):
         assert np.array_equal(horner(fn), log_q), fn
     for fn in ("aa_expf", "aa_expf_mid"):
         assert np.array_equal(horner(fn), exp_p), fn
