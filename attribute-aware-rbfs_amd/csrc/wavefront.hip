@@ -80,10 +80,11 @@ constexpr uint32_t kWaveSlice = AARBF_WAVE_SLICE; // nodes a lane opens between 
 // iterates as long as its slowest lane, and most steps are empty-space
 // steps, so taking several per iteration cuts the iterations (and the
 // per-iteration overhead every lane pays) roughly by this factor.  Each
-// such step is 29 VALU instructions (LCG, 1 - xi, logarithm, t update and
-// compare); 16 measured best against 8, 10, 12, 14, 20 and 24.
+// such step is 25 VALU instructions (LCG, 1 - xi, logarithm, t update and
+// compares); 24 and 28 measured best against 16, 20 and 32 (1.7 % faster
+// than 16 at 1080p).
 #ifndef AARBF_FAST_STEPS
-#define AARBF_FAST_STEPS 16
+#define AARBF_FAST_STEPS 24
 #endif
 constexpr int kFastSteps = AARBF_FAST_STEPS;
 // Step queues at least this long let sparse waves leave early (RenderArgs
