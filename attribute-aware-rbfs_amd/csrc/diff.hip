@@ -1452,6 +1452,7 @@ static int diff_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const aar
   A.W = W;
   A.H = H;
   A.unit = pc->unit;
+  A.inv_unit = 1.0f / pc->unit;
   A.pos = make_float3(pc->camera.pos.x, pc->camera.pos.y, pc->camera.pos.z);
   A.d00 = make_float3(pc->camera.dir_00.x, pc->camera.dir_00.y, pc->camera.dir_00.z);
   A.du = make_float3(pc->camera.dir_du.x, pc->camera.dir_du.y, pc->camera.dir_du.z);
@@ -1467,6 +1468,7 @@ static int diff_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const aar
   A.cm_n = cm->n;
   A.dmap = dm->table;
   A.dm_n = dm->n;
+  A.dm_nf = (float)dm->n;
   A.bvh = make_view(*bvh);
   if (bvh->num_levels > kStackLevels) A.bvh.groups = nullptr; // the march then queries every point
   D.particles = (const float4 *)pc->particles;

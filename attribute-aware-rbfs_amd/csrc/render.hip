@@ -181,12 +181,18 @@ __device__ void track(const RenderArgs &A, bool marching, bool shadow, uint32_t 
         }
       }
     }
-  } else { // stochasticTracking ':141-203', majorant 1
+  } else { // stochasticTracking ':141-203', majorant 1; free flights in product form (aa_flight_pos)
     t = 0.0f;
+    float tb = 0.0f, P = 1.0f;
     for (; i < A.max_depth; ++i) {
       if (STATS) c.steps++;
       float xi = aa_lcg_next(&rng);
-      t = aa_flight(t, 1.0f - xi, A.unit);
+      P = P * (1.0f - xi);
+      t = aa_flight_pos(tb, P, A.unit);
+      if ((i + 1u) % AA_FLIGHT_BLOCK == 0u) {
+        tb = t;
+        P = 1.0f;
+      }
       if (t >= tmax) return;
       point_query<STATS>(A, o.x + t * d.x, o.y + t * d.y, o.z + t * d.z, pl, c);
       if (pl.count == 0) continue;
@@ -432,6 +438,7 @@ static int render_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const a
   A.H = H;
   A.amb = pc->light.ambient;
   A.unit = pc->unit;
+  A.inv_unit = 1.0f / pc->unit;
   A.light = light_direction(pc->light.azimuth, pc->light.elevation);
   A.pos = make_float3(pc->camera.pos.x, pc->camera.pos.y, pc->camera.pos.z);
   A.d00 = make_float3(pc->camera.dir_00.x, pc->camera.dir_00.y, pc->camera.dir_00.z);
@@ -452,6 +459,7 @@ static int render_setup(aarbf_context *ctx, const aarbf_raygen_data *rg, const a
   A.cm_n = cm->n;
   A.dmap = dm->table;
   A.dm_n = dm->n;
+  A.dm_nf = (float)dm->n;
   A.stbn = stbn ? stbn->rgba : nullptr;
   A.stbn_w = stbn ? stbn->width : 0;
   A.stbn_h = stbn ? stbn->height : 0;

@@ -35,6 +35,7 @@ struct RenderArgs {
   uint32_t max_depth;
   int W, H;
   float amb, unit;
+  float inv_unit; // 1 / unit, correctly rounded (aa_flight_bound)
   float3 light, pos, d00, du, dv, lb, rt;
   const float4 *pr; // baked x, y, z, radius (0 = never covers)
   const float4 *pc; // baked alpha^3, rgb^2.2
@@ -50,6 +51,7 @@ struct RenderArgs {
   const float4 *cmap;
   const float4 *dmap;
   uint32_t cm_n, dm_n;
+  float dm_nf; // (float)dm_n
   const uint8_t *stbn;
   int stbn_w, stbn_h;
   float4 *image, *accum;
@@ -88,23 +90,13 @@ __device__ inline float fx16_r(uint32_t q, float step, float origin) { return q 
 __device__ inline float un16(uint32_t q) { return (float)q * (1.0f / 65535.0f); }
 
 // One free-flight draw of the delta tracker: advances the LCG state s and
-// returns the next t, bit for bit aa_lcg_next followed by
-// aa_flight(t, 1.0f - xi, unit), with 1 - xi as one fused multiply-add:
-// xi = (float)s * 2^-32 is exact, so fma((float)s, -2^-32, 1) rounds the
-// same value as 1 - xi.  tests/cpp/flight_identity.c checks this over all
-// 2^32 states.
-__device__ inline float flight(uint32_t &s, float t, float unit) {
+// returns 1 - xi, bit for bit 1.0f - aa_lcg_next(&s), as one fused
+// multiply-add: xi = (float)s * 2^-32 is exact, so fma((float)s, -2^-32, 1)
+// rounds the same value as 1 - xi.  tests/cpp/flight_identity.c checks
+// this over all 2^32 states.
+__device__ inline float flight_x(uint32_t &s) {
   s = s * 1664525u + 1013904223u;
-  return aa_flight(t, __builtin_fmaf((float)s, -2.3283064365386963e-10f, 1.0f), unit);
-}
-// The same draw without the x = 1 - xi = 0 case (ln 0 = -inf, t = +inf):
-// zero tells the caller that the returned t is not aa_flight's (the
-// caller draws again with flight()).
-__device__ inline float flight_nz(uint32_t &s, float t, float unit, bool &zero) {
-  s = s * 1664525u + 1013904223u;
-  const float x = __builtin_fmaf((float)s, -2.3283064365386963e-10f, 1.0f);
-  zero = x == 0.0f;
-  return __builtin_fmaf(aa_log_step_nz(x), -unit, t);
+  return __builtin_fmaf((float)s, -2.3283064365386963e-10f, 1.0f);
 }
 
 // Baked particle p: position and radius, and (alpha^3, rgb^2.2).  Q: 0 =

@@ -87,6 +87,8 @@ constexpr uint32_t kWaveSlice = AARBF_WAVE_SLICE; // nodes a lane opens between 
 #define AARBF_FAST_STEPS 24
 #endif
 constexpr int kFastSteps = AARBF_FAST_STEPS;
+// delta tracking's flight anchors: every kFlightBlock steps of a ray
+constexpr uint32_t kFlightBlock = AA_FLIGHT_BLOCK;
 // Step queues at least this long let sparse waves leave early (RenderArgs
 // min_live); shorter ones (the end of a pass) would only add rounds.
 constexpr uint32_t kMinLiveQueue = 65536;
@@ -108,6 +110,7 @@ struct WFState {
   uint4 *u;     // trng, i, s | flags << 16, seg_n
   float4 *f;    // LHS, RHS, shw, r2
   float4 *seg;  // seg_t0, seg_end, rx, unused
+  float2 *fl;   // delta tracking: flight anchor t_base, product P (aa_flight_pos)
   uint32_t *cidx; // [kCand][nslots] candidate particle index (ascending)
   float *ctin;    // [kCand][nslots]
   float *ctout;   // [kCand][nslots]
@@ -132,6 +135,7 @@ struct Lane {
   uint32_t trng, i, s, flags, seg_n;
   float LHS, RHS, shw, r2;
   float seg_t0, seg_end, rx;
+  float tb, P; // delta tracking: t = aa_flight_pos(tb, P, unit)
 };
 
 __device__ inline void load_lane(const WFState &S, uint32_t slot, Lane &L) {
@@ -157,6 +161,9 @@ __device__ inline void load_lane(const WFState &S, uint32_t slot, Lane &L) {
   L.seg_t0 = g.x;
   L.seg_end = g.y;
   L.rx = g.z;
+  const float2 h = S.fl[slot];
+  L.tb = h.x;
+  L.P = h.y;
 }
 
 __device__ inline void save_lane(const WFState &S, uint32_t slot, const Lane &L) {
@@ -167,6 +174,7 @@ __device__ inline void save_lane(const WFState &S, uint32_t slot, const Lane &L)
   S.u[slot] = make_uint4(L.trng, L.i, (L.s & 0xffffu) | (L.flags << 16), L.seg_n);
   S.f[slot] = make_float4(L.LHS, L.RHS, L.shw, L.r2);
   S.seg[slot] = make_float4(L.seg_t0, L.seg_end, L.rx, 0.0f);
+  S.fl[slot] = make_float2(L.tb, L.P);
 }
 
 // The view a pixel of the combined pixel space belongs to: with B (a
@@ -238,6 +246,8 @@ __device__ inline bool start_sample(const RenderArgs &A, const ReqArgs &R, int p
   L.d = dir;
   L.tmax = texit;
   L.t = marching ? tenter + A.unit * r1 : 0.0f;
+  L.tb = L.t; // the ray's first flight anchor
+  L.P = 1.0f;
   L.seg_t0 = marching ? tenter : 0.0f; // where any sample's primary ray can start
   L.LHS = 0.0f;
   L.RHS = marching ? -aa_logf(1.0f - r0 * 0.99f) : 0.0f;
@@ -1146,7 +1156,7 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
     if (!dm_lds) return tf_r(A.dmap, A.dm_n, u);
     uint32_t j0, j1;
     float f;
-    aa_tf_coords(u, A.dm_n, &j0, &j1, &f);
+    aa_tf_coords_nf(u, A.dm_n, A.dm_nf, &j0, &j1, &f);
     return aa_lerp_tex(s_dm[j0], s_dm[j1], f);
   };
   Counters c = {};
@@ -1189,7 +1199,11 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
     };
     // A draw that lands before lim with no active particle is a step in
     // empty space: the query at it covers nothing, so only t and i move.
-    float lim0 = 0.0f, lim = 0.0f;
+    // With delta tracking a draw lands before lim while the product of
+    // its 1 - xi since the anchor stays above T = aa_flight_bound(tb, lim).
+    // lim = min(tmax, segment end, next entry)
+    float lim = 0.0f, T = 0.0f;
+    const float inv_unit = A.inv_unit;
     auto leave = [&](float t) {
       t_exit = AA_INF_F;
       uint32_t m = act;
@@ -1220,9 +1234,9 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
       act = 0;
       eptr = 0;
       t_exit = AA_INF_F;
-      lim0 = fminf(L.tmax, L.seg_end);
       enter(L.t);
-      lim = fminf(lim0, t_enter);
+      lim = fminf(fminf(L.tmax, L.seg_end), t_enter);
+      T = aa_flight_bound(L.tb, lim, inv_unit);
     };
     setup();
 
@@ -1247,30 +1261,29 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
       const bool pending = (L.flags & F_PENDING) != 0;
       // The next free-flight draw, computed by every lane (nearly all of
       // them step in each iteration) and committed only where a step is
-      // taken.  Marching steps a fixed distance and draws nothing.
+      // taken.  Marching steps a fixed distance and draws nothing; delta
+      // tracking multiplies the draw's 1 - xi into the product P (its
+      // position is needed only where the step is not in empty space).
       uint32_t ns = L.trng;
       const bool march = is_march<B>(A, L);
-      const float tn = march ? L.t + A.unit : flight(ns, L.t, A.unit);
+      const float tm = L.t + A.unit;
+      const float pn = march ? 1.0f : L.P * flight_x(ns);
       const bool can = !pending && L.i < A.max_depth && steps < step_cap;
-      if (can && act == 0u && tn < lim) {
+      if (can && act == 0u && (march ? tm < lim : pn > T)) {
         L.trng = ns;
-        L.t = tn;
+        uint32_t taken = 1;
         // more draws in the same iteration, each committed only when it
         // too lands in empty space before the next event (the first that
         // does not is drawn again by the next iteration), within the depth
         // and step budgets left after this one
         const uint32_t room = min(A.max_depth - (L.i + 1u), step_cap - (steps + 1u));
-        uint32_t taken = 1;
-        // one unrolled loop per tracker (the choice is uniform in a wave
-        // of one view); the budgets are checked per draw only where fewer
-        // than kFastSteps - 1 steps are left (a rolled loop, rarely taken)
-        auto more = [&](auto marching) {
+        if (march) {
+          L.t = tm;
+          // the budgets are checked per step only where fewer than
+          // kFastSteps - 1 steps are left (a rolled loop, rarely taken)
           auto draw = [&]() {
-            uint32_t ns2 = L.trng;
-            bool zero = false;
-            const float tn2 = decltype(marching)::value ? L.t + A.unit : flight_nz(ns2, L.t, A.unit, zero);
-            if (!(tn2 < lim) || zero) return false;
-            L.trng = ns2;
+            const float tn2 = L.t + A.unit;
+            if (!(tn2 < lim)) return false;
             L.t = tn2;
             ++taken;
             return true;
@@ -1284,12 +1297,38 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
             for (uint32_t u = 1; u <= room; ++u)
               if (!draw()) break;
           }
-        };
-        if (march) more(std::true_type{});
-        else more(std::false_type{});
+        } else {
+          L.P = pn;
+          // delta tracking: up to the next flight anchor (L.i a multiple
+          // of kFlightBlock), where the position is formed once
+          const uint32_t left = min(room, kFlightBlock - 1u - L.i % kFlightBlock);
+          // one region per draw, not nested (a lane that stops is masked
+          // off the later ones, a wave skips a region no lane enters), so
+          // no join copies and no saved masks per draw
+          bool go = true;
+#pragma unroll
+          for (uint32_t u = 1; u < kFlightBlock; ++u) {
+            if (go) {
+              uint32_t ns2 = L.trng;
+              const float p2 = L.P * flight_x(ns2);
+              go = (p2 > T) & (u <= left);
+              if (go) {
+                L.trng = ns2;
+                L.P = p2;
+                ++taken;
+              }
+            }
+          }
+        }
         L.i += taken;
         steps += taken;
         if (STATS) c.steps += taken;
+        if (!march && L.i % kFlightBlock == 0u) { // the anchor
+          L.t = aa_flight_pos(L.tb, L.P, A.unit);
+          L.tb = L.t;
+          L.P = 1.0f;
+          T = aa_flight_bound(L.tb, lim, inv_unit);
+        }
         continue;
       }
       bool end = false;
@@ -1303,7 +1342,12 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
           ++steps;
           if (STATS) c.steps++;
           L.trng = ns;
-          L.t = tn;
+          if (march) {
+            L.t = tm;
+          } else {
+            L.P = pn;
+            L.t = aa_flight_pos(L.tb, pn, A.unit);
+          }
           if (L.t >= L.tmax) end = true;
           else L.flags |= F_PENDING;
         }
@@ -1316,9 +1360,11 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
         }
         Payload pl;
         float x = L.o.x + L.t * L.d.x, y = L.o.y + L.t * L.d.y, z = L.o.z + L.t * L.d.z;
+        bool moved = false; // lim or the flight anchor: the bound is due
         if (L.t >= t_enter) {
           enter(L.t);
-          lim = fminf(lim0, t_enter);
+          lim = fminf(fminf(L.tmax, L.seg_end), t_enter);
+          moved = true;
         }
         if (L.t > t_exit) leave(L.t);
         pl.count = 0;
@@ -1389,6 +1435,14 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
         }
         if (!end) {
           ++L.i;
+          if (!march) {
+            if (L.i % kFlightBlock == 0u) { // the anchor, at the query's position
+              L.tb = L.t;
+              L.P = 1.0f;
+              moved = true;
+            }
+            if (moved) T = aa_flight_bound(L.tb, lim, inv_unit);
+          }
           continue;
         }
       }
@@ -1407,6 +1461,8 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
         L.shw = 0.0f;
         L.LHS = 0.0f;
         L.t = is_march<B>(A, L) ? -A.unit * L.r2 : 0.0f;
+        L.tb = L.t; // the shadow ray's first flight anchor
+        L.P = 1.0f;
         L.seg_end = -AA_INF_F;
         outcome = 2;
         break;
@@ -1422,6 +1478,8 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
       }
       break;
     }
+    // empty-space steps leave the position to the anchor and the product
+    if (outcome && !is_march<B>(A, L)) L.t = aa_flight_pos(L.tb, L.P, A.unit);
     if (outcome) save_lane(S, slot, L);
   }
   // compaction: lanes with an outcome append to the next lists
@@ -1486,7 +1544,7 @@ __global__ __launch_bounds__(kSBlock, 4) void wf_step_m(RenderArgs A, WFState S,
     if (!dm_lds) return tf_r(A.dmap, A.dm_n, u);
     uint32_t j0, j1;
     float f;
-    aa_tf_coords(u, A.dm_n, &j0, &j1, &f);
+    aa_tf_coords_nf(u, A.dm_n, A.dm_nf, &j0, &j1, &f);
     return aa_lerp_tex(s_dm[j0], s_dm[j1], f);
   };
   // max of the density map over [0, u]
@@ -1716,7 +1774,7 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
     if (!dm_lds) return tf_r(A.dmap, A.dm_n, u);
     uint32_t j0, j1;
     float f;
-    aa_tf_coords(u, A.dm_n, &j0, &j1, &f);
+    aa_tf_coords_nf(u, A.dm_n, A.dm_nf, &j0, &j1, &f);
     return aa_lerp_tex(s_dm[j0], s_dm[j1], f);
   };
   const uint32_t lane = threadIdx.x & 63u;
@@ -1821,7 +1879,8 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
           L.t = L.t + A.unit;
         } else {
           float xi = aa_lcg_next(&L.trng);
-          L.t = aa_flight(L.t, 1.0f - xi, A.unit);
+          L.P = L.P * (1.0f - xi);
+          L.t = aa_flight_pos(L.tb, L.P, A.unit);
         }
         if (L.t >= L.tmax) end = true;
         else L.flags |= F_PENDING;
@@ -1869,6 +1928,10 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
       }
       if (!end) {
         ++L.i;
+        if (!is_march<B>(A, L) && L.i % kFlightBlock == 0u) { // the flight anchor
+          L.tb = L.t;
+          L.P = 1.0f;
+        }
         continue;
       }
     }
@@ -1886,6 +1949,8 @@ __global__ __launch_bounds__(kSBlock) void wf_dense_wave(RenderArgs A, WFState S
       L.shw = 0.0f;
       L.LHS = 0.0f;
       L.t = is_march<B>(A, L) ? -A.unit * L.r2 : 0.0f;
+      L.tb = L.t; // the shadow ray's first flight anchor
+      L.P = 1.0f;
       L.seg_end = -AA_INF_F;
       outcome = 2;
       break;
@@ -2222,7 +2287,7 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A0, const ReqArgs *vi
     return o;
   };
   size_t o_od0 = take(16 * nslots), o_od1 = take(16 * nslots), o_pix = take(4 * nslots), o_alb = take(16 * nslots),
-         o_u = take(16 * nslots), o_f = take(16 * nslots), o_seg = take(16 * nslots),
+         o_u = take(16 * nslots), o_f = take(16 * nslots), o_seg = take(16 * nslots), o_fl = take(8 * nslots),
          o_cidx = take(4 * kCand * nslots), o_ctin = take(4 * kCand * nslots), o_ctout = take(4 * kCand * nslots),
          o_ceo = take(kCand * nslots), o_hidx = take(4 * kCand * npix), o_htin = take(4 * kCand * npix),
          o_htout = take(4 * kCand * npix), o_heo = take(kCand * npix), o_head = take(16 * npix),
@@ -2239,6 +2304,7 @@ int render_wavefront(aarbf_context *ctx, const RenderArgs &A0, const ReqArgs *vi
   S.u = (uint4 *)(base + o_u);
   S.f = (float4 *)(base + o_f);
   S.seg = (float4 *)(base + o_seg);
+  S.fl = (float2 *)(base + o_fl);
   S.cidx = (uint32_t *)(base + o_cidx);
   S.ctin = (float *)(base + o_ctin);
   S.ctout = (float *)(base + o_ctout);

@@ -38,6 +38,7 @@
 #define AA_RINTF(a) __builtin_rintf(a)
 #define AA_FMINF(a, b) __builtin_fminf((a), (b))
 #define AA_FMAXF(a, b) __builtin_fmaxf((a), (b))
+#define AA_FABSF(a) __builtin_fabsf(a)
 #else
 #include <math.h>
 #define AA_FN static inline
@@ -47,6 +48,7 @@
 #define AA_RINTF(a) rintf(a)
 #define AA_FMINF(a, b) fminf((a), (b))
 #define AA_FMAXF(a, b) fmaxf((a), (b))
+#define AA_FABSF(a) fabsf(a)
 #endif
 
 AA_FN uint32_t aa_f2u(float f) {
@@ -184,6 +186,51 @@ AA_FN float aa_powf(float x, float y) {
   return aa_expf(y * aa_logf(x));
 }
 
+/* ln(p) for p in [0, 1], the products of tracker steps below: 0 gives
+ * -inf, a subnormal p is scaled by 2^25 first; the reduction, polynomial
+ * and tail of aa_log_step_nz, whose binary32 ln2 in e * ln2 errs by at
+ * most 126 * 1.9e-9, under 0.04 ulp of the result for e <= -1. */
+AA_FN float aa_log_unit(float p) {
+  uint32_t u = aa_f2u(p);
+  int eo = 0;
+  if (u < 0x00800000u) { /* 0 or subnormal */
+    u = aa_f2u(p * 33554432.0f);
+    eo = -25;
+  }
+  const uint32_t tmp = u - 0x3f3504f4u;
+  const int e = (int)((int32_t)tmp >> 23) + eo;
+  const uint32_t mb = u - (tmp & 0xff800000u);
+  const float f = aa_u2f(mb) - 1.0f;
+  float q = -0.07866667956113815f;
+  q = AA_FMAF(q, f, 0.12821748852729797f);
+  q = AA_FMAF(q, f, -0.131016343832016f);
+  q = AA_FMAF(q, f, 0.1418900042772293f);
+  q = AA_FMAF(q, f, -0.1662759929895401f);
+  q = AA_FMAF(q, f, 0.2000196874141693f);
+  q = AA_FMAF(q, f, -0.2500077188014984f);
+  q = AA_FMAF(q, f, 0.33333322405815125f);
+  q = AA_FMAF(q, f, -0.4999999701976776f);
+  const float f2 = f * f;
+  const float r = AA_FMAF(f2, q, f);
+  const float l = AA_FMAF((float)e, 0.693147182464599609375f, r);
+  return p == 0.0f ? -AA_INF_F : l;
+}
+
+/* Delta-tracking free flights (deviceCodeRBF.hlsl:153, majorant 1): the
+ * position after a step is t = t_base - ln(P) * unit, P the product of
+ * the steps' 1 - xi since the ray's last anchor: the ray's start, and
+ * every step after which its step count i is a multiple of
+ * AA_FLIGHT_BLOCK (i counts the steps of the primary and its shadow ray
+ * together).  At an anchor t_base = t and P = 1.  ln(P) is the sum of the
+ * steps' ln(1 - xi), so the positions are those of the reference's
+ * statement, t -= ln(1 - xi) * unit per step, rounded differently: one
+ * product per step and one fused multiply-add per position, which lets a
+ * step in empty space be a multiplication and a compare (below). */
+#define AA_FLIGHT_BLOCK 32u
+
+/* The position after a step: fma(ln P, -unit, t_base), one rounding. */
+AA_FN float aa_flight_pos(float tb, float p, float unit) { return AA_FMAF(aa_log_unit(p), -unit, tb); }
+
 /* Cube used for alpha^3 (deviceCodeRBF.hlsl:375-376). */
 AA_FN float aa_cube(float a) { return (a * a) * a; }
 
@@ -231,6 +278,25 @@ AA_FN float aa_expf_mid(float x) {
   float y = AA_FMAF(r * r, p, r) + 1.0f;
   return y * aa_u2f((uint32_t)((int)k + 127) << 23);
 }
+
+/* A product bound for empty-space steps: any binary32 p > the result
+ * places aa_flight_pos(tb, p, unit) below lim (inv_unit = 1 / unit,
+ * correctly rounded).  With c = fl(fl(tb - lim) * inv_unit), the position
+ * is below lim - |lim| 2^-22 (so it rounds below lim) once
+ * ln p > (tb - lim + |lim| 2^-22) / unit + 4 ulp of the logarithm; the
+ * three roundings in c, the ulp of lim and the 4 ulp together stay under
+ * (|c| + |lim| inv_unit) 2^-19, which is added, and exp of the sum is
+ * rounded up by (1 + 2^-20) to cover aa_expf's ulp.  The sum is clamped
+ * to [-87, 0] (a larger bound stays a bound; above 0 no p <= 1 passes).
+ * lim = +inf gives 0 (every step is in empty space), a NaN lim NaN (no
+ * step is). */
+AA_FN float aa_flight_bound(float tb, float lim, float inv_unit) {
+  const float c = (tb - lim) * inv_unit;
+  const float m = (AA_FABSF(c) + AA_FABSF(lim) * inv_unit) * 1.9073486328125e-06f; /* 2^-19 */
+  const float b = aa_expf_mid(AA_FMINF(AA_FMAXF(c + m, -87.0f), 0.0f)) * 1.0000009536743164f; /* 1 + 2^-20 */
+  return lim == AA_INF_F ? 0.0f : (lim == lim ? b : AA_NAN_F);
+}
+
 
 /* over (sharedCode.h:142-149): a composited over b. */
 AA_FN void aa_over(const float a[4], const float b[4], float out[4]) {
@@ -345,10 +411,11 @@ AA_FN uint32_t aa_rng_seed(uint32_t frame_id, uint32_t px, uint32_t py, uint32_t
  * (texel i centred at (i + 0.5) / n).  Returns component c of the sample;
  * the interpolation is a + f * (b - a), so constant tables are returned
  * exactly.  NaN coordinates read texel 0. */
-AA_FN void aa_tf_coords(float u, uint32_t n, uint32_t *j0, uint32_t *j1, float *f) {
-  float x = u * (float)n - 0.5f;
+/* aa_tf_coords with nf = (float)n supplied (a kernel keeps it uniform). */
+AA_FN void aa_tf_coords_nf(float u, uint32_t n, float nf, uint32_t *j0, uint32_t *j1, float *f) {
+  float x = u * nf - 0.5f;
   x = AA_FMAXF(x, -1.0f); /* also maps NaN to -1 */
-  x = AA_FMINF(x, (float)n);
+  x = AA_FMINF(x, nf);
   float fl = AA_FLOORF(x);
   int i0 = (int)fl;
   *f = x - fl;
@@ -356,6 +423,10 @@ AA_FN void aa_tf_coords(float u, uint32_t n, uint32_t *j0, uint32_t *j1, float *
   int b = i0 + 1 < 0 ? 0 : (i0 + 1 > (int)n - 1 ? (int)n - 1 : i0 + 1);
   *j0 = (uint32_t)a;
   *j1 = (uint32_t)b;
+}
+
+AA_FN void aa_tf_coords(float u, uint32_t n, uint32_t *j0, uint32_t *j1, float *f) {
+  aa_tf_coords_nf(u, n, (float)n, j0, j1, f);
 }
 
 AA_FN float aa_lerp_tex(float a, float b, float f) { return a + f * (b - a); }

@@ -604,15 +604,25 @@ static void track_marching(const orc_scene *s, orc_tracker *tr, const float o[3]
 }
 
 /* stochasticTracking (deviceCodeRBF.hlsl:141-203); majorant 1, so the
- * divisions by it are exact and omitted.  t restarts at TMin (':150'). */
+ * divisions by it are exact and omitted.  t restarts at TMin (':150').
+ * The free flight of ':153' in the product form of aarbf_detmath.h
+ * (aa_flight_pos): P collects the steps' 1 - xi since the anchor tb, the
+ * ray's start and every step after which i is a multiple of
+ * AA_FLIGHT_BLOCK. */
 static void track_delta(const orc_scene *s, orc_tracker *tr, const float o[3], const float dir[3],
                         float tmax) {
   const orc_params *p = s->p;
   tr->t = 0.0f;
+  float tb = 0.0f, P = 1.0f;
   for (; tr->i < tr->max_depth; ++tr->i) {
     tr->steps++;
     float xi = aa_lcg_next(&tr->rng);
-    tr->t = aa_flight(tr->t, 1.0f - xi, p->unit);
+    P = P * (1.0f - xi);
+    tr->t = aa_flight_pos(tb, P, p->unit);
+    if ((tr->i + 1u) % AA_FLIGHT_BLOCK == 0u) {
+      tb = tr->t;
+      P = 1.0f;
+    }
     if (tr->t >= tmax) return;
     float x[3];
     for (int d = 0; d < 3; ++d) x[d] = o[d] + tr->t * dir[d];
@@ -1580,6 +1590,64 @@ ORC_EXPORT double orc_log_step_max_ulp(float *at) {
     }
   }
   return mx;
+}
+
+/* Largest error of aa_log_unit over every stride-th binary32 in (0, 1)
+ * (subnormals included), in ulps of the binary32 result, against the
+ * double-precision logarithm; x where it occurs in *at.  ln 0 = -inf and
+ * ln 1 = 0 must be exact (-1 if not). */
+ORC_EXPORT double orc_log_unit_max_ulp(uint32_t stride, float *at) {
+  double mx = 0.0;
+  if (aa_log_unit(1.0f) != 0.0f || aa_log_unit(0.0f) != -AA_INF_F) return -1.0;
+  for (uint32_t u = 1u; u < 0x3f800000u; u += stride) {
+    const float x = aa_u2f(u);
+    const double ref = log((double)x);
+    const float r32 = fabsf((float)ref);
+    const double ulp = (double)(nextafterf(r32, AA_INF_F) - r32);
+    const double err = fabs((double)aa_log_unit(x) - ref) / ulp;
+    if (err > mx) {
+      mx = err;
+      *at = x;
+    }
+  }
+  return mx;
+}
+
+/* Soundness of aa_flight_bound: in n seeded trials (anchor tb, limit lim,
+ * unit), every product p above the bound, the 64 binary32 values above it
+ * and 64 random ones in (bound, 1], must place aa_flight_pos(tb, p, unit)
+ * below lim.  Returns the number of such p that do not.  naive = 1 uses
+ * exp((tb - lim) / unit) without the margins instead (a check that the
+ * trials reach the boundary). */
+ORC_EXPORT uint64_t orc_flight_bound_violations(uint64_t n, uint32_t seed, int naive) {
+  uint64_t bad = 0;
+  uint32_t s = seed;
+  for (uint64_t k = 0; k < n; ++k) {
+    const float unit = (k & 1) ? 0.0004f : 1e-5f + 1e-2f * aa_lcg_next(&s);
+    const float tb = (k & 2) ? 5.0f * aa_lcg_next(&s) : 1e-3f * aa_lcg_next(&s);
+    float lim;
+    switch (k % 5) {
+    case 0: lim = tb + unit * 40.0f * aa_lcg_next(&s); break;  /* within 40 mean steps */
+    case 1: lim = tb + unit * aa_lcg_next(&s); break;          /* within one */
+    case 2: { /* a few ulps above tb */
+      lim = tb;
+      for (uint32_t j = (uint32_t)(8.0f * aa_lcg_next(&s)); j; --j) lim = nextafterf(lim, AA_INF_F);
+      break;
+    }
+    case 3: lim = tb + unit * 100.0f * aa_lcg_next(&s); break;
+    default: lim = 2.0f * tb + 1e-6f; break;
+    }
+    const float inv = 1.0f / unit;
+    const float b = naive ? expf((tb - lim) * inv) : aa_flight_bound(tb, lim, inv);
+    float p = b;
+    for (int j = 0; j < 128; ++j) {
+      if (j < 64) p = nextafterf(p, AA_INF_F);
+      else p = b + (1.0f - b) * aa_lcg_next(&s);
+      if (!(p > b) || p > 1.0f) continue;
+      if (!(aa_flight_pos(tb, p, unit) < lim)) ++bad;
+    }
+  }
+  return bad;
 }
 
 /* get_rng(frame, pixel, dims).state (rng.h:63-70) and the first k draws. */

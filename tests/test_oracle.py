@@ -210,6 +210,29 @@ def test_log_step_within_one_ulp_on_its_domain():
     assert _math(5, np.array([0.0, 1.0], np.float32)).tolist() == [-np.inf, 0.0]
 
 
+def test_log_unit_within_one_ulp():
+    # every 61st binary32 in (0, 1), subnormals included, against the double
+    # logarithm; ln 0 = -inf and ln 1 = 0 exactly
+    L = oracle.lib()
+    L.orc_log_unit_max_ulp.restype = C.c_double
+    L.orc_log_unit_max_ulp.argtypes = [C.c_uint32, C.POINTER(C.c_float)]
+    at = C.c_float(0.0)
+    err = L.orc_log_unit_max_ulp(61, C.byref(at))
+    assert 0.0 <= err <= 1.0, (err, at.value)
+
+
+def test_flight_bound_is_sound():
+    # products above aa_flight_bound place the position below lim (400k
+    # trials, 128 products each, near the bound and random); the same
+    # trials find products above the margin-free bound that do not, so
+    # they reach the boundary
+    L = oracle.lib()
+    L.orc_flight_bound_violations.restype = C.c_uint64
+    L.orc_flight_bound_violations.argtypes = [C.c_uint64, C.c_uint32, C.c_int]
+    assert L.orc_flight_bound_violations(400000, 12345, 0) == 0
+    assert L.orc_flight_bound_violations(400000, 12345, 1) > 0
+
+
 def test_expf_within_one_ulp():
     x = np.linspace(-87.0, 88.0, 400001).astype(np.float32)
     err = _ulp_err(_math(1, x), np.exp(x.astype(np.float64)))
