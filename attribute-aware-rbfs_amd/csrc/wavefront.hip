@@ -76,13 +76,13 @@ constexpr uint32_t kWaveFront = 8u * kWaveSplit; // breadth-first frontier entri
 #define AARBF_WAVE_SLICE 32
 #endif
 constexpr uint32_t kWaveSlice = AARBF_WAVE_SLICE; // nodes a lane opens between shared-bound updates
-// Empty-space steps a lane may take per iteration of the step loop: a wave
-// iterates as long as its slowest lane, and most steps are empty-space
-// steps, so taking several per iteration cuts the iterations (and the
-// per-iteration overhead every lane pays) roughly by this factor.  Each
-// such step is 25 VALU instructions (LCG, 1 - xi, logarithm, t update and
-// compares); 24 and 28 measured best against 16, 20 and 32 (1.7 % faster
-// than 16 at 1080p).
+// Empty-space steps a marching lane may take per iteration of the step
+// loop: a wave iterates as long as its slowest lane, and most steps are
+// empty-space steps, so taking several per iteration cuts the iterations
+// (and the per-iteration overhead every lane pays) roughly by this factor.
+// Delta tracking takes up to kFlightBlock per iteration, to its next
+// flight anchor (10 VALU instructions per step: LCG, 1 - xi, product,
+// compares, selects).
 #ifndef AARBF_FAST_STEPS
 #define AARBF_FAST_STEPS 24
 #endif
