@@ -230,6 +230,34 @@ def test_sparse_waves_leaving_early_is_exact(gpu_ctx, graph_rounds):
     _compare(acc0[px[:, 1], px[:, 0]], oacc[px[:, 1], px[:, 0]], "accum vs oracle")
 
 
+@pytest.mark.parametrize("cap", [16, 48])
+def test_flight_blocks_cut_by_the_step_budget_are_exact(gpu_ctx, cap):
+    """Delta tracking's empty-space steps run to the next flight anchor (32
+    steps); a per-launch step budget of 16 or 48 cuts such blocks in the
+    middle, so the rays leave with a partial product and its anchor saved
+    and go on in the next round.  The image equals the default budget's bit
+    for bit, and the oracle's on a pixel grid."""
+    W, H = 160, 96
+    xyzw, radii = random_scene(3000, seed=17, extent=1.0, r=(0.02, 0.07))
+    tfs = (scenes.ramp_table(256), scenes.constant_table(n=4), scenes.ramp_table(64))
+    sc, op, orad = _setup(gpu_ctx, xyzw, radii, tfs, 0.07)
+    cam = default_camera(W, H, look_from=(0.1, 0.4, 2.4))
+    kw = dict(light=(0.3, 0.2, 0.5))
+    try:
+        gpu_ctx.set_tuning(gpu_ctx.TUNE_STEP_CAP, cap)
+        img, acc = _gpu_render(gpu_ctx, sc, cam, W, H, 0.004, spp=2, frame=3, **kw)
+    finally:
+        gpu_ctx.set_tuning(gpu_ctx.TUNE_STEP_CAP, 8192)
+    img0, acc0 = _gpu_render(gpu_ctx, sc, cam, W, H, 0.004, spp=2, frame=3, **kw)
+    assert (acc0[..., :3] > 0).mean() > 0.1
+    _compare(acc, acc0, f"accum, step budget {cap} vs 8192")
+    _compare(img, img0, f"image, step budget {cap} vs 8192")
+    px = np.array([(x, y) for y in range(1, H, 7) for x in range(2, W, 9)], np.int32)
+    r = oracle.Render(op, orad, sc.aabb, *tfs, cam, W, H, 0.004, 0.07, frame_id=3, accum_id=1, **kw)
+    oimg, oacc, _ = r.run(pixels=px, spp=2, threads=8)
+    _compare(acc[px[:, 1], px[:, 0]], oacc[px[:, 1], px[:, 0]], "accum vs oracle")
+
+
 @pytest.mark.parametrize("blue_noise", [False, True])
 def test_walk_budget_handoff_is_exact(gpu_ctx, blue_noise):
     """Gather walks that open more nodes than the budget hand their ray to
