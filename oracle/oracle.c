@@ -1542,7 +1542,7 @@ ORC_EXPORT int orc_shadow_march(const orc_params *p, const float origin[3], floa
 
 /* Deterministic math exposed for its accuracy tests: which = 0 log, 1 exp,
  * 2 pow(x, y), 3 rbf(d = x, r = y), 4 lcg draw from state bits of x,
- * 5 tracker-step log. */
+ * 5 tracker-step log, 6 flight bound for anchor x, limit y, unit 1/2500. */
 ORC_EXPORT void orc_math(int which, const float *x, const float *y, float *out, uint64_t n) {
   for (uint64_t i = 0; i < n; ++i) {
     switch (which) {
@@ -1551,6 +1551,7 @@ ORC_EXPORT void orc_math(int which, const float *x, const float *y, float *out, 
     case 2: out[i] = aa_powf(x[i], y[i]); break;
     case 3: out[i] = aa_rbf(x[i], y[i]); break;
     case 5: out[i] = aa_log_step(x[i]); break;
+    case 6: out[i] = aa_flight_bound(x[i], y[i], 2500.0f); break;
     default: {
       uint32_t s = aa_f2u(x[i]);
       out[i] = aa_lcg_next(&s);

@@ -233,6 +233,22 @@ def test_flight_bound_is_sound():
     assert L.orc_flight_bound_violations(400000, 12345, 1) > 0
 
 
+def test_flight_bound_special_limits():
+    # lim = +inf: every step is in empty space (bound 0); a NaN lim: none is
+    # (NaN fails every compare); lim at or behind the anchor, or -inf (a
+    # segment end not yet gathered): the bound exceeds 1, so no product
+    # passes; ahead: at least exp(-(lim - tb) / unit), and above it by at
+    # most twice the margin (|c| + |lim| / unit) 2^-19
+    tb = np.array([0.5, 0.5, 0.5, 0.5, 0.5, 0.5], np.float32)
+    lim = np.array([np.inf, np.nan, 0.5, 0.4, -np.inf, 0.5 + 0.004], np.float32)
+    b = _math(6, tb, lim)
+    assert b[0] == 0.0 and np.isnan(b[1])
+    assert (b[2:5] > 1.0).all(), b[2:5]
+    c = (0.5 - float(lim[5])) * 2500.0
+    m = (abs(c) + float(lim[5]) * 2500.0) * 2.0 ** -18
+    assert np.exp(c) <= b[5] <= np.exp(c + m) < 1.0, (b[5], np.exp(c))
+
+
 def test_expf_within_one_ulp():
     x = np.linspace(-87.0, 88.0, 400001).astype(np.float32)
     err = _ulp_err(_math(1, x), np.exp(x.astype(np.float64)))
