@@ -1217,19 +1217,25 @@ __global__ __launch_bounds__(kSBlock, WPE) void wf_step(RenderArgs A, WFState S,
           t_exit = fminf(t_exit, to);
       }
     };
-    // the candidate list of the lane's ray into LDS, active set at L.t
+    // the candidate list of the lane's ray into LDS, active set at L.t; a
+    // pixel's head list already there (the previous sample of the same
+    // pixel ended on it) is not loaded again
+    uint32_t lds_head = 0xffffffffu; // pixel whose head list the lane's LDS holds
     auto setup = [&]() {
       n = (L.flags & F_DENSE) ? 0u : L.seg_n;
       const bool hd = (L.flags & F_HEAD) != 0;
-      const uint32_t *iidx = hd ? S.hidx : S.cidx;
-      const float *itin = hd ? S.htin : S.ctin, *itout = hd ? S.htout : S.ctout;
-      const uint8_t *ieo = hd ? S.heo : S.ceo;
-      const uint64_t row = hd ? L.pix : slot, stride = hd ? S.npix : S.nslots;
-      for (uint32_t k = 0; k < n; ++k) {
-        li[k * kSBlock] = iidx[k * stride + row];
-        ltin[k * kSBlock] = itin[k * stride + row];
-        ltout[k * kSBlock] = itout[k * stride + row];
-        leo[k * kSBlock] = ieo[k * stride + row];
+      if (!(hd && lds_head == L.pix)) {
+        const uint32_t *iidx = hd ? S.hidx : S.cidx;
+        const float *itin = hd ? S.htin : S.ctin, *itout = hd ? S.htout : S.ctout;
+        const uint8_t *ieo = hd ? S.heo : S.ceo;
+        const uint64_t row = hd ? L.pix : slot, stride = hd ? S.npix : S.nslots;
+        for (uint32_t k = 0; k < n; ++k) {
+          li[k * kSBlock] = iidx[k * stride + row];
+          ltin[k * kSBlock] = itin[k * stride + row];
+          ltout[k * kSBlock] = itout[k * stride + row];
+          leo[k * kSBlock] = ieo[k * stride + row];
+        }
+        lds_head = hd ? L.pix : 0xffffffffu;
       }
       act = 0;
       eptr = 0;
