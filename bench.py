@@ -231,7 +231,8 @@ def main() -> int:
             traffic = json.load(open(a.traffic)).get("bytes_per_launch")
         except Exception:
             traffic = None
-    # the step kernel's issue limit, from the profiled SQ counters of this workload
+    # the step kernel's VALU issue rate (fraction of the nominal peak), from
+    # the profiled SQ counters of this workload
     valu = None
     vpath = os.path.join(ROOT, "profiles", "render_valu.json")
     if os.path.exists(vpath):
